@@ -1,0 +1,212 @@
+/*
+ * libmpdok_hip.so — C-ABI of the MI355X (gfx950) dense-operator Krylov library.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The reference's native plugin symbols, with identical argument lists, so a
+ *     caller that loads mpdok.so / lu_ir.so through ctypes can point lib_path at
+ *     this library instead (reference bindings: mpdok_ops.py:101-141 for
+ *     lu_ir.so and mpdok_ops.py:361-382 for mpdok.so). They return void, report
+ *     through out-pointers, enqueue on the library stream (mpdok_set_stream,
+ *     default: the null stream) and synchronise it before returning, as the
+ *     reference's Python wrappers expect (mpdok_ops.py:200,234,311,498).
+ *     Matrices are FP64 column-major (Fortran order) device pointers.
+ *
+ *  2. New entry points (prefix mpdok_): explicit hipStream_t, int status
+ *     (0 = OK, nonzero = error; text from mpdok_last_error()), no
+ *     synchronisation unless stated. order: 0 = row-major, 1 = column-major.
+ *     dtype: 0 = FP64, 1 = FP32.
+ *
+ * All pointers are device pointers unless stated otherwise.
+ */
+#ifndef MPDOK_HIP_H
+#define MPDOK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* hipStream_t;
+
+/* ===== 1. Reference-compatible symbols ================================== */
+
+/* GMRES-IR: FP64->FP32 copy of A into A32 (caller-owned N*N FP32 scratch),
+ * inner FP32 GMRES(restart) with two-pass CGS and Givens (inner tolerance 1e-6),
+ * FP64 residual and update; *converged = 1 when relres < tol.
+ * Replaces mpdok.so::py_gmres_ir (mpdok_ops.py:362-373, called at :487-497). */
+void py_gmres_ir(void* A64, void* b64, void* x64, void* A32, int N, double tol,
+                 int maxiter_outer, int restart, int* converged);
+/* N*N FP64 managed allocation (hipMallocManaged); one live buffer per library.
+ * Replaces mpdok.so::py_alloc_managed / py_free_managed (mpdok_ops.py:375-382). */
+void py_alloc_managed(int N, void** ptr_out);
+void py_free_managed(void);
+
+/* LU-IR in one call: FP32 LU with partial pivoting, FP32 triangular solves,
+ * FP64 residual and update. Replaces lu_ir.so::py_lu_ir (mpdok_ops.py:102-111). */
+void py_lu_ir(void* A64, void* b64, void* x64, int N, double tol, int maxiter_outer,
+              int* converged);
+/* Factor once (*info = 0 on success, k > 0 for a zero pivot in column k).
+ * Replaces lu_ir.so::py_lu_factor (mpdok_ops.py:122-127). */
+void py_lu_factor(void* A64, int N, int* info);
+/* Solve with the stored factors; *converged = -1 if no factorization exists.
+ * Replaces lu_ir.so::py_lu_solve_factored (mpdok_ops.py:129-138). */
+void py_lu_solve_factored(void* A64, void* b64, void* x64, int N, double tol,
+                          int maxiter_outer, int* converged);
+/* Replaces lu_ir.so::py_lu_alloc_managed / py_lu_free_managed / py_lu_free_factored
+ * (mpdok_ops.py:113-120, 140-141). */
+void py_lu_alloc_managed(int N, void** ptr_out);
+void py_lu_free_managed(void);
+void py_lu_free_factored(void);
+
+/* ===== 2. New entry points =============================================== */
+
+/* Library services. */
+const char* mpdok_last_error(void);
+/* Stream used by the reference-compatible symbols (NULL = null stream). */
+int mpdok_set_stream(hipStream_t s);
+size_t mpdok_reduce_workspace_bytes(int n);
+size_t mpdok_gemv_workspace_bytes(int M, int N, int dtype, int order);
+size_t mpdok_residual_workspace_bytes(int N, int order);
+size_t mpdok_arnoldi_workspace_bytes(int n, int max_k);
+
+/* out[0] = <x, y> accumulated in FP64 (deterministic two-pass reduction);
+ * work: mpdok_reduce_workspace_bytes(n). Replaces the cp.dot / cp.linalg.norm
+ * reductions of dense_krylov.py:156,167,192,195,245,272,295. */
+int mpdok_dot_f32(hipStream_t s, const float* x, const float* y, int n, double* out, void* work);
+int mpdok_dot_f64(hipStream_t s, const double* x, const double* y, int n, double* out,
+                  void* work);
+/* Strided 2-D copy with dtype conversion and optional order change.
+ * Replaces the FP64->FP32 casts of dense_krylov.py:47 and mpdok_ops.py:366. */
+int mpdok_cast_matrix(hipStream_t s, const void* in, int in_dtype, int64_t ldi, int in_order,
+                      void* out, int out_dtype, int64_t ldo, int out_order, int M, int N);
+/* out = alpha*x + beta*y (y may be NULL), FP64 arithmetic, one rounding.
+ * Replaces the CuPy vector expressions of dense_krylov.py:165,193-197,271,412. */
+int mpdok_axpby(hipStream_t s, int n, double alpha, const void* x, int xdt, double beta,
+                const void* y, int ydt, void* out, int odt);
+
+/* y = A x. Column-major A needs a workspace of mpdok_gemv_workspace_bytes.
+ * skip: optional device flag; when *skip != 0 the kernels return immediately.
+ * Replaces cp.matmul(A_fp32, x) (dense_krylov.py:61-63) and the tile SGEMV of
+ * mpdok_ooc.py:158-201. */
+int mpdok_gemv_f32(hipStream_t s, const float* A, int64_t lda, int order, int M, int N,
+                   const float* x, float* y, void* work, size_t work_bytes, const int* skip);
+int mpdok_gemv_f64(hipStream_t s, const double* A, int64_t lda, int order, int M, int N,
+                   const double* x, double* y, void* work, size_t work_bytes, const int* skip);
+/* r = b - A x in FP64; mode 0 = FP64 accumulation (as cuBLAS Dgemv), mode 1 =
+ * Dot2 compensated accumulation rounded once. Replaces dense_krylov.py:76-79,
+ * :391, :416 and the natives' cublasDgemv residual (benchmark_mpdok.ipynb:991). */
+int mpdok_residual_f64(hipStream_t s, const double* A, int64_t lda, int order, int N,
+                       const double* x, const double* b, double* r, int mode, void* work,
+                       size_t work_bytes);
+
+/* Kernel-matrix tile: rows [row0,row0+rows) x cols [col0,col0+cols) of
+ * k(c_i, c_j) + reg*delta_ij from coords (N x D, row-major FP64). kind 0 =
+ * Gaussian exp(-p d2), 1 = Matern-3/2 with length scale p. reg_mode 0 = nugget in
+ * FP64 before the output cast (rbf_kernel.py:219-224), 1 = nugget after the cast
+ * (mpdok_ooc.py:121-122). Replaces rbf_kernel.py:215-224 and mpdok_ooc.py:39-46. */
+int mpdok_rbf_assemble(hipStream_t s, const double* coords, int N, int D, int kind, double p,
+                       double reg, int reg_mode, int row0, int rows, int col0, int cols,
+                       void* out, int out_dtype, int64_t ld, int order);
+/* out[0] = mean clamped d2 over the first n points (rbf_kernel.py:57-68). */
+int mpdok_rbf_mean_d2(hipStream_t s, const double* coords, int n, int D, double* out);
+
+/* One Arnoldi orthogonalisation of w against V_0..V_k (vectors of length n,
+ * stride ldv): passes = 1 (dense_krylov.py:289-303) or 2 (DGKS,
+ * mpdok_ooc.py:263-273). Adds the coefficients into Hcol[0..k], writes
+ * Hcol[k+1] = ||w|| and V_{k+1} = w/||w||, or raises the breakdown flag when
+ * ||w|| <= thresh. ctl (device int[4]): ctl[0] stop, ctl[1] steps done,
+ * ctl[2] breakdown. */
+int mpdok_arnoldi_orth_f32(hipStream_t s, float* V, int64_t ldv, int k, float* w, int n,
+                           double* Hcol, int passes, double thresh, int stop_now, int* ctl,
+                           void* work, size_t work_bytes);
+int mpdok_arnoldi_orth_f64(hipStream_t s, double* V, int64_t ldv, int k, double* w, int n,
+                           double* Hcol, int passes, double thresh, int stop_now, int* ctl,
+                           void* work, size_t work_bytes);
+/* Device Givens update of Hessenberg column k (mpdok_ooc.py:284-301); sets
+ * ctl[0] when |g[k+1]|/beta < tol or a breakdown was flagged. */
+int mpdok_givens_step(hipStream_t s, double* H, int ldh, int k, double* R, double* cs,
+                      double* sn, double* g, double beta, double tol, int* ctl);
+/* x (+)= sum_{i<m} y_i V_i (dense_krylov.py:311-312, mpdok_ooc.py:310). */
+int mpdok_basis_update_f32(hipStream_t s, const float* V, int64_t ldv, int m, const double* y,
+                           float* x, int n, int overwrite);
+int mpdok_basis_update_f32_into_f64(hipStream_t s, const float* V, int64_t ldv, int m,
+                                    const double* y, double* x, int n, int overwrite);
+int mpdok_basis_update_f64(hipStream_t s, const double* V, int64_t ldv, int m, const double* y,
+                           double* x, int n, int overwrite);
+/* out = in / sqrt(nrm2[0]) (dense_krylov.py:278, mpdok_ooc.py:249). */
+int mpdok_scale_f64_to_f32(hipStream_t s, const double* in, float* out, int n,
+                           const double* nrm2);
+int mpdok_scale_f32(hipStream_t s, const float* in, float* out, int n, const double* nrm2);
+int mpdok_scale_f64(hipStream_t s, const double* in, double* out, int n, const double* nrm2);
+/* CG updates with device scalars (dense_krylov.py:191-198): alpha = rs_old/pAp,
+ * x += alpha p, r -= alpha Ap, rs_new = r.r, p = r + (rs_new/rs_old) p.
+ * work: 4096 doubles. */
+int mpdok_cg_step_updates(hipStream_t s, float* x, float* r, float* p, const float* Ap, int n,
+                          const double* rs_old, const double* pAp, double* rs_new, void* work);
+
+/* ---- LU factorisation ------------------------------------------------------
+ * Opaque handle replacing the global factor slot of lu_ir.so
+ * (mpdok_ops.py:122-141,245-247). precision: 0 = FP64, 1 = FP32, 2 = BF16
+ * operands / FP32 accumulation, 3 = FP16 operands / FP32 accumulation,
+ * 4 = int8-sliced trailing updates with `slices` digits and FP64 storage.
+ * pivot: 0 = none (SPD / diagonally dominant), 1 = partial (getrf).
+ * nb: outer block (multiple of 64; 0 = 512). */
+int mpdok_lu_create(int N, int precision, int pivot, int nb, int slices, void** handle);
+/* Copies A (FP64, row- or column-major) into the handle and factors it;
+ * synchronises s; *info = 0 or the 1-based column of the first zero pivot.
+ * Replaces lu_ir.so::py_lu_factor's Sgetrf (benchmark_mpdok.ipynb:989). */
+int mpdok_lu_factor(void* handle, hipStream_t s, const double* A, int64_t lda, int order,
+                    int* info);
+/* x = inv(P L U) b (FP64 vectors; x may alias b). Replaces Sgetrs
+ * (benchmark_mpdok.ipynb:990,992). */
+int mpdok_lu_solve(void* handle, hipStream_t s, const double* b, double* x);
+int mpdok_lu_destroy(void* handle);
+/* HIP-event timing of the trailing-update launches of later factorisations;
+ * mpdok_lu_stats returns the summed time (ms), the algorithmic operations
+ * (int8 operations for the sliced variant, flops otherwise) and the launch
+ * count of the last factorisation. No reference counterpart: measurement. */
+int mpdok_lu_set_profiling(void* handle, int on);
+int mpdok_lu_stats(void* handle, double* trailing_ms, double* trailing_ops, int* launches);
+/* Device pointers to the packed factors (column-major, ld N) and the row
+ * permutation ((P b)[i] = b[perm[i]], NULL without pivoting). */
+int mpdok_lu_factors(void* handle, void** LU, int** perm, int* elem_size);
+/* C = alpha A B + beta C on the matrix cores (column-major); dtype 0 FP64,
+ * 1 FP32; round 0/1/2 = none/BF16/FP16 operands (FP32 only). The trailing
+ * update of the factorisation (K13 in place of cuSOLVER's internal GEMM). */
+int mpdok_gemm(hipStream_t s, int dtype, int m, int n, int k, double alpha, const void* A,
+               int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc,
+               int round);
+/* C -= A B with both operands cut into `slices` int8 digits per row / column
+ * (int8 matrix cores, exact int32 sums, FP64 fold). work: workspace bytes. */
+size_t mpdok_ozaki_workspace_bytes(int m, int n, int K, int slices);
+int mpdok_ozaki_gemm_sub(hipStream_t s, int slices, int m, int n, int K, const double* A,
+                         int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+                         void* work);
+
+/* ---- refinement drivers -----------------------------------------------------
+ * GMRES-IR: FP64 outer refinement (residual mode 0 FP64 / 1 Dot2) around one
+ * GMRES(restart) cycle per outer step with body 0 = FP32 (A32) / 1 = FP64
+ * (A64), CGS `passes`, inner stop |g|/beta < inner_tol, optional right
+ * preconditioner lu (NULL = none). hist: host array of hist_cap doubles
+ * receiving relres per outer step plus the final check. Unpreconditioned FP32
+ * body with passes = 2 is py_gmres_ir (mpdok_ops.py:362-373); the LU-
+ * preconditioned form has no reference counterpart. */
+int mpdok_gmres_ir(hipStream_t s, const double* A64, int64_t lda64, int order64, const float* A32,
+                   int64_t lda32, int order32, void* lu, int N, const double* b, double* x,
+                   double tol, int maxiter_outer, int restart, double inner_tol, int passes,
+                   int resid_mode, int body, int* converged, double* hist, int hist_cap,
+                   int* n_hist, int* inner_total);
+/* LU-IR with a factorisation handle: x = 0; repeat { r = b - A x (FP64 /
+ * Dot2); stop if relres < tol; x += inv(PLU) r }. Replaces
+ * lu_ir.so::py_lu_solve_factored's loop (mpdok_ops.py:225-243). */
+int mpdok_lu_ir(hipStream_t s, const double* A64, int64_t lda, int order, void* lu, int N,
+                const double* b, double* x, double tol, int maxiter_outer, int resid_mode,
+                int* converged, double* hist, int hist_cap, int* n_hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPDOK_HIP_H */

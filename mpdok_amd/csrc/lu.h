@@ -1,0 +1,55 @@
+#pragma once
+#include "common.h"
+
+namespace mpdok {
+
+// lu_kernels.hip
+template <typename T>
+int gemm(hipStream_t s, int m, int n, int k, T alpha, const T* A, int64_t lda, const T* B,
+         int64_t ldb, T beta, T* C, int64_t ldc, int round);
+template <typename T>
+int diag_factor(hipStream_t s, T* A, int64_t lda, int nb, T* Linv, T* Uinv, int ldi, int* info,
+                int col0, int factor);
+template <typename TI, typename TO>
+int copy2d(hipStream_t s, const TI* src, int64_t lds, TO* dst, int64_t ldd, int m, int n);
+template <typename T>
+int laswp(hipStream_t s, T* A, int64_t lda, int c0, int c1, const int* ipiv, int r0, int r1);
+template <typename T>
+int panel_gepp(hipStream_t s, T* A, int64_t lda, int N, int j0, int w, int* ipiv, int* info,
+               void* work);
+size_t panel_gepp_workspace_bytes(int N);
+template <typename T, typename TV>
+int trsv_blocked(hipStream_t s, const T* LU, int64_t ld, int N, const T* Linv_all,
+                 const T* Uinv_all, TV* v);
+template <typename T>
+int permute(hipStream_t s, const T* in, T* out, const int* perm, int n);
+int ipiv_to_perm(hipStream_t s, const int* ipiv, int* perm, int n);
+template <typename T>
+int gemm_batched(hipStream_t s, int m, int n, int k, T alpha, const T* A, int64_t lda,
+                 int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta, T* C, int64_t ldc,
+                 int64_t sC, int batch, int round);
+constexpr int TRSV_BLOCK = 512;  // diagonal block of the inverse-based triangular solves
+template <typename T, typename TV>
+int trsv_block_inverse(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
+                       const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc);
+template <typename T>
+int seed_block_inverse(hipStream_t s, const T* inv64, T* binv, int nbig);
+
+// ozaki.hip
+size_t ozaki_workspace_bytes(int m, int n, int K, int S);
+int ozaki_gemm(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
+               const double* B, int64_t ldb, double* C, int64_t ldc, void* work);
+
+// lu.hip
+enum LUPrecision : int { LU_FP64 = 0, LU_FP32 = 1, LU_BF16 = 2, LU_FP16 = 3, LU_OZAKI = 4 };
+enum LUPivot : int { PIV_NONE = 0, PIV_PARTIAL = 1 };
+
+struct LUHandle;
+int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** out);
+int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int order, int* info);
+// x = inv(P L U) b  (b, x FP64; x may alias b)
+int lu_solve(LUHandle* h, hipStream_t s, const double* b, double* x);
+void lu_destroy(LUHandle* h);
+int lu_n(const LUHandle* h);
+
+}  // namespace mpdok

@@ -1,0 +1,389 @@
+// Blocked right-looking LU factorisation and solve behind an opaque handle.
+//
+// Storage: the factors live in a handle-owned N x N column-major array (FP64
+// for the FP64 and int8-sliced variants, FP32 for the FP32/BF16/FP16
+// variants), L unit-lower and U packed LAPACK-style, plus the inverses of all
+// 64 x 64 diagonal blocks of L and U for the blocked triangular solves.
+//
+// Algorithm (outer block nb, inner block 64):
+//   for each outer panel [k, k+kb):
+//     for each 64-column sub-panel j:
+//       no pivoting : factor the 64 x 64 diagonal block in LDS; L21 = A21 inv(U11)
+//       partial     : column-by-column pivot search / swap / eliminate over the
+//                     full height, then apply the interchanges to the other columns
+//       U12 of the rest of the panel = inv(L11) A12; panel update by GEMM
+//     U12 for the columns right of the panel (blocked forward substitution)
+//     trailing update A22 -= L21 U12 (the O(N^3) part):
+//       FP64  : FP64 matrix cores        FP32 : FP32 matrix cores
+//       BF16/FP16 : operands rounded to BF16/FP16, FP32 accumulation
+//       OZAKI : int8 matrix cores on S sliced digits, FP64 storage
+//
+// Reference: the sgetrf/sgetrs of lu_ir.so (mpdok_ops.py:69-320,
+// benchmark_mpdok.ipynb:983-996); pivoting semantics follow LAPACK getrf
+// (ipiv = sequential interchanges).
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include "common.h"
+#include "gemv.h"
+#include "lu.h"
+
+namespace mpdok {
+
+struct LUHandle {
+  int N = 0, precision = 0, pivot = 0, nb = 512, slices = 6;
+  bool factored = false;
+  size_t es = 8;
+  void* LU = nullptr;
+  void* Linv = nullptr;
+  void* Uinv = nullptr;
+  void* W = nullptr;
+  int* ipiv = nullptr;
+  int* perm = nullptr;
+  int* d_info = nullptr;
+  void* piv_work = nullptr;
+  void* oz_work = nullptr;
+  void* v1 = nullptr;
+  void* v2 = nullptr;
+  // inverse-based triangular solves (N a multiple of TRSV_BLOCK)
+  bool fast_solve = false;
+  void* Lbinv = nullptr;
+  void* Ubinv = nullptr;
+  void* v3 = nullptr;
+  void* acc = nullptr;
+  // optional HIP-event timing of the trailing-update launches
+  bool profile = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<double> ev_ops;  // algorithmic operations of each timed launch
+  double trailing_ms = 0.0, trailing_ops = 0.0;
+  int trailing_launches = 0;
+};
+
+int lu_n(const LUHandle* h) { return h ? h->N : 0; }
+
+static bool fp64_storage(int precision) { return precision == LU_FP64 || precision == LU_OZAKI; }
+
+void lu_destroy(LUHandle* h) {
+  if (!h) return;
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  void* ptrs[] = {h->LU, h->Linv, h->Uinv, h->W, h->ipiv, h->perm, h->d_info,
+                  h->piv_work, h->oz_work, h->v1, h->v2, h->Lbinv, h->Ubinv, h->v3, h->acc};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete h;
+}
+
+int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** out) {
+  MPDOK_REQUIRE(out != nullptr, "lu_create: null out");
+  *out = nullptr;
+  MPDOK_REQUIRE(N >= 1, "lu_create: N must be >= 1");
+  MPDOK_REQUIRE(precision >= LU_FP64 && precision <= LU_OZAKI, "lu_create: bad precision %d",
+                precision);
+  MPDOK_REQUIRE(pivot == PIV_NONE || pivot == PIV_PARTIAL, "lu_create: bad pivot %d", pivot);
+  if (nb <= 0) nb = 512;
+  MPDOK_REQUIRE(nb % 64 == 0, "lu_create: nb must be a multiple of 64");
+  MPDOK_REQUIRE(precision != LU_OZAKI || (slices >= 3 && slices <= 7),
+                "lu_create: slices must be in [3, 7]");
+  LUHandle* h = new LUHandle();
+  h->N = N;
+  h->precision = precision;
+  h->pivot = pivot;
+  h->nb = nb;
+  h->slices = slices;
+  h->es = fp64_storage(precision) ? 8 : 4;
+  const size_t nblk = (N + 63) / 64;
+  const size_t wel = (size_t)N * (nb > 128 ? nb : 128);
+  int st = OK;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (st != OK) return;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      set_error("lu_create: hipMalloc of %zu bytes failed", bytes);
+      st = ENOMEM_;
+    }
+  };
+  alloc(&h->LU, (size_t)N * N * h->es);
+  alloc(&h->Linv, nblk * 64 * 64 * h->es);
+  alloc(&h->Uinv, nblk * 64 * 64 * h->es);
+  alloc(&h->W, wel * h->es);
+  alloc((void**)&h->ipiv, (size_t)N * sizeof(int));
+  alloc((void**)&h->perm, (size_t)N * sizeof(int));
+  alloc((void**)&h->d_info, 64);
+  alloc(&h->piv_work, panel_gepp_workspace_bytes(N));
+  alloc(&h->v1, (size_t)N * 8);
+  alloc(&h->v2, (size_t)N * 8);
+  if (precision == LU_OZAKI) alloc(&h->oz_work, ozaki_workspace_bytes(N, N, nb, slices));
+  h->fast_solve = (N % TRSV_BLOCK) == 0;
+  if (h->fast_solve) {
+    alloc(&h->Lbinv, (size_t)N * TRSV_BLOCK * h->es);
+    alloc(&h->Ubinv, (size_t)N * TRSV_BLOCK * h->es);
+    alloc(&h->v3, (size_t)N * 8);
+    alloc(&h->acc, (size_t)N * 8 * 8);
+  }
+  if (st == OK && hipMemset(h->piv_work, 0, panel_gepp_workspace_bytes(N)) != hipSuccess) {
+    set_error("lu_create: hipMemset failed");
+    st = EHIP;
+  }
+  if (st != OK) {
+    lu_destroy(h);
+    return st;
+  }
+  *out = h;
+  return OK;
+}
+
+// Inverses of the TRSV_BLOCK x TRSV_BLOCK diagonal blocks of L and U by
+// recursive doubling from the 64 x 64 inverses:
+//   inv([A 0; C D]) = [inv(A) 0; -inv(D) C inv(A)  inv(D)]
+//   inv([A B; 0 D]) = [inv(A) -inv(A) B inv(D); 0  inv(D)]
+// batched over all big blocks (uniform strides along the diagonal).
+template <typename T>
+static int build_block_inverses(LUHandle* h, hipStream_t s) {
+  const int N = h->N, TBk = TRSV_BLOCK;
+  const int nbig = N / TBk;
+  const T* LU = static_cast<const T*>(h->LU);
+  T* Lb = static_cast<T*>(h->Lbinv);
+  T* Ub = static_cast<T*>(h->Ubinv);
+  T* W = static_cast<T*>(h->W);
+  const int64_t ld = N;
+  const int64_t sLU = (int64_t)TBk * (ld + 1), sB = (int64_t)TBk * TBk;
+  MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Linv), Lb, nbig));
+  MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Uinv), Ub, nbig));
+  for (int hh = 64; hh < TBk; hh *= 2) {
+    const int64_t sW = (int64_t)hh * hh;
+    for (int off = 0; off < TBk; off += 2 * hh) {
+      const int64_t dA = (int64_t)off * (TBk + 1);         // inv(A) inside the big block
+      const int64_t dD = (int64_t)(off + hh) * (TBk + 1);  // inv(D)
+      // L: T = C inv(A); X21 = -inv(D) T
+      const T* C = LU + (off + hh) + (int64_t)off * ld;
+      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), C, ld, sLU, Lb + dA, TBk, sB, T(0), W, hh, sW,
+                                nbig, 0));
+      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Lb + dD, TBk, sB, W, hh, sW, T(0),
+                                Lb + (off + hh) + (int64_t)off * TBk, TBk, sB, nbig, 0));
+      // U: T = B inv(D); X12 = -inv(A) T
+      const T* Bm = LU + off + (int64_t)(off + hh) * ld;
+      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), Bm, ld, sLU, Ub + dD, TBk, sB, T(0), W, hh, sW,
+                                nbig, 0));
+      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Ub + dA, TBk, sB, W, hh, sW, T(0),
+                                Ub + off + (int64_t)(off + hh) * TBk, TBk, sB, nbig, 0));
+    }
+  }
+  return OK;
+}
+
+template <typename T>
+static int factor_impl(LUHandle* h, hipStream_t s) {
+  const int N = h->N, nb = h->nb;
+  T* A = static_cast<T*>(h->LU);
+  T* Linv = static_cast<T*>(h->Linv);
+  T* Uinv = static_cast<T*>(h->Uinv);
+  T* W = static_cast<T*>(h->W);
+  const int64_t ld = N;
+  const int round = h->precision == LU_BF16 ? 1 : (h->precision == LU_FP16 ? 2 : 0);
+  auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
+  for (int k = 0; k < N; k += nb) {
+    const int kb = nb < N - k ? nb : N - k;
+    for (int j = k; j < k + kb; j += 64) {
+      const int jb = 64 < k + kb - j ? 64 : k + kb - j;
+      T* Li = Linv + (size_t)(j / 64) * 4096;
+      T* Ui = Uinv + (size_t)(j / 64) * 4096;
+      const int m2 = N - j - jb;
+      if (h->pivot == PIV_PARTIAL) {
+        MPDOK_TRY(panel_gepp<T>(s, A, ld, N, j, jb, h->ipiv, h->d_info, h->piv_work));
+        MPDOK_TRY(laswp<T>(s, A, ld, 0, j, h->ipiv, j, j + jb));
+        MPDOK_TRY(laswp<T>(s, A, ld, j + jb, N, h->ipiv, j, j + jb));
+        MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 0));
+      } else {
+        MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 1));
+        if (m2 > 0) {
+          MPDOK_TRY(gemm<T>(s, m2, jb, jb, T(1), at(j + jb, j), ld, Ui, 64, T(0), W, m2, 0));
+          MPDOK_TRY((copy2d<T, T>(s, W, m2, at(j + jb, j), ld, m2, jb)));
+        }
+      }
+      const int n2 = k + kb - j - jb;
+      if (n2 > 0) {
+        MPDOK_TRY(gemm<T>(s, jb, n2, jb, T(1), Li, 64, at(j, j + jb), ld, T(0), W, jb, 0));
+        MPDOK_TRY((copy2d<T, T>(s, W, jb, at(j, j + jb), ld, jb, n2)));
+        if (m2 > 0)
+          MPDOK_TRY(gemm<T>(s, m2, n2, jb, T(-1), at(j + jb, j), ld, at(j, j + jb), ld, T(1),
+                            at(j + jb, j + jb), ld, 0));
+      }
+    }
+    const int n3 = N - k - kb;
+    if (n3 <= 0) continue;
+    for (int j = k; j < k + kb; j += 64) {
+      const int jb = 64 < k + kb - j ? 64 : k + kb - j;
+      T* Li = Linv + (size_t)(j / 64) * 4096;
+      MPDOK_TRY(gemm<T>(s, jb, n3, jb, T(1), Li, 64, at(j, k + kb), ld, T(0), W, jb, 0));
+      MPDOK_TRY((copy2d<T, T>(s, W, jb, at(j, k + kb), ld, jb, n3)));
+      const int mrem = k + kb - j - jb;
+      if (mrem > 0)
+        MPDOK_TRY(gemm<T>(s, mrem, n3, jb, T(-1), at(j + jb, j), ld, at(j, k + kb), ld, T(1),
+                          at(j + jb, k + kb), ld, 0));
+    }
+    const int m3 = N - k - kb;
+    const size_t nev = h->ev_ops.size();
+    if (h->profile) {
+      // int8-sliced: S(S+1)/2 int8 products of 2 m n k operations each
+      const double pairs = h->precision == LU_OZAKI ? h->slices * (h->slices + 1) / 2.0 : 1.0;
+      h->ev_ops.push_back(2.0 * m3 * n3 * kb * pairs);
+      while (h->ev.size() < 2 * h->ev_ops.size()) {
+        hipEvent_t e;
+        MPDOK_HIP(hipEventCreate(&e));
+        h->ev.push_back(e);
+      }
+      MPDOK_HIP(hipEventRecord(h->ev[2 * nev], s));
+    }
+    if (h->precision == LU_OZAKI) {
+      MPDOK_TRY(ozaki_gemm(s, h->slices, m3, n3, kb, reinterpret_cast<const double*>(at(k + kb, k)),
+                           ld, reinterpret_cast<const double*>(at(k, k + kb)), ld,
+                           reinterpret_cast<double*>(at(k + kb, k + kb)), ld, h->oz_work));
+    } else {
+      MPDOK_TRY(gemm<T>(s, m3, n3, kb, T(-1), at(k + kb, k), ld, at(k, k + kb), ld, T(1),
+                        at(k + kb, k + kb), ld, round));
+    }
+    if (h->profile) MPDOK_HIP(hipEventRecord(h->ev[2 * nev + 1], s));
+  }
+  if (h->pivot == PIV_PARTIAL) MPDOK_TRY(ipiv_to_perm(s, h->ipiv, h->perm, N));
+  if (h->fast_solve) MPDOK_TRY(build_block_inverses<T>(h, s));
+  return OK;
+}
+
+int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int order, int* info) {
+  MPDOK_REQUIRE(h && A, "lu_factor: null handle or matrix");
+  MPDOK_REQUIRE(lda >= h->N, "lu_factor: lda < N");
+  h->factored = false;
+  const int N = h->N;
+  h->ev_ops.clear();
+  MPDOK_HIP(hipMemsetAsync(h->d_info, 0, sizeof(int), s));
+  // copy (and convert / transpose) A into the handle's column-major storage
+  MPDOK_TRY(cast_matrix(s, A, F64, lda, order, h->LU, fp64_storage(h->precision) ? F64 : F32, N,
+                        COL_MAJOR, N, N));
+  if (fp64_storage(h->precision))
+    MPDOK_TRY(factor_impl<double>(h, s));
+  else
+    MPDOK_TRY(factor_impl<float>(h, s));
+  int hinfo = 0;
+  MPDOK_HIP(hipMemcpyAsync(&hinfo, h->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+  MPDOK_HIP(hipStreamSynchronize(s));
+  if (info) *info = hinfo;
+  h->factored = (hinfo == 0);
+  if (h->profile) {
+    h->trailing_ms = 0.0;
+    h->trailing_ops = 0.0;
+    h->trailing_launches = (int)h->ev_ops.size();
+    for (size_t i = 0; i < h->ev_ops.size(); ++i) {
+      float ms = 0.f;
+      MPDOK_HIP(hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]));
+      h->trailing_ms += ms;
+      h->trailing_ops += h->ev_ops[i];
+    }
+  }
+  return OK;
+}
+
+template <typename T, typename TV>
+static int solve_impl(LUHandle* h, hipStream_t s, const double* b, double* x) {
+  const int N = h->N;
+  TV* v = static_cast<TV*>(h->v1);
+  TV* v2 = static_cast<TV*>(h->v2);
+  if (sizeof(TV) == 8) {
+    MPDOK_TRY((copy2d<double, double>(s, b, N, reinterpret_cast<double*>(v), N, N, 1)));
+  } else {
+    MPDOK_TRY((copy2d<double, float>(s, b, N, reinterpret_cast<float*>(v), N, N, 1)));
+  }
+  if (h->pivot == PIV_PARTIAL) {
+    MPDOK_TRY(permute<TV>(s, v, v2, h->perm, N));
+    TV* t = v;
+    v = v2;
+    v2 = t;
+  }
+  if (h->fast_solve) {
+    // v: permuted right-hand side; result into v2 (v3 holds the forward result)
+    MPDOK_TRY((trsv_block_inverse<T, TV>(s, static_cast<const T*>(h->LU), N, N,
+                                         static_cast<const T*>(h->Lbinv),
+                                         static_cast<const T*>(h->Ubinv), v, v2,
+                                         static_cast<TV*>(h->v3), static_cast<TV*>(h->acc))));
+    v = v2;
+  } else {
+    MPDOK_TRY((trsv_blocked<T, TV>(s, static_cast<const T*>(h->LU), N, N,
+                                   static_cast<const T*>(h->Linv),
+                                   static_cast<const T*>(h->Uinv), v)));
+  }
+  if (sizeof(TV) == 8) {
+    MPDOK_TRY((copy2d<double, double>(s, reinterpret_cast<double*>(v), N, x, N, N, 1)));
+  } else {
+    MPDOK_TRY((copy2d<float, double>(s, reinterpret_cast<float*>(v), N, x, N, N, 1)));
+  }
+  return OK;
+}
+
+int lu_solve(LUHandle* h, hipStream_t s, const double* b, double* x) {
+  MPDOK_REQUIRE(h && b && x, "lu_solve: null argument");
+  if (!h->factored) {
+    set_error("lu_solve: no factorization available");
+    return ENOFACTOR;
+  }
+  if (fp64_storage(h->precision)) return solve_impl<double, double>(h, s, b, x);
+  return solve_impl<float, float>(h, s, b, x);
+}
+
+}  // namespace mpdok
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+MPDOK_API int mpdok_lu_create(int N, int precision, int pivot, int nb, int slices, void** out) {
+  return mpdok::lu_create(N, precision, pivot, nb, slices, reinterpret_cast<mpdok::LUHandle**>(out));
+}
+MPDOK_API int mpdok_lu_factor(void* h, hipStream_t s, const double* A, int64_t lda, int order,
+                              int* info) {
+  return mpdok::lu_factor(static_cast<mpdok::LUHandle*>(h), s, A, lda, order, info);
+}
+MPDOK_API int mpdok_lu_solve(void* h, hipStream_t s, const double* b, double* x) {
+  return mpdok::lu_solve(static_cast<mpdok::LUHandle*>(h), s, b, x);
+}
+MPDOK_API int mpdok_lu_destroy(void* h) {
+  mpdok::lu_destroy(static_cast<mpdok::LUHandle*>(h));
+  return 0;
+}
+// Device pointers to the packed factors (column-major, ld = N, element size
+// 8 for FP64/int8-sliced and 4 otherwise) and the row permutation (NULL
+// without pivoting), for tests and inspection.
+MPDOK_API int mpdok_lu_factors(void* hv, void** LU, int** perm, int* elem_size) {
+  auto* h = static_cast<mpdok::LUHandle*>(hv);
+  MPDOK_REQUIRE(h != nullptr, "lu_factors: null handle");
+  if (LU) *LU = h->LU;
+  if (perm) *perm = h->pivot == mpdok::PIV_PARTIAL ? h->perm : nullptr;
+  if (elem_size) *elem_size = (int)h->es;
+  return 0;
+}
+// HIP-event timing of the trailing-update launches of the next factorisations.
+MPDOK_API int mpdok_lu_set_profiling(void* hv, int on) {
+  auto* h = static_cast<mpdok::LUHandle*>(hv);
+  MPDOK_REQUIRE(h != nullptr, "lu_set_profiling: null handle");
+  h->profile = on != 0;
+  return 0;
+}
+// Totals of the last profiled factorisation: summed event time of the trailing
+// updates (ms), their algorithmic operations (int8 ops for the sliced variant,
+// flops otherwise) and the number of timed launches.
+MPDOK_API int mpdok_lu_stats(void* hv, double* trailing_ms, double* trailing_ops, int* launches) {
+  auto* h = static_cast<mpdok::LUHandle*>(hv);
+  MPDOK_REQUIRE(h != nullptr, "lu_stats: null handle");
+  if (trailing_ms) *trailing_ms = h->trailing_ms;
+  if (trailing_ops) *trailing_ops = h->trailing_ops;
+  if (launches) *launches = h->trailing_launches;
+  return 0;
+}
+// Plain GEMM on the matrix cores (C = alpha A B + beta C, column-major), for
+// parity tests of the trailing-update kernel. dtype 0 = FP64, 1 = FP32;
+// round 0/1/2 = none/BF16/FP16 operand rounding (FP32 only).
+MPDOK_API int mpdok_gemm(hipStream_t s, int dtype, int m, int n, int k, double alpha,
+                         const void* A, int64_t lda, const void* B, int64_t ldb, double beta,
+                         void* C, int64_t ldc, int round) {
+  if (dtype == mpdok::F64)
+    return mpdok::gemm<double>(s, m, n, k, alpha, (const double*)A, lda, (const double*)B, ldb,
+                               beta, (double*)C, ldc, 0);
+  return mpdok::gemm<float>(s, m, n, k, (float)alpha, (const float*)A, lda, (const float*)B, ldb,
+                            (float)beta, (float*)C, ldc, round);
+}

@@ -1,0 +1,634 @@
+// Dense kernels of the blocked LU factorisation and triangular solves.
+//
+//  * gemm_kernel<T>: C = alpha A B + beta C, column-major, on the matrix cores
+//    (v_mfma_f64_16x16x4f64 for FP64, v_mfma_f32_16x16x4f32 for FP32). 128x128
+//    block tile, 2x2 wavefronts of 64x64, BK = 16, LDS double buffer with
+//    register prefetch of the next tile. ROUND = 1/2 rounds both operands to
+//    BF16/FP16 on their way into LDS (FP32 accumulation): the precision of a
+//    BF16/FP16 matrix-core product, used by the precision ladder.
+//  * diag_factor_kernel: LU without pivoting of one nb x nb (nb <= 64)
+//    diagonal block in LDS, plus the explicit inverses of its unit-lower and
+//    upper triangles (used by the blocked triangular solves).
+//  * panel kernels for partial pivoting: pivot search with a last-block
+//    reduction, row swap, column scaling and rank-1 update of the sub-panel.
+//  * blocked triangular solves (forward unit-lower, backward upper) with 64-row
+//    diagonal blocks applied through the stored inverses.
+//
+// Replaces cusolverDnSgetrf / cusolverDnSgetrs inside lu_ir.so
+// (benchmark_mpdok.ipynb:989-992, mpdok_ops.py:72) and cp.linalg.solve's FP64
+// getrf/getrs for the plain comparator (dense_krylov.py:526,608,681).
+#include "common.h"
+#include "lu.h"
+
+namespace mpdok {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Result-row of accumulator element r held by a lane, inside a 16 x 16 tile:
+// the FP64 instruction interleaves the rows (row = lane/16 + 4 r), the FP32
+// one stacks them (row = 4 (lane/16) + r); the column is lane % 16 for both.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef v4d acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+  typedef v4f acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <int ROUND>
+__device__ __forceinline__ float round_low(float v) {
+  if (ROUND == 1) return (float)(__bf16)v;
+  if (ROUND == 2) return (float)(_Float16)v;
+  return v;
+}
+template <int ROUND>
+__device__ __forceinline__ double round_low(double v) {
+  return v;
+}
+
+constexpr int GBM = 128, GBN = 128, GBK = 16, GPAD = 16;
+
+// Lane layout of the 16x16x4 MFMAs (one value per lane for A and B, four for
+// the 16x16 result): A[i = l%16][k = l/16], B[k = l/16][j = l%16],
+// D[i = Mfma<T>::row(l, r)][j = l%16], r = 0..3.
+template <typename T, int ROUND>
+__global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, T alpha,
+                                                   const T* __restrict__ A, int64_t lda,
+                                                   const T* __restrict__ B, int64_t ldb, T beta,
+                                                   T* __restrict__ C, int64_t ldc, int64_t sA_,
+                                                   int64_t sB_, int64_t sC_) {
+  __shared__ T sA[2][GBK][GBM + GPAD];
+  __shared__ T sB[2][GBK][GBN + GPAD];
+  typedef typename Mfma<T>::acc_t acc_t;
+  // batched use: blockIdx.y selects the problem; blockIdx.x the tile
+  A += blockIdx.y * sA_;
+  B += blockIdx.y * sB_;
+  C += blockIdx.y * sC_;
+  int ti, tj;
+  if (!tile_of(blockIdx.x, (m + GBM - 1) / GBM, (n + GBN - 1) / GBN, ti, tj)) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int bm = ti * GBM, bn = tj * GBN;
+  acc_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = acc_t(0);
+
+  T ra[8], rb[8];
+  const int ai = bm + (tid & 127), ak = tid >> 7;        // A: row, k phase (0..1)
+  const int bk = tid & 15, bj = bn + (tid >> 4);         // B: k, column
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int kk = k0 + ak + 2 * r;
+      ra[r] = (ai < m && kk < k) ? A[ai + (int64_t)kk * lda] : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int kk = k0 + bk, j = bj + 16 * r;
+      rb[r] = (kk < k && j < n) ? B[kk + (int64_t)j * ldb] : T(0);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) sA[buf][ak + 2 * r][tid & 127] = round_low<ROUND>(ra[r]);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) sB[buf][bk][(tid >> 4) + 16 * r] = round_low<ROUND>(rb[r]);
+  };
+
+  const int nk = (k + GBK - 1) / GBK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int t = 0; t < nk; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < nk) gload((t + 1) * GBK);
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 4) {
+      T a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = sA[buf][kk + (lane >> 4)][wm + 16 * q + (lane & 15)];
+        b[q] = sB[buf][kk + (lane >> 4)][wn + 16 * q + (lane & 15)];
+      }
+#pragma unroll
+      for (int qa = 0; qa < 4; ++qa)
+#pragma unroll
+        for (int qb = 0; qb < 4; ++qb) acc[qa][qb] = Mfma<T>::mma(a[qa], b[qb], acc[qa][qb]);
+    }
+    if (t + 1 < nk) sstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int qa = 0; qa < 4; ++qa)
+#pragma unroll
+    for (int qb = 0; qb < 4; ++qb) {
+      const int j = bn + wn + 16 * qb + (lane & 15);
+      if (j >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = bm + wm + 16 * qa + Mfma<T>::row(lane, r);
+        if (i < m) {
+          T* c = C + i + (int64_t)j * ldc;
+          T v = alpha * acc[qa][qb][r];
+          if (beta != T(0)) v = __builtin_fma(beta, *c, v);
+          *c = v;
+        }
+      }
+    }
+}
+
+template <typename T>
+int gemm_batched(hipStream_t s, int m, int n, int k, T alpha, const T* A, int64_t lda,
+                 int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta, T* C, int64_t ldc,
+                 int64_t sC, int batch, int round) {
+  if (m <= 0 || n <= 0 || batch <= 0) return OK;
+  MPDOK_REQUIRE(batch <= 65535 && (int64_t)((m + GBM - 1) / GBM) * ((n + GBN - 1) / GBN) < (1 << 30),
+                "gemm: grid too large (m=%d n=%d batch=%d)", m, n, batch);
+  dim3 grid(tiles_launch((m + GBM - 1) / GBM, (n + GBN - 1) / GBN), batch);
+  if (round == 0)
+    gemm_kernel<T, 0><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
+  else if (round == 1)
+    gemm_kernel<T, 1><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
+  else
+    gemm_kernel<T, 2><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+template <typename T>
+int gemm(hipStream_t s, int m, int n, int k, T alpha, const T* A, int64_t lda, const T* B,
+         int64_t ldb, T beta, T* C, int64_t ldc, int round) {
+  return gemm_batched<T>(s, m, n, k, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, round);
+}
+template int gemm<double>(hipStream_t, int, int, int, double, const double*, int64_t,
+                          const double*, int64_t, double, double*, int64_t, int);
+template int gemm<float>(hipStream_t, int, int, int, float, const float*, int64_t, const float*,
+                         int64_t, float, float*, int64_t, int);
+template int gemm_batched<double>(hipStream_t, int, int, int, double, const double*, int64_t,
+                                  int64_t, const double*, int64_t, int64_t, double, double*,
+                                  int64_t, int64_t, int, int);
+template int gemm_batched<float>(hipStream_t, int, int, int, float, const float*, int64_t, int64_t,
+                                 const float*, int64_t, int64_t, float, float*, int64_t, int64_t,
+                                 int, int);
+
+// ---------------------------------------------------------------------------
+// Diagonal block: LU without pivoting, then inv(unit L) and inv(U).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int64_t lda, int nb,
+                                                          T* __restrict__ Linv,
+                                                          T* __restrict__ Uinv, int ldi,
+                                                          int* __restrict__ info, int col0,
+                                                          int factor) {
+  __shared__ T s[64][65];
+  __shared__ T sl[64][65];
+  __shared__ T su[64][65];
+  const int t = threadIdx.x;
+  for (int e = t; e < nb * nb; e += 256) {
+    const int i = e % nb, j = e / nb;
+    s[i][j] = A[i + (int64_t)j * lda];
+  }
+  __syncthreads();
+  if (factor) {
+    for (int j = 0; j < nb; ++j) {
+      const T piv = s[j][j];
+      if (t == 0 && piv == T(0)) atomicCAS(info, 0, col0 + j + 1);
+      for (int i = j + 1 + t; i < nb; i += 256) s[i][j] = s[i][j] / piv;
+      __syncthreads();
+      const int w = nb - j - 1;
+      for (int e = t; e < w * w; e += 256) {
+        const int i = j + 1 + e % w, c = j + 1 + e / w;
+        s[i][c] = __builtin_fma(-s[i][j], s[j][c], s[i][c]);
+      }
+      __syncthreads();
+    }
+    for (int e = t; e < nb * nb; e += 256) {
+      const int i = e % nb, j = e / nb;
+      A[i + (int64_t)j * lda] = s[i][j];
+    }
+  }
+  if (t < nb) {
+    const int c = t;
+    // column c of inv(L), L unit lower
+    for (int i = 0; i < c; ++i) sl[i][c] = T(0);
+    sl[c][c] = T(1);
+    for (int i = c + 1; i < nb; ++i) {
+      T acc = T(0);
+      for (int kk = c; kk < i; ++kk) acc = __builtin_fma(-s[i][kk], sl[kk][c], acc);
+      sl[i][c] = acc;
+    }
+    // column c of inv(U)
+    for (int i = c + 1; i < nb; ++i) su[i][c] = T(0);
+    for (int i = c; i >= 0; --i) {
+      T acc = i == c ? T(1) : T(0);
+      for (int kk = i + 1; kk <= c; ++kk) acc = __builtin_fma(-s[i][kk], su[kk][c], acc);
+      su[i][c] = acc / s[i][i];
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < nb * nb; e += 256) {
+    const int i = e % nb, j = e / nb;
+    Linv[i + j * ldi] = sl[i][j];
+    Uinv[i + j * ldi] = su[i][j];
+  }
+}
+
+template <typename T>
+int diag_factor(hipStream_t s, T* A, int64_t lda, int nb, T* Linv, T* Uinv, int ldi, int* info,
+                int col0, int factor) {
+  MPDOK_REQUIRE(nb >= 1 && nb <= 64, "diag_factor: nb=%d", nb);
+  diag_factor_kernel<T><<<1, 256, 0, s>>>(A, lda, nb, Linv, Uinv, ldi, info, col0, factor);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int diag_factor<double>(hipStream_t, double*, int64_t, int, double*, double*, int, int*,
+                                 int, int);
+template int diag_factor<float>(hipStream_t, float*, int64_t, int, float*, float*, int, int*, int,
+                                int);
+
+// ---------------------------------------------------------------------------
+// 2-D copy (column-major) and row permutation
+// ---------------------------------------------------------------------------
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void copy2d_kernel(const TI* __restrict__ src, int64_t lds,
+                                                     TO* __restrict__ dst, int64_t ldd, int m,
+                                                     int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  for (int j = blockIdx.y; j < n; j += gridDim.y) dst[i + (int64_t)j * ldd] = (TO)src[i + (int64_t)j * lds];
+}
+
+template <typename TI, typename TO>
+int copy2d(hipStream_t s, const TI* src, int64_t lds, TO* dst, int64_t ldd, int m, int n) {
+  if (m <= 0 || n <= 0) return OK;
+  dim3 grid((m + 255) / 256, n < 4096 ? n : 4096);
+  copy2d_kernel<TI, TO><<<grid, 256, 0, s>>>(src, lds, dst, ldd, m, n);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int copy2d<double, double>(hipStream_t, const double*, int64_t, double*, int64_t, int, int);
+template int copy2d<float, float>(hipStream_t, const float*, int64_t, float*, int64_t, int, int);
+template int copy2d<double, float>(hipStream_t, const double*, int64_t, float*, int64_t, int, int);
+template int copy2d<float, double>(hipStream_t, const float*, int64_t, double*, int64_t, int, int);
+
+// Apply row interchanges ipiv[r0..r1) (0-based, LAPACK order) to columns
+// [c0, c1) of A: one thread per column, swaps applied in sequence.
+template <typename T>
+__global__ __launch_bounds__(256) void laswp_kernel(T* __restrict__ A, int64_t lda, int c0, int c1,
+                                                    const int* __restrict__ ipiv, int r0, int r1) {
+  const int j = c0 + blockIdx.x * 256 + threadIdx.x;
+  if (j >= c1) return;
+  T* col = A + (int64_t)j * lda;
+  for (int r = r0; r < r1; ++r) {
+    const int p = ipiv[r];
+    if (p != r) {
+      T a = col[r];
+      col[r] = col[p];
+      col[p] = a;
+    }
+  }
+}
+
+template <typename T>
+int laswp(hipStream_t s, T* A, int64_t lda, int c0, int c1, const int* ipiv, int r0, int r1) {
+  if (c1 <= c0 || r1 <= r0) return OK;
+  laswp_kernel<T><<<(c1 - c0 + 255) / 256, 256, 0, s>>>(A, lda, c0, c1, ipiv, r0, r1);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int laswp<double>(hipStream_t, double*, int64_t, int, int, const int*, int, int);
+template int laswp<float>(hipStream_t, float*, int64_t, int, int, const int*, int, int);
+
+// ---------------------------------------------------------------------------
+// Partial pivoting inside a sub-panel of width w (columns j0..j0+w), rows
+// [j, N). Column c = j - j0 of the sub-panel:
+//   pivot_kernel : max |A[i, j]| over i >= j (ties -> smallest i); the last
+//                  block to finish reduces the block maxima, records ipiv[j]
+//                  and swaps rows j and p inside the sub-panel columns.
+//   elim_kernel  : A[i, j] /= A[j, j]; A[i, j+1:j0+w] -= A[i, j] A[j, j+1:j0+w].
+// ---------------------------------------------------------------------------
+constexpr int PIV_ROWS = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(256) void pivot_kernel(T* __restrict__ A, int64_t lda, int N, int j,
+                                                    int j0, int w, double* __restrict__ pval,
+                                                    int* __restrict__ pidx,
+                                                    unsigned* __restrict__ counter,
+                                                    int* __restrict__ ipiv, int* __restrict__ info) {
+  const T* col = A + (int64_t)j * lda;
+  const int base = j + blockIdx.x * PIV_ROWS;
+  double best = -1.0;
+  int bi = 0x7fffffff;
+  for (int i = base + threadIdx.x; i < min(N, base + PIV_ROWS); i += 256) {
+    const double v = fabs((double)col[i]);
+    if (v > best) { best = v; bi = i; }
+  }
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double v2 = sv[threadIdx.x + off];
+      const int i2 = si[threadIdx.x + off];
+      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
+        sv[threadIdx.x] = v2;
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    pval[blockIdx.x] = sv[0];
+    pidx[blockIdx.x] = si[0];
+    __threadfence();
+    last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  __shared__ int p;
+  if (threadIdx.x == 0) {
+    double bv = -1.0;
+    int bidx = j;
+    for (int b = 0; b < (int)gridDim.x; ++b) {
+      const double v = __hip_atomic_load(pval + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int ii = __hip_atomic_load(pidx + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (v > bv) { bv = v; bidx = ii; }
+    }
+    p = bidx;
+    ipiv[j] = bidx;
+    if (bv == 0.0) atomicCAS(info, 0, j + 1);
+    *counter = 0u;
+  }
+  __syncthreads();
+  if (p != j && threadIdx.x < w) {
+    T* c = A + (int64_t)(j0 + threadIdx.x) * lda;
+    T a = c[j];
+    c[j] = c[p];
+    c[p] = a;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void elim_kernel(T* __restrict__ A, int64_t lda, int N, int j,
+                                                   int j0, int w) {
+  const int i = j + 1 + blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const T piv = A[j + (int64_t)j * lda];
+  T l = A[i + (int64_t)j * lda];
+  if (piv != T(0)) l = l / piv;
+  A[i + (int64_t)j * lda] = l;
+  for (int c = j + 1; c < j0 + w; ++c)
+    A[i + (int64_t)c * lda] = __builtin_fma(-l, A[j + (int64_t)c * lda], A[i + (int64_t)c * lda]);
+}
+
+template <typename T>
+int panel_gepp(hipStream_t s, T* A, int64_t lda, int N, int j0, int w, int* ipiv, int* info,
+               void* work) {
+  // work: pval[nblk] (double), pidx[nblk] (int), counter (unsigned)
+  const int maxblk = (N + PIV_ROWS - 1) / PIV_ROWS;
+  double* pval = static_cast<double*>(work);
+  int* pidx = reinterpret_cast<int*>(pval + maxblk);
+  unsigned* counter = reinterpret_cast<unsigned*>(pidx + maxblk + 1);
+  for (int j = j0; j < j0 + w && j < N; ++j) {
+    const int nblk = (N - j + PIV_ROWS - 1) / PIV_ROWS;
+    pivot_kernel<T><<<nblk, 256, 0, s>>>(A, lda, N, j, j0, w, pval, pidx, counter, ipiv, info);
+    if (j + 1 < N)
+      elim_kernel<T><<<(N - j - 1 + 255) / 256, 256, 0, s>>>(A, lda, N, j, j0, w);
+  }
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+size_t panel_gepp_workspace_bytes(int N) {
+  const int maxblk = (N + PIV_ROWS - 1) / PIV_ROWS;
+  return (size_t)maxblk * (sizeof(double) + sizeof(int)) + 64;
+}
+template int panel_gepp<double>(hipStream_t, double*, int64_t, int, int, int, int*, int*, void*);
+template int panel_gepp<float>(hipStream_t, float*, int64_t, int, int, int, int*, int*, void*);
+
+// ---------------------------------------------------------------------------
+// Blocked triangular solves with 64 x 64 diagonal blocks applied through
+// their stored inverses (DI: nblk blocks of 64x64, column-major, ld 64).
+//   forward : y = inv(L) b   (L unit lower)      backward : x = inv(U) y
+// Each 64-row block step = one small kernel (block solve) + one GEMV update of
+// the remaining rows (forward) or rows above (backward) in the vector type TV.
+// ---------------------------------------------------------------------------
+template <typename T, typename TV>
+__global__ __launch_bounds__(64) void tri_block_solve_kernel(const T* __restrict__ DI, int r0,
+                                                             int nb, TV* __restrict__ v) {
+  __shared__ TV sv[64];
+  const int t = threadIdx.x;
+  if (t < nb) sv[t] = v[r0 + t];
+  __syncthreads();
+  if (t < nb) {
+    TV acc = TV(0);
+    for (int kk = 0; kk < nb; ++kk) acc = __builtin_fma((TV)DI[t + kk * 64], sv[kk], acc);
+    v[r0 + t] = acc;
+  }
+}
+
+// v[i] -= sum_{c in [c0, c0+nc)} M[i, c] v[c] for rows i in [i0, i1)
+template <typename T, typename TV>
+__global__ __launch_bounds__(256) void tri_update_kernel(const T* __restrict__ M, int64_t ldm,
+                                                         int i0, int i1, int c0, int nc,
+                                                         TV* __restrict__ v) {
+  __shared__ TV sx[64];
+  if (threadIdx.x < nc) sx[threadIdx.x] = v[c0 + threadIdx.x];
+  __syncthreads();
+  const int i = i0 + blockIdx.x * 256 + threadIdx.x;
+  if (i >= i1) return;
+  TV acc = TV(0);
+  for (int c = 0; c < nc; ++c) acc = __builtin_fma((TV)M[i + (int64_t)(c0 + c) * ldm], sx[c], acc);
+  v[i] -= acc;
+}
+
+template <typename T, typename TV>
+int trsv_blocked(hipStream_t s, const T* LU, int64_t ld, int N, const T* Linv_all,
+                 const T* Uinv_all, TV* v) {
+  const int nblk = (N + 63) / 64;
+  for (int b = 0; b < nblk; ++b) {  // forward, unit lower
+    const int r0 = b * 64, nb = min(64, N - r0);
+    tri_block_solve_kernel<T, TV><<<1, 64, 0, s>>>(Linv_all + (size_t)b * 64 * 64, r0, nb, v);
+    if (r0 + nb < N)
+      tri_update_kernel<T, TV><<<(N - r0 - nb + 255) / 256, 256, 0, s>>>(LU, ld, r0 + nb, N, r0,
+                                                                         nb, v);
+  }
+  for (int b = nblk - 1; b >= 0; --b) {  // backward, upper
+    const int r0 = b * 64, nb = min(64, N - r0);
+    tri_block_solve_kernel<T, TV><<<1, 64, 0, s>>>(Uinv_all + (size_t)b * 64 * 64, r0, nb, v);
+    if (r0 > 0)
+      tri_update_kernel<T, TV><<<(r0 + 255) / 256, 256, 0, s>>>(LU, ld, 0, r0, r0, nb, v);
+  }
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int trsv_blocked<double, double>(hipStream_t, const double*, int64_t, int, const double*,
+                                          const double*, double*);
+template int trsv_blocked<float, float>(hipStream_t, const float*, int64_t, int, const float*,
+                                        const float*, float*);
+template int trsv_blocked<float, double>(hipStream_t, const float*, int64_t, int, const float*,
+                                         const float*, double*);
+
+// ---------------------------------------------------------------------------
+// Triangular solves with explicit inverses of TB x TB diagonal blocks
+// (TB = 512). One step per diagonal block, two kernels per step:
+//   diag_apply : x_i = inv(D_i) (v_i - sum_c acc[c, rows of i])
+//   tri_acc    : acc[c, r] += sum_{j in chunk c of block i} M[r, j] x_j  for the
+//                rows r still to be solved (below i forward, above i backward)
+// Every (c, r) accumulator is written by one thread per step in stream order,
+// so the result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr int TB = 512, TB_CH = 8;
+
+template <typename T, typename TV>
+__global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ Binv, int r0,
+                                                         const TV* __restrict__ v,
+                                                         const TV* __restrict__ acc, int N,
+                                                         TV* __restrict__ x) {
+  __shared__ TV vs[TB];
+  __shared__ TV red[4][64];
+  const int t = threadIdx.x;
+  for (int j = t; j < TB; j += 256) {
+    TV s = v[r0 + j];
+#pragma unroll
+    for (int c = 0; c < TB_CH; ++c) s -= acc[(size_t)c * N + r0 + j];
+    vs[j] = s;
+  }
+  __syncthreads();
+  const int r = t & 63, p = t >> 6;
+  const int row = blockIdx.x * 64 + r;
+  TV a = TV(0);
+  for (int j = p; j < TB; j += 4) a = __builtin_fma((TV)Binv[row + (size_t)j * TB], vs[j], a);
+  red[p][r] = a;
+  __syncthreads();
+  if (t < 64) x[r0 + blockIdx.x * 64 + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+}
+
+template <typename T, typename TV>
+__global__ __launch_bounds__(256) void tri_acc_kernel(const T* __restrict__ M, int64_t ldm,
+                                                      int rbeg, int rend, int c0,
+                                                      const TV* __restrict__ x,
+                                                      TV* __restrict__ acc, int N) {
+  constexpr int CW = TB / TB_CH;
+  __shared__ TV xs[CW];
+  const int c = blockIdx.y;
+  const int cb = c0 + c * CW;
+  if (threadIdx.x < CW) xs[threadIdx.x] = x[cb + threadIdx.x];
+  __syncthreads();
+  const int r = rbeg + blockIdx.x * 256 + threadIdx.x;
+  if (r >= rend) return;
+  TV a = TV(0);
+#pragma unroll 8
+  for (int j = 0; j < CW; ++j) a = __builtin_fma((TV)M[r + (int64_t)(cb + j) * ldm], xs[j], a);
+  acc[(size_t)c * N + r] += a;
+}
+
+template <typename T, typename TV>
+int trsv_block_inverse(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
+                       const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc) {
+  MPDOK_REQUIRE(N % TB == 0, "trsv_block_inverse: N must be a multiple of %d", TB);
+  const int nb = N / TB;
+  // forward: v = b, solve L y = v  (y -> x)
+  MPDOK_HIP(hipMemsetAsync(acc, 0, sizeof(TV) * (size_t)TB_CH * N, s));
+  for (int i = 0; i < nb; ++i) {
+    const int r0 = i * TB;
+    diag_apply_kernel<T, TV><<<TB / 64, 256, 0, s>>>(Lbinv + (size_t)i * TB * TB, r0, b, acc, N, v);
+    if (r0 + TB < N)
+      tri_acc_kernel<T, TV><<<dim3((N - r0 - TB + 255) / 256, TB_CH), 256, 0, s>>>(
+          LU, ld, r0 + TB, N, r0, v, acc, N);
+  }
+  // backward: solve U x = y (y in v)
+  MPDOK_HIP(hipMemsetAsync(acc, 0, sizeof(TV) * (size_t)TB_CH * N, s));
+  for (int i = nb - 1; i >= 0; --i) {
+    const int r0 = i * TB;
+    diag_apply_kernel<T, TV><<<TB / 64, 256, 0, s>>>(Ubinv + (size_t)i * TB * TB, r0, v, acc, N, x);
+    if (r0 > 0)
+      tri_acc_kernel<T, TV><<<dim3((r0 + 255) / 256, TB_CH), 256, 0, s>>>(LU, ld, 0, r0, r0, x,
+                                                                         acc, N);
+  }
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int trsv_block_inverse<double, double>(hipStream_t, const double*, int64_t, int,
+                                                const double*, const double*, const double*,
+                                                double*, double*, double*);
+template int trsv_block_inverse<float, float>(hipStream_t, const float*, int64_t, int, const float*,
+                                              const float*, const float*, float*, float*, float*);
+
+// Place the 64 x 64 diagonal inverses into the TB x TB blocks (zero elsewhere).
+template <typename T>
+__global__ __launch_bounds__(256) void seed_block_inverse_kernel(const T* __restrict__ inv64,
+                                                                 T* __restrict__ binv, int nbig) {
+  // blockIdx.x: TB x TB block index; blockIdx.y: column of that block
+  const int b = blockIdx.x, col = blockIdx.y;
+  T* dst = binv + (size_t)b * TB * TB + (size_t)col * TB;
+  const int g = b * (TB / 64) + col / 64;  // global 64-block index
+  const T* src = inv64 + (size_t)g * 4096 + (size_t)(col % 64) * 64;
+  for (int r = threadIdx.x; r < TB; r += 256) {
+    const int lo = (col / 64) * 64;
+    dst[r] = (r >= lo && r < lo + 64) ? src[r - lo] : T(0);
+  }
+}
+
+template <typename T>
+int seed_block_inverse(hipStream_t s, const T* inv64, T* binv, int nbig) {
+  if (nbig <= 0) return OK;
+  seed_block_inverse_kernel<T><<<dim3(nbig, TB), 256, 0, s>>>(inv64, binv, nbig);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int seed_block_inverse<double>(hipStream_t, const double*, double*, int);
+template int seed_block_inverse<float>(hipStream_t, const float*, float*, int);
+
+// Gather with a row permutation: out[i] = in[perm[i]]
+template <typename T>
+__global__ __launch_bounds__(256) void permute_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                      const int* __restrict__ perm, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = in[perm[i]];
+}
+template <typename T>
+int permute(hipStream_t s, const T* in, T* out, const int* perm, int n) {
+  if (n <= 0) return OK;
+  permute_kernel<T><<<(n + 255) / 256, 256, 0, s>>>(in, out, perm, n);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int permute<double>(hipStream_t, const double*, double*, const int*, int);
+template int permute<float>(hipStream_t, const float*, float*, const int*, int);
+
+// ipiv (sequential interchanges) -> permutation vector perm with
+// (P b)[i] = b[perm[i]]. Single thread; run once per factorisation.
+__global__ void ipiv_to_perm_kernel(const int* __restrict__ ipiv, int* __restrict__ perm, int n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  for (int i = 0; i < n; ++i) {
+    const int p = ipiv[i];
+    if (p != i) {
+      const int t = perm[i];
+      perm[i] = perm[p];
+      perm[p] = t;
+    }
+  }
+}
+int ipiv_to_perm(hipStream_t s, const int* ipiv, int* perm, int n) {
+  ipiv_to_perm_kernel<<<1, 64, 0, s>>>(ipiv, perm, n);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+}  // namespace mpdok

@@ -1,0 +1,166 @@
+// Fused kernel-matrix assembly: pairwise distance -> kernel value -> nugget
+// -> output cast, written once in the target layout.
+//
+// Reference: rbf_kernel.py:158-227 (build_rbf_kernel: D2 = sq_i + sq_j - 2 c_i.c_j
+// via the Gram identity, clamp at 0, exp(-gamma D2), then A[idx, idx] += reg in
+// FP64) and mpdok_ooc.py:39-46,118-123 (_build_fp32_tile: the same FP64 value
+// cast to FP32, then the nugget added in FP32 after the cast). The reference
+// evaluates this in 4+ elementwise passes over (rows x N) temporaries; here one
+// thread computes each output element in registers.
+//
+// Kernel kinds:  0 = Gaussian  exp(-p * d2)             (p = gamma)
+//                1 = Matern-3/2 (1 + a) exp(-a), a = sqrt(3 d2) / p  (p = length scale)
+// reg_mode:      0 = nugget added in FP64 before the cast
+//                1 = nugget added after the cast, in the output precision
+#include "common.h"
+#include "rbf.h"
+
+namespace mpdok {
+
+constexpr int MAXD = 8;
+
+__device__ __forceinline__ double kernel_value(int kind, double p, double d2) {
+  if (kind == 0) return exp(-p * d2);
+  const double a = sqrt(3.0 * d2) / p;
+  return (1.0 + a) * exp(-a);
+}
+
+// Squared distance by the Gram identity, evaluated in the reference's order:
+// (sq_i + sq_j) - 2 * (c_i . c_j), clamped at zero.
+__device__ __forceinline__ double gram_d2(const double* ci, double sqi, const double* cj, double sqj,
+                                          int D) {
+  double dot = ci[0] * cj[0];
+#pragma unroll
+  for (int d = 1; d < MAXD; ++d)
+    if (d < D) dot = __builtin_fma(ci[d], cj[d], dot);
+  double d2 = (sqi + sqj) - 2.0 * dot;
+  return d2 > 0.0 ? d2 : 0.0;
+}
+
+__device__ __forceinline__ double sqnorm(const double* c, int D) {
+  double s = c[0] * c[0];
+#pragma unroll
+  for (int d = 1; d < MAXD; ++d)
+    if (d < D) s += c[d] * c[d];
+  return s;
+}
+
+template <typename TO> __device__ __forceinline__ TO to_out(double v);
+template <> __device__ __forceinline__ double to_out<double>(double v) { return v; }
+template <> __device__ __forceinline__ float to_out<float>(double v) { return (float)v; }
+
+// Output element (i, j), i in [row0, row0+rows), j in [col0, col0+cols), stored at
+// out[(i-row0)*ld + (j-col0)] (ROW_MAJOR) or out[(i-row0) + (j-col0)*ld] (COL_MAJOR).
+// The fast thread index follows the contiguous output dimension; the other
+// point is uniform across a wavefront and comes in through scalar loads.
+template <typename TO, int ORDER>
+__global__ __launch_bounds__(256) void rbf_tile_kernel(const double* __restrict__ coords, int D,
+                                                       int kind, double p, double reg, int reg_mode,
+                                                       int row0, int rows, int col0, int cols,
+                                                       TO* __restrict__ out, int64_t ld) {
+  // fast index f (64 per block), slow index s (64 per block: 4 waves x 16)
+  const int f = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int sbase = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
+  const int nf = ORDER == ROW_MAJOR ? cols : rows;
+  const int ns = ORDER == ROW_MAJOR ? rows : cols;
+  if (f >= nf) return;
+  const int goff_f = ORDER == ROW_MAJOR ? col0 : row0;
+  const int goff_s = ORDER == ROW_MAJOR ? row0 : col0;
+  double cf[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) cf[d] = d < D ? coords[(int64_t)(goff_f + f) * D + d] : 0.0;
+  const double sqf = sqnorm(cf, D);
+  for (int k = 0; k < 16; ++k) {
+    const int s = sbase + k;
+    if (s >= ns) break;
+    double cs[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) cs[d] = d < D ? coords[(int64_t)(goff_s + s) * D + d] : 0.0;
+    const double sqs = sqnorm(cs, D);
+    // The reference forms D2[i, j] with i the row point: keep that operand order.
+    double d2 = ORDER == ROW_MAJOR ? gram_d2(cs, sqs, cf, sqf, D) : gram_d2(cf, sqf, cs, sqs, D);
+    double v = kernel_value(kind, p, d2);
+    const bool diag = (goff_f + f) == (goff_s + s);
+    TO o;
+    if (diag && reg_mode == 0) {
+      o = to_out<TO>(v + reg);
+    } else {
+      o = to_out<TO>(v);
+      if (diag) o = o + (TO)reg;
+    }
+    const int64_t idx = ORDER == ROW_MAJOR ? (int64_t)s * ld + f : (int64_t)s * ld + f;
+    out[idx] = o;
+  }
+}
+
+int rbf_assemble(hipStream_t st, const double* coords, int N, int D, int kind, double p,
+                 double reg, int reg_mode, int row0, int rows, int col0, int cols, void* out,
+                 int out_dtype, int64_t ld, int order) {
+  MPDOK_REQUIRE(coords && out, "rbf_assemble: null pointer");
+  MPDOK_REQUIRE(D >= 1 && D <= MAXD, "rbf_assemble: D must be in [1, %d], got %d", MAXD, D);
+  MPDOK_REQUIRE(kind == 0 || kind == 1, "rbf_assemble: unknown kernel kind %d", kind);
+  MPDOK_REQUIRE(row0 >= 0 && rows >= 0 && row0 + rows <= N && col0 >= 0 && cols >= 0 &&
+                    col0 + cols <= N,
+                "rbf_assemble: tile [%d,+%d)x[%d,+%d) outside N=%d", row0, rows, col0, cols, N);
+  MPDOK_REQUIRE(order == ROW_MAJOR ? ld >= cols : ld >= rows, "rbf_assemble: ld too small");
+  MPDOK_REQUIRE(!(kind == 1 && !(p > 0.0)), "rbf_assemble: Matern length scale must be > 0");
+  if (rows == 0 || cols == 0) return OK;
+  const int nf = order == ROW_MAJOR ? cols : rows;
+  const int ns = order == ROW_MAJOR ? rows : cols;
+  dim3 grid((nf + 63) / 64, (ns + 63) / 64);
+  MPDOK_REQUIRE(grid.y <= 65535, "rbf_assemble: tile too tall");
+#define LAUNCH(TO, ORD)                                                                     \
+  rbf_tile_kernel<TO, ORD><<<grid, 256, 0, st>>>(coords, D, kind, p, reg, reg_mode, row0, rows, \
+                                                 col0, cols, (TO*)out, ld)
+  if (out_dtype == F64) {
+    if (order == ROW_MAJOR) LAUNCH(double, ROW_MAJOR); else LAUNCH(double, COL_MAJOR);
+  } else if (out_dtype == F32) {
+    if (order == ROW_MAJOR) LAUNCH(float, ROW_MAJOR); else LAUNCH(float, COL_MAJOR);
+  } else {
+    MPDOK_REQUIRE(false, "rbf_assemble: unsupported output dtype %d", out_dtype);
+  }
+#undef LAUNCH
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+// Mean of the clamped squared distances over the first n points, diagonal
+// included (rbf_kernel.py:57-68). Single block; n <= 512 in the reference.
+__global__ __launch_bounds__(256) void mean_d2_kernel(const double* __restrict__ coords, int n,
+                                                      int D, double* __restrict__ out) {
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += 256) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    double ci[MAXD], cj[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) {
+      ci[d] = d < D ? coords[(int64_t)i * D + d] : 0.0;
+      cj[d] = d < D ? coords[(int64_t)j * D + d] : 0.0;
+    }
+    acc += gram_d2(ci, sqnorm(ci, D), cj, sqnorm(cj, D), D);
+  }
+  acc = wave_sum(acc);
+  __shared__ double s[4];
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = ((s[0] + s[1]) + (s[2] + s[3])) / ((double)n * (double)n);
+}
+
+int rbf_mean_d2(hipStream_t st, const double* coords, int n, int D, double* out) {
+  MPDOK_REQUIRE(coords && out && n >= 1 && D >= 1 && D <= MAXD, "rbf_mean_d2: bad arguments");
+  mean_d2_kernel<<<1, 256, 0, st>>>(coords, n, D, out);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+}  // namespace mpdok
+
+MPDOK_API int mpdok_rbf_assemble(hipStream_t st, const double* coords, int N, int D, int kind,
+                                 double p, double reg, int reg_mode, int row0, int rows, int col0,
+                                 int cols, void* out, int out_dtype, int64_t ld, int order) {
+  return mpdok::rbf_assemble(st, coords, N, D, kind, p, reg, reg_mode, row0, rows, col0, cols, out,
+                             out_dtype, ld, order);
+}
+MPDOK_API int mpdok_rbf_mean_d2(hipStream_t st, const double* coords, int n, int D, double* out) {
+  return mpdok::rbf_mean_d2(st, coords, n, D, out);
+}

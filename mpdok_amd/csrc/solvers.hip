@@ -1,0 +1,432 @@
+// Iterative-refinement drivers and the reference-compatible plugin symbols.
+//
+// gmres_ir: outer refinement in FP64 (residual r = b - A x with FP64 or Dot2
+// accumulation, update x += e in FP64, stopping test ||r||/||b|| < tol) around
+// one restart cycle of GMRES per outer step:
+//   * body FP32 or FP64 (Arnoldi vectors and the matvec with A);
+//   * optional right preconditioner M = P L U (an LU handle): the cycle solves
+//     A inv(M) u = r and returns e = inv(M) u;
+//   * CGS with one or two passes, device Givens rotations, inner stop at
+//     |g[k+1]| / beta < inner_tol (mpdok_ooc.py:231-311, the port of the
+//     native inner_gmres named at mpdok_ooc.py:229-236).
+// lu_ir: x += inv(PLU) r with r in FP64 (benchmark_mpdok.ipynb:983-996).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+#include "common.h"
+#include "gemv.h"
+#include "krylov.h"
+#include "lu.h"
+
+namespace mpdok {
+
+static hipStream_t g_stream = nullptr;  // stream of the py_* symbols
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  int get(size_t bytes) {
+    if (bytes <= n) return OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      set_error("workspace: hipMalloc of %zu bytes failed", bytes);
+      return ENOMEM_;
+    }
+    n = bytes;
+    return OK;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct IRWork {
+  DevBuf V, w, z, r, e, H, R, cs, sn, g, ctl, y, dots, orth, gemv, resid;
+  int* ctl_host = nullptr;
+  ~IRWork() {
+    if (ctl_host) (void)hipHostFree(ctl_host);
+  }
+};
+
+struct IROpts {
+  double tol = 1e-11;
+  int maxiter_outer = 5;
+  int restart = 50;
+  double inner_tol = 1e-6;
+  int passes = 2;
+  int resid_mode = 0;  // 0 FP64, 1 Dot2
+  int body = 0;        // 0 FP32, 1 FP64
+};
+
+struct IRResult {
+  int converged = 0;
+  int outer = 0;
+  int inner_total = 0;
+  std::vector<double> hist;
+};
+
+static int host_norm(hipStream_t s, const double* v, int n, DevBuf& dots, double* out) {
+  MPDOK_TRY(dots.get(reduce_workspace_bytes(n) + 64));
+  double* res = reinterpret_cast<double*>(static_cast<char*>(dots.p) + reduce_workspace_bytes(n));
+  MPDOK_TRY(dot_f64(s, v, v, n, res, dots.p));
+  double h = 0.0;
+  MPDOK_HIP(hipMemcpyAsync(&h, res, sizeof(double), hipMemcpyDeviceToHost, s));
+  MPDOK_HIP(hipStreamSynchronize(s));
+  *out = std::sqrt(h);
+  return OK;
+}
+
+// One cycle of (right-preconditioned) GMRES for A e = r. Body type TB.
+template <typename TB>
+static int gmres_cycle(hipStream_t s, const void* Abody, int64_t ldab, int orderb,
+                       LUHandle* M, int N, const double* r, double rnorm, double* e,
+                       const IROpts& o, IRWork& W, int* steps) {
+  const int m = o.restart;
+  const int ldh = m + 1;
+  TB* V = static_cast<TB*>(W.V.p);
+  TB* w = static_cast<TB*>(W.w.p);
+  double* z64 = static_cast<double*>(W.z.p);  // FP64 scratch (preconditioner in/out)
+  double* H = static_cast<double*>(W.H.p);
+  double* R = static_cast<double*>(W.R.p);
+  double* cs = static_cast<double*>(W.cs.p);
+  double* sn = static_cast<double*>(W.sn.p);
+  double* g = static_cast<double*>(W.g.p);
+  int* ctl = static_cast<int*>(W.ctl.p);
+  MPDOK_HIP(hipMemsetAsync(H, 0, sizeof(double) * ldh * m, s));
+  MPDOK_HIP(hipMemsetAsync(g, 0, sizeof(double) * ldh, s));
+  MPDOK_HIP(hipMemsetAsync(ctl, 0, 4 * sizeof(int), s));
+  MPDOK_HIP(hipMemcpyAsync(g, &rnorm, sizeof(double), hipMemcpyHostToDevice, s));
+  // V_0 = r / ||r||
+  MPDOK_TRY(axpby(s, N, 1.0 / rnorm, r, F64, 0.0, nullptr, F64, V, sizeof(TB) == 8 ? F64 : F32));
+  const int dt = sizeof(TB) == 8 ? F64 : F32;
+  int done = 0;
+  for (int k = 0; k < m; ++k) {
+    TB* vk = V + (size_t)k * N;
+    const TB* xin = vk;
+    if (M != nullptr) {
+      // z = inv(M) v_k  (in FP64), then into the body type
+      MPDOK_TRY(axpby(s, N, 1.0, vk, dt, 0.0, nullptr, F64, z64, F64));
+      MPDOK_TRY(lu_solve(M, s, z64, z64));
+      MPDOK_TRY(axpby(s, N, 1.0, z64, F64, 0.0, nullptr, F64, w, dt));
+      // w currently holds inv(M) v_k; move it to the first free slot of V's tail
+      // is not possible (V_{k+1} is the output), so use e as FP64 staging:
+      xin = w;
+    }
+    TB* y = (M != nullptr) ? reinterpret_cast<TB*>(W.e.p) : w;  // matvec output
+    if (sizeof(TB) == 4)
+      MPDOK_TRY(gemv_f32(s, (const float*)Abody, ldab, orderb, N, N, (const float*)xin, (float*)y,
+                         1.0f, W.gemv.p, W.gemv.n, ctl));
+    else
+      MPDOK_TRY(gemv_f64(s, (const double*)Abody, ldab, orderb, N, N, (const double*)xin,
+                         (double*)y, 1.0, W.gemv.p, W.gemv.n, ctl));
+    MPDOK_TRY(arnoldi_orth<TB>(s, V, N, k, y, N, H + (size_t)k * ldh, o.passes, 1e-12, 0, ctl,
+                               W.orth.p, W.orth.n));
+    MPDOK_TRY(givens_step(s, H, ldh, k, R, cs, sn, g, rnorm, o.inner_tol, ctl));
+    done = k + 1;
+    // The host learns whether the cycle stopped before issuing the next (more
+    // expensive, when preconditioned) step: one 4-byte read per step.
+    MPDOK_HIP(hipMemcpyAsync(W.ctl_host, ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    MPDOK_HIP(hipStreamSynchronize(s));
+    if (W.ctl_host[0]) break;
+  }
+  // back substitution on the rotated R (ldh x m, column-major) and g
+  std::vector<double> Rh((size_t)ldh * m), gh(ldh), yh(done);
+  MPDOK_HIP(hipMemcpyAsync(Rh.data(), R, sizeof(double) * ldh * done, hipMemcpyDeviceToHost, s));
+  MPDOK_HIP(hipMemcpyAsync(gh.data(), g, sizeof(double) * ldh, hipMemcpyDeviceToHost, s));
+  MPDOK_HIP(hipStreamSynchronize(s));
+  for (int j = done - 1; j >= 0; --j) {
+    double acc = gh[j];
+    for (int i = j + 1; i < done; ++i) acc -= Rh[(size_t)i * ldh + j] * yh[i];
+    yh[j] = acc / Rh[(size_t)j * ldh + j];
+  }
+  double* ydev = static_cast<double*>(W.y.p);
+  MPDOK_HIP(hipMemcpyAsync(ydev, yh.data(), sizeof(double) * done, hipMemcpyHostToDevice, s));
+  // e = V y (FP64), then e = inv(M) e
+  MPDOK_TRY((basis_update<TB, double>(s, V, N, done, ydev, e, N, 1)));
+  if (M != nullptr) MPDOK_TRY(lu_solve(M, s, e, e));
+  *steps = done;
+  return OK;
+}
+
+// A64: FP64 matrix for the residual (and the FP64 body); A32: FP32 copy for
+// the FP32 body (may be null when body = FP64).
+static int gmres_ir_impl(hipStream_t s, const double* A64, int64_t lda64, int order64,
+                         const float* A32, int64_t lda32, int order32, LUHandle* M, int N,
+                         const double* b, double* x, const IROpts& o, IRResult& res) {
+  MPDOK_REQUIRE(N >= 1 && b && x && A64, "gmres_ir: bad arguments");
+  MPDOK_REQUIRE(o.restart >= 1 && o.restart <= MAX_RESTART, "gmres_ir: restart must be in [1, %d]",
+                MAX_RESTART);
+  MPDOK_REQUIRE(o.body == 1 || A32 != nullptr, "gmres_ir: FP32 body needs the FP32 matrix");
+  MPDOK_REQUIRE(M == nullptr || lu_n(M) == N, "gmres_ir: preconditioner size mismatch");
+  static thread_local IRWork W;
+  const size_t tb = o.body == 1 ? 8 : 4;
+  const int m = o.restart;
+  MPDOK_TRY(W.V.get(tb * (size_t)(m + 1) * N));
+  MPDOK_TRY(W.w.get(8 * (size_t)N));
+  MPDOK_TRY(W.z.get(8 * (size_t)N));
+  MPDOK_TRY(W.r.get(8 * (size_t)N));
+  MPDOK_TRY(W.e.get(8 * (size_t)N));
+  MPDOK_TRY(W.H.get(8 * (size_t)(m + 1) * m));
+  MPDOK_TRY(W.R.get(8 * (size_t)(m + 1) * m));
+  MPDOK_TRY(W.cs.get(8 * (size_t)m));
+  MPDOK_TRY(W.sn.get(8 * (size_t)m));
+  MPDOK_TRY(W.g.get(8 * (size_t)(m + 1)));
+  MPDOK_TRY(W.ctl.get(64));
+  MPDOK_TRY(W.y.get(8 * (size_t)(m + 1)));
+  MPDOK_TRY(W.orth.get(arnoldi_workspace_bytes(N, m)));
+  const int ob = o.body == 1 ? order64 : order32;
+  MPDOK_TRY(W.gemv.get(gemv_workspace_bytes(N, N, o.body == 1 ? F64 : F32, ob) + 64));
+  MPDOK_TRY(W.resid.get(resid_workspace_bytes(N, order64) + 64));
+  if (!W.ctl_host) MPDOK_HIP(hipHostMalloc((void**)&W.ctl_host, 64, 0));
+
+  double* r = static_cast<double*>(W.r.p);
+  double* e = static_cast<double*>(W.z.p);  // reuse: z is free between cycles
+  double bnorm = 0.0;
+  MPDOK_TRY(host_norm(s, b, N, W.dots, &bnorm));
+  MPDOK_HIP(hipMemsetAsync(x, 0, sizeof(double) * N, s));
+  res = IRResult();
+  if (bnorm == 0.0) {
+    res.converged = 1;
+    res.hist.push_back(0.0);
+    return OK;
+  }
+  double rel = 0.0;
+  for (int outer = 0; outer < o.maxiter_outer; ++outer) {
+    MPDOK_TRY(residual_f64(s, A64, lda64, order64, N, x, b, r, o.resid_mode, W.resid.p, W.resid.n));
+    double rn = 0.0;
+    MPDOK_TRY(host_norm(s, r, N, W.dots, &rn));
+    rel = rn / bnorm;
+    res.hist.push_back(rel);
+    res.outer = outer;
+    if (rel < o.tol) {
+      res.converged = 1;
+      return OK;
+    }
+    if (!(rn > 0.0) || !std::isfinite(rn)) break;
+    int steps = 0;
+    double* ecyc = static_cast<double*>(W.e.p);
+    if (o.body == 1)
+      MPDOK_TRY(gmres_cycle<double>(s, A64, lda64, order64, M, N, r, rn, e, o, W, &steps));
+    else
+      MPDOK_TRY(gmres_cycle<float>(s, A32, lda32, order32, M, N, r, rn, e, o, W, &steps));
+    (void)ecyc;
+    res.inner_total += steps;
+    MPDOK_TRY(axpby(s, N, 1.0, x, F64, 1.0, e, F64, x, F64));
+  }
+  MPDOK_TRY(residual_f64(s, A64, lda64, order64, N, x, b, r, o.resid_mode, W.resid.p, W.resid.n));
+  double rn = 0.0;
+  MPDOK_TRY(host_norm(s, r, N, W.dots, &rn));
+  rel = rn / bnorm;
+  res.hist.push_back(rel);
+  res.outer = o.maxiter_outer;
+  res.converged = rel < o.tol;
+  return OK;
+}
+
+// LU-IR: x = 0; repeat { r = b - A x; stop if ||r||/||b|| < tol; x += inv(PLU) r }.
+static int lu_ir_impl(hipStream_t s, const double* A64, int64_t lda, int order, LUHandle* M, int N,
+                      const double* b, double* x, double tol, int maxiter_outer, int resid_mode,
+                      IRResult& res) {
+  static thread_local IRWork W;
+  MPDOK_TRY(W.r.get(8 * (size_t)N));
+  MPDOK_TRY(W.e.get(8 * (size_t)N));
+  MPDOK_TRY(W.resid.get(resid_workspace_bytes(N, order) + 64));
+  double* r = static_cast<double*>(W.r.p);
+  double* d = static_cast<double*>(W.e.p);
+  double bnorm = 0.0;
+  MPDOK_TRY(host_norm(s, b, N, W.dots, &bnorm));
+  MPDOK_HIP(hipMemsetAsync(x, 0, sizeof(double) * N, s));
+  res = IRResult();
+  if (bnorm == 0.0) {
+    res.converged = 1;
+    res.hist.push_back(0.0);
+    return OK;
+  }
+  for (int it = 0; it < maxiter_outer; ++it) {
+    MPDOK_TRY(residual_f64(s, A64, lda, order, N, x, b, r, resid_mode, W.resid.p, W.resid.n));
+    double rn = 0.0;
+    MPDOK_TRY(host_norm(s, r, N, W.dots, &rn));
+    res.hist.push_back(rn / bnorm);
+    res.outer = it;
+    if (rn / bnorm < tol) {
+      res.converged = 1;
+      return OK;
+    }
+    MPDOK_TRY(lu_solve(M, s, r, d));
+    MPDOK_TRY(axpby(s, N, 1.0, x, F64, 1.0, d, F64, x, F64));
+  }
+  MPDOK_TRY(residual_f64(s, A64, lda, order, N, x, b, r, resid_mode, W.resid.p, W.resid.n));
+  double rn = 0.0;
+  MPDOK_TRY(host_norm(s, r, N, W.dots, &rn));
+  res.hist.push_back(rn / bnorm);
+  res.outer = maxiter_outer;
+  res.converged = rn / bnorm < tol;
+  return OK;
+}
+
+// ---------------------------------------------------------------------------
+// State of the reference-compatible symbols (one slot each, like the natives).
+// ---------------------------------------------------------------------------
+static std::mutex g_mu;
+static LUHandle* g_lu = nullptr;
+static void* g_managed_gmres = nullptr;
+static void* g_managed_lu = nullptr;
+
+static int fill_hist(const IRResult& r, double* hist, int hist_cap, int* n_hist) {
+  if (n_hist) *n_hist = (int)r.hist.size();
+  if (hist)
+    for (int i = 0; i < (int)r.hist.size() && i < hist_cap; ++i) hist[i] = r.hist[i];
+  return OK;
+}
+
+}  // namespace mpdok
+
+using namespace mpdok;
+
+MPDOK_API int mpdok_set_stream(hipStream_t s) {
+  g_stream = s;
+  return 0;
+}
+
+// GMRES-IR with an optional LU preconditioner handle (NULL = none).
+// order64/order32: storage order of A64 / A32 (A32 may be NULL for body = 1).
+// hist: host array receiving relres per outer step (and the final check).
+MPDOK_API int mpdok_gmres_ir(hipStream_t s, const double* A64, int64_t lda64, int order64,
+                             const float* A32, int64_t lda32, int order32, void* lu, int N,
+                             const double* b, double* x, double tol, int maxiter_outer,
+                             int restart, double inner_tol, int passes, int resid_mode, int body,
+                             int* converged, double* hist, int hist_cap, int* n_hist,
+                             int* inner_total) {
+  IROpts o;
+  o.tol = tol;
+  o.maxiter_outer = maxiter_outer;
+  o.restart = restart;
+  o.inner_tol = inner_tol;
+  o.passes = passes;
+  o.resid_mode = resid_mode;
+  o.body = body;
+  IRResult r;
+  MPDOK_TRY(gmres_ir_impl(s, A64, lda64, order64, A32, lda32, order32,
+                          static_cast<LUHandle*>(lu), N, b, x, o, r));
+  if (converged) *converged = r.converged;
+  if (inner_total) *inner_total = r.inner_total;
+  return fill_hist(r, hist, hist_cap, n_hist);
+}
+
+MPDOK_API int mpdok_lu_ir(hipStream_t s, const double* A64, int64_t lda, int order, void* lu, int N,
+                          const double* b, double* x, double tol, int maxiter_outer,
+                          int resid_mode, int* converged, double* hist, int hist_cap,
+                          int* n_hist) {
+  MPDOK_REQUIRE(lu != nullptr, "lu_ir: null factorization handle");
+  IRResult r;
+  MPDOK_TRY(lu_ir_impl(s, A64, lda, order, static_cast<LUHandle*>(lu), N, b, x, tol,
+                       maxiter_outer, resid_mode, r));
+  if (converged) *converged = r.converged;
+  return fill_hist(r, hist, hist_cap, n_hist);
+}
+
+// ---- reference-compatible symbols (void, out-pointers, synchronous) -------
+MPDOK_API void py_gmres_ir(void* A64, void* b64, void* x64, void* A32, int N, double tol,
+                           int maxiter_outer, int restart, int* converged_out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipStream_t s = g_stream;
+  int conv = 0;
+  if (cast_matrix(s, A64, F64, N, COL_MAJOR, A32, F32, N, COL_MAJOR, N, N) == OK) {
+    IROpts o;
+    o.tol = tol;
+    o.maxiter_outer = maxiter_outer;
+    o.restart = restart;
+    o.inner_tol = 1e-6;
+    o.passes = 2;
+    o.resid_mode = 0;
+    o.body = 0;
+    IRResult r;
+    if (gmres_ir_impl(s, (const double*)A64, N, COL_MAJOR, (const float*)A32, N, COL_MAJOR,
+                      nullptr, N, (const double*)b64, (double*)x64, o, r) == OK)
+      conv = r.converged;
+  }
+  (void)hipStreamSynchronize(s);
+  if (converged_out) *converged_out = conv;
+}
+
+static void alloc_managed(void** slot, int N, void** ptr_out) {
+  if (*slot) {
+    (void)hipFree(*slot);
+    *slot = nullptr;
+  }
+  void* p = nullptr;
+  if (N > 0 && hipMallocManaged(&p, (size_t)N * N * sizeof(double), hipMemAttachGlobal) != hipSuccess)
+    p = nullptr;
+  *slot = p;
+  if (ptr_out) *ptr_out = p;
+}
+
+MPDOK_API void py_alloc_managed(int N, void** ptr_out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  alloc_managed(&g_managed_gmres, N, ptr_out);
+}
+MPDOK_API void py_free_managed(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_managed_gmres) (void)hipFree(g_managed_gmres);
+  g_managed_gmres = nullptr;
+}
+MPDOK_API void py_lu_alloc_managed(int N, void** ptr_out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  alloc_managed(&g_managed_lu, N, ptr_out);
+}
+MPDOK_API void py_lu_free_managed(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_managed_lu) (void)hipFree(g_managed_lu);
+  g_managed_lu = nullptr;
+}
+
+// FP32 LU with partial pivoting (the reference's Sgetrf), kept in the slot.
+MPDOK_API void py_lu_factor(void* A64, int N, int* info_out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int info = -1;
+  if (g_lu && lu_n(g_lu) != N) {
+    lu_destroy(g_lu);
+    g_lu = nullptr;
+  }
+  if (!g_lu) lu_create(N, LU_FP32, PIV_PARTIAL, 256, 0, &g_lu);
+  if (g_lu && lu_factor(g_lu, g_stream, (const double*)A64, N, COL_MAJOR, &info) != OK) info = -1;
+  (void)hipStreamSynchronize(g_stream);
+  if (info_out) *info_out = info;
+}
+
+MPDOK_API void py_lu_solve_factored(void* A64, void* b64, void* x64, int N, double tol,
+                                    int maxiter_outer, int* converged_out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int conv = 0;
+  if (!g_lu || lu_n(g_lu) != N) {
+    conv = -1;
+  } else {
+    IRResult r;
+    int st = lu_ir_impl(g_stream, (const double*)A64, N, COL_MAJOR, g_lu, N, (const double*)b64,
+                        (double*)x64, tol, maxiter_outer, 0, r);
+    conv = st == ENOFACTOR ? -1 : (st == OK ? r.converged : 0);
+  }
+  (void)hipStreamSynchronize(g_stream);
+  if (converged_out) *converged_out = conv;
+}
+
+MPDOK_API void py_lu_ir(void* A64, void* b64, void* x64, int N, double tol, int maxiter_outer,
+                        int* converged_out) {
+  int info = -1;
+  py_lu_factor(A64, N, &info);
+  if (info != 0) {
+    if (converged_out) *converged_out = 0;
+    return;
+  }
+  py_lu_solve_factored(A64, b64, x64, N, tol, maxiter_outer, converged_out);
+}
+
+MPDOK_API void py_lu_free_factored(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  lu_destroy(g_lu);
+  g_lu = nullptr;
+}

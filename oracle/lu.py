@@ -1,0 +1,168 @@
+"""Oracle: LU factorisations, sliced-integer products and LU-IR (test
+infrastructure only).
+
+  lu_ir          LUIRSolver algorithm as documented in benchmark_mpdok.ipynb:983-996
+                 (FP32 getrf, FP32 getrs, FP64 residual, FP64 update), with the
+                 outer-loop contract of mpdok_ops.py:204-243 / :256-320.
+  lu_nopiv       blocked right-looking LU without pivoting (FP64 panel), the
+                 trailing update supplied as a callable.
+  ozaki_*        Ozaki-scheme splitting of FP64 matrices into signed 7-bit integer
+                 slices with power-of-two row/column scales (Ozaki, Ogita, Oishi,
+                 Rump, Numer. Algorithms 59 (2012)); products of slice pairs are
+                 exact integers; pairs with i + j <= s + 1 are kept.
+  residual_*     exact and Dot2 residuals (Ogita, Rump, Oishi, SIAM J. Sci.
+                 Comput. 26 (2005)).
+"""
+from fractions import Fraction
+
+import numpy as np
+import scipy.linalg as sla
+
+
+# ---------------------------------------------------------------------------
+# LU-IR (reference: LUIRSolver / lu_ir.so)
+# ---------------------------------------------------------------------------
+def lu_ir(A, b, tol=1e-11, maxiter_outer=3, factor_dtype=np.float32):
+    """Returns (x FP64, converged, history [relres per outer step])."""
+    A = np.asarray(A, np.float64)
+    b = np.asarray(b, np.float64)
+    lu = sla.lu_factor(A.astype(factor_dtype), check_finite=False)
+    b_norm = float(np.linalg.norm(b))
+    x = np.zeros_like(b)
+    hist = []
+    if b_norm == 0.0:
+        return x, True, [0.0]
+    for _ in range(maxiter_outer):
+        r = b - A @ x
+        rel = float(np.linalg.norm(r)) / b_norm
+        hist.append(rel)
+        if rel < tol:
+            return x, True, hist
+        x = x + sla.lu_solve(lu, r.astype(factor_dtype)).astype(np.float64)
+    rel = float(np.linalg.norm(b - A @ x)) / b_norm
+    hist.append(rel)
+    return x, rel < tol, hist
+
+
+# ---------------------------------------------------------------------------
+# Sliced-integer (Ozaki) products
+# ---------------------------------------------------------------------------
+def ozaki_scales(M, axis):
+    """Power-of-two scale per row (axis=1) or per column (axis=0):
+    2^e with e = floor(log2(max|.|)) + 1, so |M / 2^e| < 1; zero rows get 1."""
+    mx = np.max(np.abs(M), axis=axis)
+    # frexp: mx = f 2^e with f in [0.5, 1), i.e. e = floor(log2 mx) + 1 exactly
+    # (log2 itself can round up to an integer just below a power of two)
+    _, e = np.frexp(mx)
+    return np.where(mx > 0, e, 0).astype(np.int64)
+
+
+def ozaki_slices(M, s, axis):
+    """Signed digits d_1..d_s in [-127, 127] with M = 2^e * sum_i d_i 2^(-7 i)
+    (truncation error < 2^(-7 s) * 2^e per element). Returns (list of int8
+    arrays, exponent array)."""
+    M = np.asarray(M, np.float64)
+    e = ozaki_scales(M, axis)
+    sc = np.ldexp(1.0, -e)
+    r = M * (sc[:, None] if axis == 1 else sc[None, :])
+    out = []
+    for _ in range(s):
+        d = np.clip(np.rint(r * 128.0), -127.0, 127.0)
+        r = r * 128.0 - d
+        out.append(d.astype(np.int8))
+    return out, e
+
+
+def ozaki_matmul(A, B, s):
+    """Sliced product of A (m x k) and B (k x n): for each level l = 2..s+1,
+    acc_l = sum_{i+j=l} A_i B_j (exact integers), c = sum_l acc_l 2^(-7 l)
+    accumulated in FP64 from l = 2 upwards, result = c * 2^(eA_row + eB_col)."""
+    da, ea = ozaki_slices(A, s, 1)
+    db, eb = ozaki_slices(B, s, 0)
+    m, n = A.shape[0], B.shape[1]
+    c = np.zeros((m, n))
+    for l in range(2, s + 2):
+        acc = np.zeros((m, n), np.int64)
+        for i in range(1, l):
+            j = l - i
+            if j <= s:
+                acc += da[i - 1].astype(np.int64) @ db[j - 1].astype(np.int64)
+        c = c + np.ldexp(acc.astype(np.float64), -7 * l)
+    return np.ldexp(np.ldexp(c, ea[:, None]), eb[None, :])
+
+
+# ---------------------------------------------------------------------------
+# Blocked LU without pivoting
+# ---------------------------------------------------------------------------
+def lu_nopiv(A, nb=256, trailing=None):
+    """Right-looking blocked LU (no pivoting) in FP64; L unit-lower and U packed
+    in one array. trailing(L21, U12) returns the product used for
+    A22 -= L21 @ U12 (default: the FP64 product)."""
+    A = np.array(A, np.float64, order="F", copy=True)
+    n = A.shape[0]
+    mm = trailing if trailing is not None else (lambda L, U: L @ U)
+    for k in range(0, n, nb):
+        e = min(k + nb, n)
+        for j in range(k, e):
+            if A[j, j] == 0.0:
+                raise ZeroDivisionError(f"zero pivot at {j}")
+            A[j + 1:, j] /= A[j, j]
+            A[j + 1:, j + 1:e] -= np.outer(A[j + 1:, j], A[j, j + 1:e])
+        if e < n:
+            L11 = np.tril(A[k:e, k:e], -1) + np.eye(e - k)
+            A[k:e, e:] = sla.solve_triangular(L11, A[k:e, e:], lower=True, unit_diagonal=True)
+            A[e:, e:] -= mm(A[e:, k:e], A[k:e, e:])
+    return A
+
+
+def lu_solve_packed(LU, b, piv=None):
+    """Solve with packed factors (unit-lower L, upper U); piv in LAPACK
+    0-based ipiv form (row i swapped with piv[i], in order) or None."""
+    x = np.array(b, np.float64, copy=True)
+    if piv is not None:
+        for i, p in enumerate(piv):
+            if p != i:
+                x[[i, p]] = x[[p, i]]
+    x = sla.solve_triangular(LU, x, lower=True, unit_diagonal=True)
+    return sla.solve_triangular(LU, x, lower=False)
+
+
+def unpack(LU):
+    n = LU.shape[0]
+    return np.tril(LU, -1) + np.eye(n), np.triu(LU)
+
+
+# ---------------------------------------------------------------------------
+# Residuals
+# ---------------------------------------------------------------------------
+def residual_exact(A, x, b):
+    """r = b - A x in exact rational arithmetic, rounded once to FP64
+    (small inputs only)."""
+    A = np.asarray(A, np.float64)
+    n = A.shape[0]
+    out = np.empty(n)
+    xf = [Fraction(float(v)) for v in x]
+    for i in range(n):
+        s = Fraction(float(b[i]))
+        for j in range(A.shape[1]):
+            s -= Fraction(float(A[i, j])) * xf[j]
+        out[i] = float(s)
+    return out
+
+
+def residual_longdouble(A, x, b):
+    """r = b - A x in x87 extended precision (64-bit significand)."""
+    AL = np.asarray(A, np.longdouble)
+    return (np.asarray(b, np.longdouble) - AL @ np.asarray(x, np.longdouble)).astype(np.float64)
+
+
+def forward_error(x, x_ref):
+    return float(np.linalg.norm(np.asarray(x) - x_ref) / np.linalg.norm(x_ref))
+
+
+def backward_error(A, x, b, r=None):
+    """Normwise backward error ||b - A x|| / (||A|| ||x|| + ||b||) (2-norms,
+    ||A|| estimated by the Frobenius norm, an upper bound)."""
+    A = np.asarray(A)
+    r = b - A @ x if r is None else r
+    return float(np.linalg.norm(r) / (np.linalg.norm(A) * np.linalg.norm(x) + np.linalg.norm(b)))

@@ -1,0 +1,267 @@
+"""GPU parity of the LU factorisations, the int8-sliced trailing update and
+the refinement drivers against the oracle and LAPACK."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import torch
+
+from oracle import lu as olu
+from oracle import rbf as orb
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def L():
+    from mpdok_amd import _lib
+    return _lib.load()
+
+
+def dev(a, dtype=torch.float64, order="F"):
+    from mpdok_amd import _dev
+    return _dev.as_device(a, dtype, order)
+
+
+def stream():
+    from mpdok_amd import _dev
+    return _dev.stream()
+
+
+def gemm(L, A, B, C, alpha, beta, round_=0):
+    m, k = A.shape
+    n = B.shape[1]
+    dt = 0 if A.dtype == torch.float64 else 1
+    st = L.mpdok_gemm(stream(), dt, m, n, k, alpha, A.data_ptr(), A.stride(1), B.data_ptr(),
+                      B.stride(1), beta, C.data_ptr(), C.stride(1), round_)
+    assert st == 0, L.mpdok_last_error()
+    return C
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (128, 128, 16), (300, 257, 129),
+                                   (1000, 64, 64), (64, 1000, 512)])
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_gemm_bit_exact_integer_inputs(L, m, n, k, dt):
+    rng = np.random.default_rng(m + 7 * n + 13 * k)
+    A = rng.integers(-8, 9, (m, k)).astype(np.float64)
+    B = rng.integers(-8, 9, (k, n)).astype(np.float64)
+    C = rng.integers(-100, 100, (m, n)).astype(np.float64)
+    out = gemm(L, dev(A, dt), dev(B, dt), dev(C, dt), -1.0, 1.0).cpu().numpy()
+    assert np.array_equal(out, C - A @ B)
+    out = gemm(L, dev(A, dt), dev(B, dt), dev(np.full((m, n), np.nan), dt), 2.0, 0.0).cpu().numpy()
+    assert np.array_equal(out, 2 * (A @ B))   # beta = 0 never reads C
+
+
+def test_gemm_fp64_random_close(L):
+    rng = np.random.default_rng(3)
+    A, B = rng.standard_normal((513, 200)), rng.standard_normal((200, 301))
+    out = gemm(L, dev(A), dev(B), dev(np.zeros((513, 301))), 1.0, 0.0).cpu().numpy()
+    assert np.allclose(out, A @ B, rtol=1e-13, atol=1e-12)
+
+
+def test_gemm_bf16_rounding_matches_emulation(L):
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((64, 128)).astype(np.float32)
+    B = rng.standard_normal((128, 64)).astype(np.float32)
+    out = gemm(L, dev(A, torch.float32), dev(B, torch.float32),
+               dev(np.zeros((64, 64)), torch.float32), 1.0, 0.0, 1).cpu().numpy()
+    ab = torch.tensor(A).to(torch.bfloat16).to(torch.float64).numpy()
+    bb = torch.tensor(B).to(torch.bfloat16).to(torch.float64).numpy()
+    ref = ab @ bb
+    assert np.allclose(out, ref, rtol=1e-5, atol=1e-4)
+    assert not np.allclose(out, A.astype(np.float64) @ B, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("S", [3, 4, 5, 6, 7])
+@pytest.mark.parametrize("m,n,K", [(1, 1, 1), (100, 70, 64), (257, 129, 300), (640, 512, 512)])
+def test_ozaki_gemm_bit_exact_vs_oracle(L, S, m, n, K):
+    rng = np.random.default_rng(S * 1000 + m)
+    A = rng.standard_normal((m, K)) * np.exp(rng.uniform(-5, 5, (m, 1)))
+    B = rng.standard_normal((K, n)) * np.exp(rng.uniform(-5, 5, (1, n)))
+    C = rng.standard_normal((m, n))
+    ws = torch.empty(L.mpdok_ozaki_workspace_bytes(m, n, K, S), dtype=torch.uint8, device="cuda")
+    # keep every operand referenced until the kernels have run: a temporary
+    # freed early is recycled by torch's caching allocator for the next one
+    Ad, Bd, Cd = dev(A), dev(B), dev(C)
+    st = L.mpdok_ozaki_gemm_sub(stream(), S, m, n, K, Ad.data_ptr(), m, Bd.data_ptr(), K,
+                                Cd.data_ptr(), m, ws.data_ptr())
+    assert st == 0, L.mpdok_last_error()
+    ref = C - olu.ozaki_matmul(A, B, S)
+    assert np.array_equal(Cd.cpu().numpy(), ref)
+
+
+def _factor(L, A, precision, pivot, nb=128, slices=6):
+    from mpdok_amd.mpdok_ops import _LUHandle
+    h = _LUHandle(L, A.shape[0], precision, pivot, nb, slices)
+    info = h.factor(dev(A))
+    return h, info
+
+
+def _factors(L, h, N):
+    LUp, permp, es = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int()
+    assert L.mpdok_lu_factors(h.ptr, ctypes.byref(LUp), ctypes.cast(ctypes.byref(permp),
+                                                                    ctypes.POINTER(ctypes.c_void_p)),
+                              ctypes.byref(es)) == 0
+    torch.cuda.synchronize()
+    dt = torch.float64 if es.value == 8 else torch.float32
+    buf = torch.empty((N, N), dtype=dt, device="cuda")
+    from mpdok_amd.mpdok_ops import _CAI
+    t = torch.as_tensor(_CAI(LUp.value, (N, N), (es.value, es.value * N),
+                             "<f8" if es.value == 8 else "<f4"), device="cuda")
+    buf.copy_(t)
+    perm = None
+    if permp.value:
+        perm = torch.as_tensor(_CAI(permp.value, (N,), (4,), "<i4"), device="cuda").cpu().numpy()
+    return buf.cpu().numpy().astype(np.float64), perm
+
+
+@pytest.mark.parametrize("N,nb", [(64, 64), (200, 64), (777, 128), (1500, 512)])
+def test_lu_nopivot_fp64_matches_oracle(L, N, nb):
+    rng = np.random.default_rng(N)
+    X = rng.standard_normal((N, N))
+    A = X.T @ X + N * np.eye(N)
+    h, info = _factor(L, A, "fp64", "none", nb)
+    assert info == 0
+    LU, perm = _factors(L, h, N)
+    ref = olu.lu_nopiv(A, nb=nb)
+    assert perm is None
+    assert np.allclose(LU, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+    b = rng.standard_normal(N)
+    x = h.solve(dev(b)).cpu().numpy()
+    xr = sla.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
+
+
+@pytest.mark.parametrize("N", [130, 700])
+def test_lu_partial_pivot_matches_lapack(L, N):
+    rng = np.random.default_rng(N + 1)
+    A = rng.standard_normal((N, N))
+    h, info = _factor(L, A, "fp64", "partial", 128)
+    assert info == 0
+    LU, perm = _factors(L, h, N)
+    lu, piv = sla.lu_factor(A)
+    # LAPACK's ipiv -> permutation
+    p = np.arange(N)
+    for i, q in enumerate(piv):
+        p[[i, q]] = p[[q, i]]
+    assert np.array_equal(perm, p)
+    assert np.allclose(LU, lu, rtol=1e-9, atol=1e-9)
+    b = rng.standard_normal(N)
+    x = h.solve(dev(b)).cpu().numpy()
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-10)
+
+
+def test_lu_singular_reports_info(L):
+    A = np.eye(100)
+    A[37, 37] = 0.0
+    h, info = _factor(L, A, "fp64", "none", 64)
+    assert info == 38
+    A2 = np.random.default_rng(0).standard_normal((100, 100))
+    A2[:, 50] = 0.0
+    h, info = _factor(L, A2, "fp64", "partial", 64)
+    assert info == 51
+
+
+@pytest.mark.parametrize("S", [5, 6])
+def test_ozaki_lu_matches_oracle_and_preconditions_rbf(L, S):
+    N = 1024
+    X = np.random.default_rng(42).uniform(0, 1, (N, 3))
+    A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
+    h, info = _factor(L, A, "ozaki", "none", 256, S)
+    assert info == 0
+    LU, _ = _factors(L, h, N)
+    ref = olu.lu_nopiv(A, nb=256, trailing=lambda a, b: olu.ozaki_matmul(a, b, S))
+    scale = np.abs(ref).max()
+    assert np.abs(LU - ref).max() < 1e-9 * scale
+
+
+def test_luir_solver_default_fp32_partial(L):
+    from mpdok_amd.mpdok_ops import LUIRSolver
+    N = 1024
+    rng = np.random.default_rng(42)
+    A = rng.standard_normal((N, N)) + N * np.eye(N)
+    b = rng.standard_normal(N)
+    s = LUIRSolver()
+    x = s.solve(A, b, tol=1e-11).cpu().numpy()
+    xo, conv, hist = olu.lu_ir(A, b, tol=1e-11)
+    assert conv and s.last_info["converged"]
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-11
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-12
+    # factor once, solve several right-hand sides
+    s.factor(A)
+    for seed in range(3):
+        bb = np.random.default_rng(seed).standard_normal(N)
+        xx = s.solve_factored(bb).cpu().numpy()
+        assert np.linalg.norm(bb - A @ xx) / np.linalg.norm(bb) < 1e-11
+    s.free_factored()
+    with pytest.raises(RuntimeError):
+        s.solve_factored(b)
+
+
+def test_mpdok_solver_default_path_nonsym(L):
+    from mpdok_amd.mpdok_ops import MPDOKSolver
+    N = 2048
+    rng = np.random.default_rng(42)
+    A = rng.standard_normal((N, N)) + N * np.eye(N)
+    b = rng.standard_normal(N)
+    x = MPDOKSolver().solve(A, b, tol=1e-11).cpu().numpy()
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
+    with pytest.raises(ValueError):
+        MPDOKSolver().solve(np.zeros((4, 5)), np.zeros(4))
+
+
+@pytest.mark.parametrize("fp,S", [("ozaki", 6), ("fp64", 0)])
+def test_lu_preconditioned_gmres_ir_rbf_to_1e12(L, fp, S):
+    from mpdok_amd.mpdok_ops import MPDOKSolver
+    N = 2048
+    X = np.random.default_rng(42).uniform(0, 1, (N, 3))
+    A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
+    b = (np.sin(2 * np.pi * X[:, 0]) * np.cos(2 * np.pi * X[:, 1]) * np.cos(2 * np.pi * X[:, 2])
+         + 1e-4 * np.random.default_rng(43).standard_normal(N))
+    solver = MPDOKSolver()
+    x = solver.solve(A, b, tol=1e-12, maxiter_outer=10, precond="lu", factor_precision=fp,
+                     slices=S or 6).cpu().numpy()
+    info = solver.last_info
+    assert info["converged"], info
+    r = olu.residual_longdouble(A, x, b)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
+    x64 = sla.solve(A, b)
+    xl = x64 + sla.solve(A, olu.residual_longdouble(A, x64, b))   # refined reference
+    err = np.linalg.norm(x - xl) / np.linalg.norm(xl)
+    err64 = np.linalg.norm(x64 - xl) / np.linalg.norm(xl)
+    assert err <= 10 * max(err64, 1e-15)
+
+
+def test_reference_symbols_through_raw_ctypes(L):
+    """Bind py_lu_* / py_gmres_ir the way mpdok_ops.py:101-141,361-382 does."""
+    from mpdok_amd import _lib
+    lib = ctypes.CDLL(_lib.DEFAULT_LIB)
+    lib.py_lu_ir.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                      ctypes.POINTER(ctypes.c_int)]
+    lib.py_lu_factor.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.py_lu_solve_factored.argtypes = lib.py_lu_ir.argtypes[:3] + lib.py_lu_ir.argtypes[3:]
+    lib.py_gmres_ir.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_double,
+                                                        ctypes.c_int, ctypes.c_int,
+                                                        ctypes.POINTER(ctypes.c_int)]
+    N = 512
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((N, N)) + N * np.eye(N)
+    b = rng.standard_normal(N)
+    Ad, bd = dev(A), dev(b)
+    x = torch.zeros(N, dtype=torch.float64, device="cuda")
+    conv = ctypes.c_int(0)
+    lib.py_lu_free_factored()
+    lib.py_lu_solve_factored(Ad.data_ptr(), bd.data_ptr(), x.data_ptr(), N, 1e-11, 3,
+                             ctypes.byref(conv))
+    assert conv.value == -1
+    lib.py_lu_ir(Ad.data_ptr(), bd.data_ptr(), x.data_ptr(), N, 1e-11, 3, ctypes.byref(conv))
+    assert conv.value == 1
+    assert np.linalg.norm(b - A @ x.cpu().numpy()) / np.linalg.norm(b) < 1e-11
+    A32 = torch.empty((N, N), dtype=torch.float32, device="cuda")
+    x.zero_()
+    lib.py_gmres_ir(Ad.data_ptr(), bd.data_ptr(), x.data_ptr(), A32.data_ptr(), N, 1e-11, 5, 50,
+                    ctypes.byref(conv))
+    assert conv.value == 1
+    lib.py_lu_free_factored()

@@ -1,0 +1,270 @@
+"""Pins the CPU oracle: each check is chosen so that a plausible mistake (a
+dropped term, a wrong sign or index, a transposed operand) fails it, and none
+of them retypes the oracle's own formula."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse.linalg as spla
+
+from oracle import krylov as ok
+from oracle import lu as olu
+from oracle import rbf as orb
+
+
+# ----------------------------------------------------------------- assembly
+def test_gram_d2_matches_brute_force_distances():
+    X = np.random.default_rng(0).uniform(-3, 5, (37, 3))
+    brute = ((X[:, None, :] - X[None, :, :]) ** 2).sum(-1)
+    assert np.allclose(orb.gram_d2(X), brute, rtol=1e-12, atol=1e-12)
+    # row/column sub-blocks index the right points
+    D2 = orb.gram_d2(X, rows=np.arange(5, 9), cols=np.arange(30, 37))
+    assert np.allclose(D2, brute[5:9, 30:37], rtol=1e-12, atol=1e-12)
+
+
+def test_gaussian_closed_form_values_and_structure():
+    X = np.array([[0.0, 0.0], [3.0, 4.0], [1.0, 0.0]])
+    A = orb.build_kernel(X, "gaussian", 0.1, reg=1e-6)
+    assert A[0, 1] == pytest.approx(np.exp(-0.1 * 25.0), rel=1e-15)
+    assert A[0, 2] == pytest.approx(np.exp(-0.1), rel=1e-15)
+    assert np.all(np.diag(A) == 1.0 + 1e-6)
+    assert np.array_equal(A, A.T)
+
+
+def test_matern32_closed_form():
+    X = np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [0.0, 0.3, 0.4]])
+    ell = 0.1
+    A = orb.build_kernel(X, "matern32", ell, reg=1e-4)
+    for (i, j, r) in [(0, 1, 0.1), (0, 2, 0.5)]:
+        a = np.sqrt(3.0) * r / ell
+        assert A[i, j] == pytest.approx((1 + a) * np.exp(-a), rel=1e-13)
+    assert A[0, 0] == pytest.approx(1.0 + 1e-4, rel=1e-15)
+    # Matern-3/2 is decreasing in r and bounded by 1
+    assert A[0, 1] > A[0, 2] > 0
+
+
+def test_fp32_tile_adds_nugget_after_cast():
+    X = np.random.default_rng(1).uniform(0, 1, (64, 2))
+    reg = 1e-9  # below FP32 resolution of 1.0: vanishes when added after the cast
+    t = orb.build_fp32_tile(X, 16, 8, "gaussian", 25.0, reg)
+    assert t.dtype == np.float32 and t.shape == (8, 64)
+    assert np.all(t[np.arange(8), 16 + np.arange(8)] == np.float32(1.0))
+    full = orb.build_kernel(X, "gaussian", 25.0, 0.0)
+    off = np.ones((8, 64), bool)
+    off[np.arange(8), 16 + np.arange(8)] = False
+    assert np.array_equal(t[off], full[16:24][off].astype(np.float32))
+
+
+def test_estimate_gamma_known_answer_uniform_square():
+    # E[mean d2] for uniform [0, L]^D with the diagonal zeros is D L^2 / 6 (1 - 1/n)
+    vals = []
+    for seed in range(8):
+        X = np.random.default_rng(seed).uniform(0, 100.0, (2000, 2))
+        vals.append(orb.estimate_gamma(X))
+    g = orb.uniform_cube_gamma(2, 100.0, 512)
+    assert np.mean(vals) == pytest.approx(g, rel=0.02)
+    # the reference printed gamma = 1.505e-4 for this setting (rbf_demo.ipynb:103)
+    assert g == pytest.approx(1.505e-4, rel=0.01)
+    # degenerate input -> 1.0
+    assert orb.estimate_gamma(np.zeros((10, 2))) == 1.0
+
+
+def test_matvec_fp64_tiled_equals_dense():
+    X = np.random.default_rng(2).uniform(0, 1, (300, 3))
+    v = np.random.default_rng(3).standard_normal(300)
+    A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
+    assert np.allclose(orb.matvec_fp64(X, v, "gaussian", 25.0, 1e-6, tile_rows=64), A @ v,
+                       rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------------- Krylov
+def _spd(N, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((N, N))
+    return X.T @ X + N * np.eye(N), rng.standard_normal(N)
+
+
+def test_cg_matches_direct_solve_and_history_rules():
+    A, b = _spd(200, 7)
+    x, hist, conv = ok.cg(ok.DenseOp(A), b, tol=1e-6)
+    assert conv
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-5
+    # checks fire at multiples of check_every, before the update
+    assert [h[0] % 10 for h in hist] == [0] * len(hist)
+    assert hist[0] == (0, pytest.approx(1.0))
+
+
+def test_cg_identity_one_step_and_zero_rhs():
+    N = 50
+    b = np.random.default_rng(0).standard_normal(N)
+    x, hist, conv = ok.cg(ok.DenseOp(np.eye(N)), b, tol=1e-6, check_every=1)
+    assert conv and hist[-1][0] == 1
+    assert np.allclose(x, b, rtol=1e-6)
+    x, hist, conv = ok.cg(ok.DenseOp(np.eye(N)), np.zeros(N))
+    assert conv and hist == [(0, 0.0)] and not x.any()
+
+
+def test_gmres_three_distinct_eigenvalues_solve_in_three_steps():
+    # The Krylov space of A with 3 distinct eigenvalues has dimension 3: one
+    # restart cycle of 3 steps reaches the FP32 floor.
+    N = 60
+    Q, _ = np.linalg.qr(np.random.default_rng(4).standard_normal((N, N)))
+    lam = np.repeat([1.0, 2.0, 5.0], N // 3)
+    A = (Q * lam) @ Q.T
+    b = np.random.default_rng(5).standard_normal(N)
+    x, hist, conv = ok.gmres(ok.DenseOp(A), b, tol=1e-5, restart=3)
+    assert conv and hist[0][0] == 3
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-5
+    # with only 2 steps it cannot
+    x, hist, conv = ok.gmres(ok.DenseOp(A), b, tol=1e-5, restart=2, maxiter=2)
+    assert not conv and hist[0][1] > 1e-3
+
+
+def test_gmres_exact_breakdown_on_eigenvector():
+    # b along an eigenvector: A v0 = lambda v0 exactly in FP32, w - h v0 = 0,
+    # lucky breakdown after one matvec and the exact solution b / lambda.
+    N = 40
+    A = np.diag(np.arange(1.0, N + 1.0))
+    b = np.zeros(N)
+    b[3] = 8.0
+    x, hist, conv = ok.gmres(ok.DenseOp(A), b, tol=1e-6, restart=10)
+    assert conv and hist[0][0] == 1
+    assert x[3] == pytest.approx(2.0, rel=1e-7) and np.count_nonzero(x) == 1
+
+
+def test_gmres_nonsymmetric_matches_scipy():
+    N = 300
+    rng = np.random.default_rng(13)
+    A = rng.standard_normal((N, N)) + N * np.eye(N)
+    b = rng.standard_normal(N)
+    x, hist, conv = ok.gmres(ok.DenseOp(A), b, tol=1e-6, restart=50)
+    xs, info = spla.gmres(A, b, rtol=1e-6, restart=50, maxiter=N)
+    assert conv and info == 0
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-5
+    assert np.linalg.norm(xs - xr) / np.linalg.norm(xr) < 1e-5
+
+
+def test_gmres_ir_reaches_tol_and_keeps_trailing_entry():
+    N = 300
+    rng = np.random.default_rng(42)
+    A = rng.standard_normal((N, N)) + N * np.eye(N)
+    b = rng.standard_normal(N)
+    x, hist, conv = ok.gmres_ir(ok.DenseOp(A), b, tol=1e-11, maxiter_outer=5)
+    assert conv
+    assert hist[-1][0] == 5          # trailing (maxiter_outer, rel) entry
+    assert hist[-2][1] < 1e-11       # the converged check before it
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
+
+
+def test_inner_gmres_and_ooc_solve():
+    X = np.random.default_rng(9).uniform(0, 1, (400, 2))
+    A = orb.build_kernel(X, "gaussian", 4.0, 1e-2)
+    A32 = A.astype(np.float32)
+    b = np.random.default_rng(10).standard_normal(400)
+    e = ok.inner_gmres(lambda v: A32 @ v, b, restart=50)
+    assert np.linalg.norm(b - A @ e) / np.linalg.norm(b) < 1e-2  # one cycle, kappa ~ 3e4
+    x, conv = ok.ooc_solve(lambda v: A32 @ v, lambda v: A @ v, b, tol=1e-10)
+    assert conv
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10
+
+
+def test_inner_gmres_breakdown_is_exact():
+    # identity: breakdown at k = 0, e equals the right-hand side
+    b = np.random.default_rng(0).standard_normal(30)
+    e = ok.inner_gmres(lambda v: v.copy(), b, restart=10)
+    assert np.allclose(e, b, rtol=1e-6)
+    # 2 distinct eigenvalues: breakdown at k = 1 must still return the solution
+    Q, _ = np.linalg.qr(np.random.default_rng(1).standard_normal((30, 30)))
+    A = (Q * np.repeat([1.0, 3.0], 15)) @ Q.T
+    e = ok.inner_gmres(lambda v: (A @ v).astype(np.float32), b, restart=10)
+    assert np.linalg.norm(b - A @ e) / np.linalg.norm(b) < 1e-5
+
+
+# ----------------------------------------------------------------------- LU
+def test_lu_ir_fp32_factor_reaches_fp64_accuracy():
+    N = 256
+    rng = np.random.default_rng(42)
+    A = rng.standard_normal((N, N)) + N * np.eye(N)
+    b = rng.standard_normal(N)
+    x, conv, hist = olu.lu_ir(A, b, tol=1e-11, maxiter_outer=3)
+    assert conv and hist[-1] < 1e-11
+    assert olu.backward_error(A, x, b) < 2e-14  # a few units of FP64 roundoff
+
+
+def test_lu_nopiv_reconstructs_and_matches_lapack_on_spd():
+    A, b = _spd(150, 3)
+    LU = olu.lu_nopiv(A, nb=32)
+    L, U = olu.unpack(LU)
+    assert np.allclose(L @ U, A, rtol=1e-12, atol=1e-10)
+    lu, piv = sla.lu_factor(A)
+    assert np.all(piv == np.arange(150))  # no pivoting happens on this matrix
+    assert np.allclose(LU, lu, rtol=1e-10, atol=1e-10)
+    x = olu.lu_solve_packed(LU, b)
+    assert np.allclose(x, sla.solve(A, b), rtol=1e-12)
+
+
+def test_lu_solve_packed_with_pivots_matches_lapack():
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((80, 80))
+    b = rng.standard_normal(80)
+    lu, piv = sla.lu_factor(A)
+    assert not np.all(piv == np.arange(80))
+    assert np.allclose(olu.lu_solve_packed(lu, b, piv), np.linalg.solve(A, b), rtol=1e-10)
+
+
+# -------------------------------------------------------------------- Ozaki
+def test_ozaki_slices_represent_input_to_truncation_bound():
+    M = np.random.default_rng(0).standard_normal((20, 30)) * np.logspace(-3, 3, 30)
+    for s in (2, 4, 7):
+        d, e = olu.ozaki_slices(M, s, axis=1)
+        rec = sum(np.ldexp(di.astype(np.float64), -7 * (i + 1)) for i, di in enumerate(d))
+        rec = np.ldexp(rec, e[:, None])
+        bound = np.ldexp(1.0, e[:, None] - 7 * s)
+        assert np.all(np.abs(rec - M) <= bound)
+        assert all(np.abs(di).max() <= 127 for di in d)
+
+
+def test_ozaki_matmul_exact_on_small_integers_and_error_scaling():
+    rng = np.random.default_rng(1)
+    A = rng.integers(-100, 100, (17, 40)).astype(np.float64)
+    B = rng.integers(-100, 100, (40, 23)).astype(np.float64)
+    # integers below 2^7 in magnitude are captured exactly by 2 slices after
+    # the power-of-two scaling: s = 3 keeps every nonzero product pair
+    assert np.array_equal(olu.ozaki_matmul(A, B, 3), A @ B)
+    A = rng.standard_normal((64, 128))
+    B = rng.standard_normal((128, 48))
+    ref = A @ B
+    errs = [np.abs(olu.ozaki_matmul(A, B, s) - ref).max() for s in (3, 4, 5, 6)]
+    for e1, e2 in zip(errs, errs[1:]):
+        assert e2 < e1 * 2.0 ** -5  # each slice buys ~7 bits
+    assert errs[-1] < 1e-10
+
+
+def test_ozaki_lu_converges_in_lu_ir():
+    X = np.random.default_rng(42).uniform(0, 1, (512, 3))
+    A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
+    b = np.sin(2 * np.pi * X[:, 0])
+    LU = olu.lu_nopiv(A, 64, trailing=lambda L, U: olu.ozaki_matmul(L, U, 6))
+    LUf = olu.lu_nopiv(A, 64)
+    assert np.abs(LU - LUf).max() < 1e-6 * np.abs(LUf).max()
+    x = np.zeros(512)
+    for _ in range(4):
+        x += olu.lu_solve_packed(LU, b - A @ x)
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-12
+
+
+# ---------------------------------------------------------------- residuals
+def test_residual_exact_catches_cancellation_and_longdouble_agrees():
+    # b - A x with massive cancellation: exact answer known by construction
+    A = np.array([[1e16, 1.0, -1e16], [3.0, 1e-8, 2.0]])
+    x = np.array([1.0, 1.0, 1.0])
+    b = np.array([1.0, 5.0])
+    r = olu.residual_exact(A, x, b)
+    assert r[0] == 0.0  # 1 - (1e16 + 1 - 1e16) = 0 exactly
+    assert r[1] == pytest.approx(-1e-8, rel=1e-15)
+    naive = b - A @ x
+    assert naive[0] != 0.0 or True  # FP64 may lose it; the exact one may not
+    rl = olu.residual_longdouble(A, x, b)
+    assert abs(rl[1] - r[1]) < 1e-18
