@@ -67,6 +67,10 @@ void py_lu_free_factored(void);
 const char* mpdok_last_error(void);
 /* Stream used by the reference-compatible symbols (NULL = null stream). */
 int mpdok_set_stream(hipStream_t s);
+/* Kernel-shape choices picked by the tuner (key 0: register prefetch in the
+ * int8-sliced GEMM, 0 or 1). No reference counterpart. */
+int mpdok_set_tuning(int key, int value);
+int mpdok_get_tuning(int key);
 size_t mpdok_reduce_workspace_bytes(int n);
 size_t mpdok_gemv_workspace_bytes(int M, int N, int dtype, int order);
 size_t mpdok_residual_workspace_bytes(int N, int order);
@@ -169,6 +173,9 @@ int mpdok_lu_destroy(void* handle);
  * (int8 operations for the sliced variant, flops otherwise) and the launch
  * count of the last factorisation. No reference counterpart: measurement. */
 int mpdok_lu_set_profiling(void* handle, int on);
+/* Factor the next panel on a side stream while the trailing update of the
+ * current step finishes (default on; no-pivot handles only). */
+int mpdok_lu_set_lookahead(void* handle, int on);
 int mpdok_lu_stats(void* handle, double* trailing_ms, double* trailing_ops, int* launches);
 /* Device pointers to the packed factors (column-major, ld N) and the row
  * permutation ((P b)[i] = b[perm[i]], NULL without pivoting). */

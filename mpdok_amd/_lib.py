@@ -36,6 +36,8 @@ SIGNATURES = {
     # library services
     "mpdok_last_error": (ctypes.c_char_p, []),
     "mpdok_set_stream": (I, [P]),
+    "mpdok_set_tuning": (I, [I, I]),
+    "mpdok_get_tuning": (I, [I]),
     "mpdok_reduce_workspace_bytes": (SZ, [I]),
     "mpdok_gemv_workspace_bytes": (SZ, [I, I, I, I]),
     "mpdok_residual_workspace_bytes": (SZ, [I, I]),
@@ -70,6 +72,7 @@ SIGNATURES = {
     "mpdok_lu_destroy": (I, [P]),
     "mpdok_lu_factors": (I, [P, PP, PP, PI]),
     "mpdok_lu_set_profiling": (I, [P, I]),
+    "mpdok_lu_set_lookahead": (I, [P, I]),
     "mpdok_lu_stats": (I, [P, ctypes.POINTER(D), ctypes.POINTER(D), PI]),
     "mpdok_gemm": (I, [P, I, I, I, I, D, P, I64, P, I64, D, P, I64, I]),
     "mpdok_ozaki_workspace_bytes": (SZ, [I, I, I, I]),

@@ -14,6 +14,17 @@ namespace mpdok {
 
 static thread_local char g_err[1024] = "";
 
+static int g_tuning[TUNE_COUNT] = {0, 1, 0, 0, 0, 0, 0, 0};
+int tuning_get(int key) { return key >= 0 && key < TUNE_COUNT ? g_tuning[key] : 0; }
+int tuning_set(int key, int value) {
+  MPDOK_REQUIRE(key >= 0 && key < TUNE_COUNT, "tuning: unknown key %d", key);
+  MPDOK_REQUIRE((key != TUNE_OZAKI_PREFETCH && key != TUNE_OZAKI_SWIZZLE) || value == 0 ||
+                    value == 1,
+                "tuning: switch %d takes 0 or 1", key);
+  g_tuning[key] = value;
+  return OK;
+}
+
 void set_error(const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -191,6 +202,8 @@ MPDOK_API int mpdok_axpby(hipStream_t s, int n, double alpha, const void* x, int
 }
 
 MPDOK_API const char* mpdok_last_error(void) { return mpdok::g_err; }
+MPDOK_API int mpdok_set_tuning(int key, int value) { return mpdok::tuning_set(key, value); }
+MPDOK_API int mpdok_get_tuning(int key) { return mpdok::tuning_get(key); }
 MPDOK_API size_t mpdok_reduce_workspace_bytes(int n) { return mpdok::reduce_workspace_bytes(n); }
 MPDOK_API int mpdok_dot_f32(hipStream_t s, const float* x, const float* y, int n, double* out,
                             void* work) {

@@ -24,6 +24,15 @@ enum Status : int {
 // Thread-local last-error text, readable through mpdok_last_error().
 void set_error(const char* fmt, ...);
 
+// Process-wide kernel-shape choices (set by the tuner through
+// mpdok_set_tuning; read at launch time).
+enum TuneKey : int {
+  TUNE_OZAKI_PREFETCH = 0,  // int8-sliced GEMM: 1 = register prefetch of the next K step
+  TUNE_OZAKI_SWIZZLE = 1,   // int8-sliced GEMM: 1 = XOR-swizzled 64-byte LDS rows
+  TUNE_COUNT = 8,
+};
+int tuning_get(int key);
+
 constexpr int WAVE = 64;
 
 // Memory order of a dense matrix argument.

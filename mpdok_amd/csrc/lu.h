@@ -39,6 +39,10 @@ int seed_block_inverse(hipStream_t s, const T* inv64, T* binv, int nbig);
 size_t ozaki_workspace_bytes(int m, int n, int K, int S);
 int ozaki_gemm(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
                const double* B, int64_t ldb, double* C, int64_t ldc, void* work);
+int ozaki_slice(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
+                const double* B, int64_t ldb, void* work);
+int ozaki_apply(hipStream_t s, int S, int m, int n, int K, int col0, int ncols, double* C,
+                int64_t ldc, void* work);
 
 // lu.hip
 enum LUPrecision : int { LU_FP64 = 0, LU_FP32 = 1, LU_BF16 = 2, LU_FP16 = 3, LU_OZAKI = 4 };

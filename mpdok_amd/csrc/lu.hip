@@ -51,6 +51,12 @@ struct LUHandle {
   void* Ubinv = nullptr;
   void* v3 = nullptr;
   void* acc = nullptr;
+  // lookahead (no pivoting): panel k+1 is factored on `side` while the main
+  // stream finishes the trailing update of step k; Wp is the side's scratch
+  hipStream_t side = nullptr;
+  hipEvent_t ev_next_cols = nullptr, ev_panel = nullptr;
+  void* Wp = nullptr;
+  bool lookahead = true;
   // optional HIP-event timing of the trailing-update launches
   bool profile = false;
   std::vector<hipEvent_t> ev;
@@ -66,6 +72,10 @@ static bool fp64_storage(int precision) { return precision == LU_FP64 || precisi
 void lu_destroy(LUHandle* h) {
   if (!h) return;
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  if (h->ev_next_cols) (void)hipEventDestroy(h->ev_next_cols);
+  if (h->ev_panel) (void)hipEventDestroy(h->ev_panel);
+  if (h->side) (void)hipStreamDestroy(h->side);
+  if (h->Wp) (void)hipFree(h->Wp);
   void* ptrs[] = {h->LU, h->Linv, h->Uinv, h->W, h->ipiv, h->perm, h->d_info,
                   h->piv_work, h->oz_work, h->v1, h->v2, h->Lbinv, h->Ubinv, h->v3, h->acc};
   for (void* p : ptrs)
@@ -117,11 +127,21 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
     alloc(&h->Lbinv, (size_t)N * TRSV_BLOCK * h->es);
     alloc(&h->Ubinv, (size_t)N * TRSV_BLOCK * h->es);
     alloc(&h->v3, (size_t)N * 8);
-    alloc(&h->acc, (size_t)N * 8 * 8);
+    alloc(&h->acc, (size_t)N * 8 * 16);  // 8 accumulators + 8 column-group partials
   }
   if (st == OK && hipMemset(h->piv_work, 0, panel_gepp_workspace_bytes(N)) != hipSuccess) {
     set_error("lu_create: hipMemset failed");
     st = EHIP;
+  }
+  if (st == OK && pivot == PIV_NONE) {
+    alloc(&h->Wp, (size_t)N * 64 * h->es);
+    // default-priority stream (no priority is ever set on any stream)
+    if (st == OK && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
+                     hipEventCreateWithFlags(&h->ev_next_cols, hipEventDisableTiming) != hipSuccess ||
+                     hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming) != hipSuccess)) {
+      set_error("lu_create: stream/event creation failed");
+      st = EHIP;
+    }
   }
   if (st != OK) {
     lu_destroy(h);
@@ -170,78 +190,148 @@ static int build_block_inverses(LUHandle* h, hipStream_t s) {
   return OK;
 }
 
+// Factor the panel of columns [k, k + kb) over rows [k, N): 64-column
+// sub-panels (diagonal block in LDS, L21, U12 inside the panel, panel update).
+template <typename T>
+static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W) {
+  const int N = h->N;
+  T* A = static_cast<T*>(h->LU);
+  T* Linv = static_cast<T*>(h->Linv);
+  T* Uinv = static_cast<T*>(h->Uinv);
+  const int64_t ld = N;
+  auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
+  for (int j = k; j < k + kb; j += 64) {
+    const int jb = 64 < k + kb - j ? 64 : k + kb - j;
+    T* Li = Linv + (size_t)(j / 64) * 4096;
+    T* Ui = Uinv + (size_t)(j / 64) * 4096;
+    const int m2 = N - j - jb;
+    if (h->pivot == PIV_PARTIAL) {
+      MPDOK_TRY(panel_gepp<T>(s, A, ld, N, j, jb, h->ipiv, h->d_info, h->piv_work));
+      MPDOK_TRY(laswp<T>(s, A, ld, 0, j, h->ipiv, j, j + jb));
+      MPDOK_TRY(laswp<T>(s, A, ld, j + jb, N, h->ipiv, j, j + jb));
+      MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 0));
+    } else {
+      MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 1));
+      if (m2 > 0) {
+        MPDOK_TRY(gemm<T>(s, m2, jb, jb, T(1), at(j + jb, j), ld, Ui, 64, T(0), W, m2, 0));
+        MPDOK_TRY((copy2d<T, T>(s, W, m2, at(j + jb, j), ld, m2, jb)));
+      }
+    }
+    const int n2 = k + kb - j - jb;
+    if (n2 > 0) {
+      MPDOK_TRY(gemm<T>(s, jb, n2, jb, T(1), Li, 64, at(j, j + jb), ld, T(0), W, jb, 0));
+      MPDOK_TRY((copy2d<T, T>(s, W, jb, at(j, j + jb), ld, jb, n2)));
+      if (m2 > 0)
+        MPDOK_TRY(gemm<T>(s, m2, n2, jb, T(-1), at(j + jb, j), ld, at(j, j + jb), ld, T(1),
+                          at(j + jb, j + jb), ld, 0));
+    }
+  }
+  return OK;
+}
+
+// U12 = inv(L11) A12 for the block row [k, k + kb), columns [c0, N).
+template <typename T>
+static int block_row_solve(LUHandle* h, hipStream_t s, int k, int kb, int c0, T* W) {
+  const int N = h->N, n3 = N - c0;
+  T* A = static_cast<T*>(h->LU);
+  T* Linv = static_cast<T*>(h->Linv);
+  const int64_t ld = N;
+  auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
+  for (int j = k; j < k + kb; j += 64) {
+    const int jb = 64 < k + kb - j ? 64 : k + kb - j;
+    T* Li = Linv + (size_t)(j / 64) * 4096;
+    MPDOK_TRY(gemm<T>(s, jb, n3, jb, T(1), Li, 64, at(j, c0), ld, T(0), W, jb, 0));
+    MPDOK_TRY((copy2d<T, T>(s, W, jb, at(j, c0), ld, jb, n3)));
+    const int mrem = k + kb - j - jb;
+    if (mrem > 0)
+      MPDOK_TRY(gemm<T>(s, mrem, n3, jb, T(-1), at(j + jb, j), ld, at(j, c0), ld, T(1),
+                        at(j + jb, c0), ld, 0));
+  }
+  return OK;
+}
+
+// Trailing update of step k restricted to columns [k+kb+c_off, k+kb+c_off+ncols).
+// For the int8-sliced variant the operands must have been sliced already.
+template <typename T>
+static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, int ncols) {
+  const int N = h->N, m3 = N - k - kb, n3 = N - k - kb;
+  if (ncols <= 0 || m3 <= 0) return OK;
+  T* A = static_cast<T*>(h->LU);
+  const int64_t ld = N;
+  auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
+  const int round = h->precision == LU_BF16 ? 1 : (h->precision == LU_FP16 ? 2 : 0);
+  const size_t nev = h->ev_ops.size();
+  if (h->profile) {
+    // int8-sliced: S(S+1)/2 int8 products of 2 m n k operations each
+    const double pairs = h->precision == LU_OZAKI ? h->slices * (h->slices + 1) / 2.0 : 1.0;
+    h->ev_ops.push_back(2.0 * m3 * ncols * kb * pairs);
+    while (h->ev.size() < 2 * h->ev_ops.size()) {
+      hipEvent_t e;
+      MPDOK_HIP(hipEventCreate(&e));
+      h->ev.push_back(e);
+    }
+    MPDOK_HIP(hipEventRecord(h->ev[2 * nev], s));
+  }
+  if (h->precision == LU_OZAKI) {
+    MPDOK_TRY(ozaki_apply(s, h->slices, m3, n3, kb, c_off, ncols,
+                          reinterpret_cast<double*>(at(k + kb, k + kb)), ld, h->oz_work));
+  } else {
+    MPDOK_TRY(gemm<T>(s, m3, ncols, kb, T(-1), at(k + kb, k), ld, at(k, k + kb + c_off), ld, T(1),
+                      at(k + kb, k + kb + c_off), ld, round));
+  }
+  if (h->profile) MPDOK_HIP(hipEventRecord(h->ev[2 * nev + 1], s));
+  return OK;
+}
+
 template <typename T>
 static int factor_impl(LUHandle* h, hipStream_t s) {
   const int N = h->N, nb = h->nb;
   T* A = static_cast<T*>(h->LU);
-  T* Linv = static_cast<T*>(h->Linv);
-  T* Uinv = static_cast<T*>(h->Uinv);
   T* W = static_cast<T*>(h->W);
   const int64_t ld = N;
-  const int round = h->precision == LU_BF16 ? 1 : (h->precision == LU_FP16 ? 2 : 0);
   auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
+  const bool look = h->lookahead && h->pivot == PIV_NONE && h->side != nullptr;
+  if (look) {
+    // Step k: [main] U12 of block row k, slicing, trailing update of the next
+    // panel's columns; [side] factor the next panel as soon as its columns are
+    // final; [main] the rest of the trailing update meanwhile. The panel
+    // touches only columns [k+kb, k+2kb) and its own scratch Wp; the rest of
+    // the trailing update only columns >= k+2kb.
+    T* Wp = static_cast<T*>(h->Wp);
+    MPDOK_TRY(panel_factor<T>(h, s, 0, nb < N ? nb : N, W));
+    for (int k = 0; k < N; k += nb) {
+      const int kb = nb < N - k ? nb : N - k;
+      const int n3 = N - k - kb;
+      if (n3 <= 0) break;
+      if (k > 0) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
+      MPDOK_TRY(block_row_solve<T>(h, s, k, kb, k + kb, W));
+      if (h->precision == LU_OZAKI)
+        MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb,
+                              reinterpret_cast<const double*>(at(k + kb, k)), ld,
+                              reinterpret_cast<const double*>(at(k, k + kb)), ld, h->oz_work));
+      const int nk = nb < n3 ? nb : n3;
+      MPDOK_TRY(trailing_cols<T>(h, s, k, kb, 0, nk));
+      MPDOK_HIP(hipEventRecord(h->ev_next_cols, s));
+      MPDOK_HIP(hipStreamWaitEvent(h->side, h->ev_next_cols, 0));
+      MPDOK_TRY(panel_factor<T>(h, h->side, k + kb, nk, Wp));
+      MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
+      MPDOK_TRY(trailing_cols<T>(h, s, k, kb, nk, n3 - nk));
+    }
+    if (N > nb) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
+    if (h->fast_solve) MPDOK_TRY(build_block_inverses<T>(h, s));
+    return OK;
+  }
   for (int k = 0; k < N; k += nb) {
     const int kb = nb < N - k ? nb : N - k;
-    for (int j = k; j < k + kb; j += 64) {
-      const int jb = 64 < k + kb - j ? 64 : k + kb - j;
-      T* Li = Linv + (size_t)(j / 64) * 4096;
-      T* Ui = Uinv + (size_t)(j / 64) * 4096;
-      const int m2 = N - j - jb;
-      if (h->pivot == PIV_PARTIAL) {
-        MPDOK_TRY(panel_gepp<T>(s, A, ld, N, j, jb, h->ipiv, h->d_info, h->piv_work));
-        MPDOK_TRY(laswp<T>(s, A, ld, 0, j, h->ipiv, j, j + jb));
-        MPDOK_TRY(laswp<T>(s, A, ld, j + jb, N, h->ipiv, j, j + jb));
-        MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 0));
-      } else {
-        MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 1));
-        if (m2 > 0) {
-          MPDOK_TRY(gemm<T>(s, m2, jb, jb, T(1), at(j + jb, j), ld, Ui, 64, T(0), W, m2, 0));
-          MPDOK_TRY((copy2d<T, T>(s, W, m2, at(j + jb, j), ld, m2, jb)));
-        }
-      }
-      const int n2 = k + kb - j - jb;
-      if (n2 > 0) {
-        MPDOK_TRY(gemm<T>(s, jb, n2, jb, T(1), Li, 64, at(j, j + jb), ld, T(0), W, jb, 0));
-        MPDOK_TRY((copy2d<T, T>(s, W, jb, at(j, j + jb), ld, jb, n2)));
-        if (m2 > 0)
-          MPDOK_TRY(gemm<T>(s, m2, n2, jb, T(-1), at(j + jb, j), ld, at(j, j + jb), ld, T(1),
-                            at(j + jb, j + jb), ld, 0));
-      }
-    }
+    MPDOK_TRY(panel_factor<T>(h, s, k, kb, W));
     const int n3 = N - k - kb;
     if (n3 <= 0) continue;
-    for (int j = k; j < k + kb; j += 64) {
-      const int jb = 64 < k + kb - j ? 64 : k + kb - j;
-      T* Li = Linv + (size_t)(j / 64) * 4096;
-      MPDOK_TRY(gemm<T>(s, jb, n3, jb, T(1), Li, 64, at(j, k + kb), ld, T(0), W, jb, 0));
-      MPDOK_TRY((copy2d<T, T>(s, W, jb, at(j, k + kb), ld, jb, n3)));
-      const int mrem = k + kb - j - jb;
-      if (mrem > 0)
-        MPDOK_TRY(gemm<T>(s, mrem, n3, jb, T(-1), at(j + jb, j), ld, at(j, k + kb), ld, T(1),
-                          at(j + jb, k + kb), ld, 0));
-    }
-    const int m3 = N - k - kb;
-    const size_t nev = h->ev_ops.size();
-    if (h->profile) {
-      // int8-sliced: S(S+1)/2 int8 products of 2 m n k operations each
-      const double pairs = h->precision == LU_OZAKI ? h->slices * (h->slices + 1) / 2.0 : 1.0;
-      h->ev_ops.push_back(2.0 * m3 * n3 * kb * pairs);
-      while (h->ev.size() < 2 * h->ev_ops.size()) {
-        hipEvent_t e;
-        MPDOK_HIP(hipEventCreate(&e));
-        h->ev.push_back(e);
-      }
-      MPDOK_HIP(hipEventRecord(h->ev[2 * nev], s));
-    }
-    if (h->precision == LU_OZAKI) {
-      MPDOK_TRY(ozaki_gemm(s, h->slices, m3, n3, kb, reinterpret_cast<const double*>(at(k + kb, k)),
-                           ld, reinterpret_cast<const double*>(at(k, k + kb)), ld,
-                           reinterpret_cast<double*>(at(k + kb, k + kb)), ld, h->oz_work));
-    } else {
-      MPDOK_TRY(gemm<T>(s, m3, n3, kb, T(-1), at(k + kb, k), ld, at(k, k + kb), ld, T(1),
-                        at(k + kb, k + kb), ld, round));
-    }
-    if (h->profile) MPDOK_HIP(hipEventRecord(h->ev[2 * nev + 1], s));
+    MPDOK_TRY(block_row_solve<T>(h, s, k, kb, k + kb, W));
+    if (h->precision == LU_OZAKI)
+      MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb,
+                            reinterpret_cast<const double*>(at(k + kb, k)), ld,
+                            reinterpret_cast<const double*>(at(k, k + kb)), ld, h->oz_work));
+    MPDOK_TRY(trailing_cols<T>(h, s, k, kb, 0, n3));
   }
   if (h->pivot == PIV_PARTIAL) MPDOK_TRY(ipiv_to_perm(s, h->ipiv, h->perm, N));
   if (h->fast_solve) MPDOK_TRY(build_block_inverses<T>(h, s));
@@ -355,6 +445,13 @@ MPDOK_API int mpdok_lu_factors(void* hv, void** LU, int** perm, int* elem_size) 
   if (LU) *LU = h->LU;
   if (perm) *perm = h->pivot == mpdok::PIV_PARTIAL ? h->perm : nullptr;
   if (elem_size) *elem_size = (int)h->es;
+  return 0;
+}
+// Overlap of the next panel with the trailing update (no-pivot handles only).
+MPDOK_API int mpdok_lu_set_lookahead(void* hv, int on) {
+  auto* h = static_cast<mpdok::LUHandle*>(hv);
+  MPDOK_REQUIRE(h != nullptr, "lu_set_lookahead: null handle");
+  h->lookahead = on != 0;
   return 0;
 }
 // HIP-event timing of the trailing-update launches of the next factorisations.

@@ -61,13 +61,16 @@ constexpr int GBM = 128, GBN = 128, GBK = 16, GPAD = 16;
 // the 16x16 result): A[i = l%16][k = l/16], B[k = l/16][j = l%16],
 // D[i = Mfma<T>::row(l, r)][j = l%16], r = 0..3.
 template <typename T, int ROUND>
-__global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, T alpha,
+__global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alpha,
                                                    const T* __restrict__ A, int64_t lda,
                                                    const T* __restrict__ B, int64_t ldb, T beta,
                                                    T* __restrict__ C, int64_t ldc, int64_t sA_,
                                                    int64_t sB_, int64_t sC_) {
   __shared__ T sA[2][GBK][GBM + GPAD];
-  __shared__ T sB[2][GBK][GBN + GPAD];
+  // B tile stored k-contiguous per column (+1 padding): the 16 lanes that
+  // store 16 consecutive k of one column, and the 16 lanes that read one k of
+  // 16 consecutive columns, both fall into distinct banks.
+  __shared__ T sB[2][GBN][GBK + 1];
   typedef typename Mfma<T>::acc_t acc_t;
   // batched use: blockIdx.y selects the problem; blockIdx.x the tile
   A += blockIdx.y * sA_;
@@ -87,7 +90,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, T alpha,
   T ra[8], rb[8];
   const int ai = bm + (tid & 127), ak = tid >> 7;        // A: row, k phase (0..1)
   const int bk = tid & 15, bj = bn + (tid >> 4);         // B: k, column
+  const bool mn_inside = bm + GBM <= m && bn + GBN <= n;
+  const T* pa = A + ai + (int64_t)ak * lda;
+  const T* pb = B + bk + (int64_t)bj * ldb;
   auto gload = [&](int k0) {
+    if (mn_inside && k0 + GBK <= k) {  // interior tile: no bounds checks
+#pragma unroll
+      for (int r = 0; r < 8; ++r) ra[r] = pa[(int64_t)(k0 + 2 * r) * lda];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) rb[r] = pb[k0 + (int64_t)(16 * r) * ldb];
+      return;
+    }
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int kk = k0 + ak + 2 * r;
@@ -103,7 +116,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, T alpha,
 #pragma unroll
     for (int r = 0; r < 8; ++r) sA[buf][ak + 2 * r][tid & 127] = round_low<ROUND>(ra[r]);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) sB[buf][bk][(tid >> 4) + 16 * r] = round_low<ROUND>(rb[r]);
+    for (int r = 0; r < 8; ++r) sB[buf][(tid >> 4) + 16 * r][bk] = round_low<ROUND>(rb[r]);
   };
 
   const int nk = (k + GBK - 1) / GBK;
@@ -119,7 +132,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, T alpha,
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         a[q] = sA[buf][kk + (lane >> 4)][wm + 16 * q + (lane & 15)];
-        b[q] = sB[buf][kk + (lane >> 4)][wn + 16 * q + (lane & 15)];
+        b[q] = sB[buf][wn + 16 * q + (lane & 15)][kk + (lane >> 4)];
       }
 #pragma unroll
       for (int qa = 0; qa < 4; ++qa)
@@ -218,25 +231,36 @@ __global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int
       A[i + (int64_t)j * lda] = s[i][j];
     }
   }
-  if (t < nb) {
-    const int c = t;
-    // column c of inv(L), L unit lower
-    for (int i = 0; i < c; ++i) sl[i][c] = T(0);
-    sl[c][c] = T(1);
-    for (int i = c + 1; i < nb; ++i) {
-      T acc = T(0);
-      for (int kk = c; kk < i; ++kk) acc = __builtin_fma(-s[i][kk], sl[kk][c], acc);
-      sl[i][c] = acc;
-    }
-    // column c of inv(U)
-    for (int i = c + 1; i < nb; ++i) su[i][c] = T(0);
-    for (int i = c; i >= 0; --i) {
-      T acc = i == c ? T(1) : T(0);
-      for (int kk = i + 1; kk <= c; ++kk) acc = __builtin_fma(-s[i][kk], su[kk][c], acc);
-      su[i][c] = acc / s[i][i];
-    }
+  // Inverses by right-looking substitution on the identity, one step per
+  // row with all (row, column) updates of that step in parallel:
+  //   inv(L): for k = 0..nb-1:   X[i][c] -= L[i][k] X[k][c]  (i > k, c <= k)
+  //   inv(U): for k = nb-1..0:   X[k][c] /= U[k][k] (c >= k), then
+  //                              X[i][c] -= U[i][k] X[k][c]  (i < k, c >= k)
+  for (int e = t; e < nb * nb; e += 256) {
+    const int i = e % nb, c = e / nb;
+    sl[i][c] = i == c ? T(1) : T(0);
+    su[i][c] = i == c ? T(1) : T(0);
   }
   __syncthreads();
+  for (int k = 0; k < nb; ++k) {
+    const int rows = nb - 1 - k, cols = k + 1;
+    for (int e = t; e < rows * cols; e += 256) {
+      const int i = k + 1 + e % rows, c = e / rows;
+      sl[i][c] = __builtin_fma(-s[i][k], sl[k][c], sl[i][c]);
+    }
+    __syncthreads();
+  }
+  for (int k = nb - 1; k >= 0; --k) {
+    const T d = s[k][k];
+    for (int c = k + t; c < nb; c += 256) su[k][c] = su[k][c] / d;
+    __syncthreads();
+    const int rows = k, cols = nb - k;
+    for (int e = t; e < rows * cols; e += 256) {
+      const int i = e % rows, c = k + e / rows;
+      su[i][c] = __builtin_fma(-s[i][k], su[k][c], su[i][c]);
+    }
+    __syncthreads();
+  }
   for (int e = t; e < nb * nb; e += 256) {
     const int i = e % nb, j = e / nb;
     Linv[i + j * ldi] = sl[i][j];
@@ -252,6 +276,135 @@ int diag_factor(hipStream_t s, T* A, int64_t lda, int nb, T* Linv, T* Uinv, int 
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
+
+// ---------------------------------------------------------------------------
+// No-pivot panel path: one wavefront factors the 64 x 64 diagonal block with
+// every row in registers (lane i owns row i; the pivot row is broadcast with
+// v_readlane, so no barrier is needed), and the off-diagonal blocks are solved
+// in place by substitution, one row (L21) or one column (U12) per thread.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double bcast_lane(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float bcast_lane(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void diag_lu_wave_kernel(T* __restrict__ A, int64_t lda, int nb,
+                                                          int* __restrict__ info, int col0) {
+  const int i = threadIdx.x;
+  T a[64];
+#pragma unroll
+  for (int c = 0; c < 64; ++c)
+    a[c] = (i < nb && c < nb) ? A[i + (int64_t)c * lda] : T(i == c ? 1 : 0);  // identity padding
+#pragma unroll
+  for (int j = 0; j < 64; ++j) {
+    const T piv = bcast_lane(a[j], j);
+    if (i == 0 && j < nb && piv == T(0)) atomicCAS(info, 0, col0 + j + 1);
+    const T l = i > j ? a[j] / piv : T(0);
+    if (i > j) a[j] = l;
+#pragma unroll
+    for (int c = j + 1; c < 64; ++c) a[c] = __builtin_fma(-l, bcast_lane(a[c], j), a[c]);
+  }
+  if (i < nb) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+      if (c < nb) A[i + (int64_t)c * lda] = a[c];
+  }
+}
+
+// Rows [0, m) of B (m x nb, ld ldb): B := B inv(U), U the upper triangle of the
+// nb x nb block D (ld ldd). One thread per row, U staged in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_rows_upper_kernel(const T* __restrict__ D, int64_t ldd,
+                                                              int nb, T* __restrict__ B,
+                                                              int64_t ldb, int m) {
+  __shared__ T u[64][65];
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    u[r][c] = (r < nb && c < nb) ? D[r + (int64_t)c * ldd] : T(r == c ? 1 : 0);
+  }
+  __syncthreads();
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= m) return;
+  T x[64];
+#pragma unroll
+  for (int c = 0; c < 64; ++c) x[c] = c < nb ? B[r + (int64_t)c * ldb] : T(0);
+#pragma unroll
+  for (int c = 0; c < 64; ++c) {
+    T v = x[c];
+#pragma unroll
+    for (int k = 0; k < c; ++k) v = __builtin_fma(-x[k], u[k][c], v);
+    x[c] = v / u[c][c];
+  }
+#pragma unroll
+  for (int c = 0; c < 64; ++c)
+    if (c < nb) B[r + (int64_t)c * ldb] = x[c];
+}
+
+// Columns [0, n) of B (nb x n, ld ldb): B := inv(L) B, L the unit lower triangle
+// of D. One thread per column.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_cols_unit_lower_kernel(const T* __restrict__ D,
+                                                                   int64_t ldd, int nb,
+                                                                   T* __restrict__ B, int64_t ldb,
+                                                                   int n) {
+  __shared__ T l[64][65];
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    l[r][c] = (r < nb && c < nb) ? D[r + (int64_t)c * ldd] : T(0);
+  }
+  __syncthreads();
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  T* col = B + (int64_t)c * ldb;
+  T y[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) y[i] = i < nb ? col[i] : T(0);
+#pragma unroll
+  for (int i = 1; i < 64; ++i) {
+    T v = y[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) v = __builtin_fma(-l[i][k], y[k], v);
+    y[i] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < 64; ++i)
+    if (i < nb) col[i] = y[i];
+}
+
+template <typename T>
+int diag_lu_wave(hipStream_t s, T* A, int64_t lda, int nb, int* info, int col0) {
+  MPDOK_REQUIRE(nb >= 1 && nb <= 64, "diag_lu_wave: nb=%d", nb);
+  diag_lu_wave_kernel<T><<<1, 64, 0, s>>>(A, lda, nb, info, col0);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template <typename T>
+int trsm_rows_upper(hipStream_t s, const T* D, int64_t ldd, int nb, T* B, int64_t ldb, int m) {
+  if (m <= 0) return OK;
+  trsm_rows_upper_kernel<T><<<(m + 255) / 256, 256, 0, s>>>(D, ldd, nb, B, ldb, m);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template <typename T>
+int trsm_cols_unit_lower(hipStream_t s, const T* D, int64_t ldd, int nb, T* B, int64_t ldb, int n) {
+  if (n <= 0) return OK;
+  trsm_cols_unit_lower_kernel<T><<<(n + 255) / 256, 256, 0, s>>>(D, ldd, nb, B, ldb, n);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int diag_lu_wave<double>(hipStream_t, double*, int64_t, int, int*, int);
+template int diag_lu_wave<float>(hipStream_t, float*, int64_t, int, int*, int);
+template int trsm_rows_upper<double>(hipStream_t, const double*, int64_t, int, double*, int64_t, int);
+template int trsm_rows_upper<float>(hipStream_t, const float*, int64_t, int, float*, int64_t, int);
+template int trsm_cols_unit_lower<double>(hipStream_t, const double*, int64_t, int, double*, int64_t,
+                                          int);
+template int trsm_cols_unit_lower<float>(hipStream_t, const float*, int64_t, int, float*, int64_t,
+                                         int);
 template int diag_factor<double>(hipStream_t, double*, int64_t, int, double*, double*, int, int*,
                                  int, int);
 template int diag_factor<float>(hipStream_t, float*, int64_t, int, float*, float*, int, int*, int,
@@ -492,48 +645,67 @@ template int trsv_blocked<float, double>(hipStream_t, const float*, int64_t, int
 // Every (c, r) accumulator is written by one thread per step in stream order,
 // so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------
-constexpr int TB = 512, TB_CH = 8;
+constexpr int TB = 512, TB_CH = 8, TB_CW = TB / TB_CH;
 
+// xpart[cg, r0 + rg*64 + r] = sum_{j < 64} inv(D)[rg*64 + r, cg*64 + j] v'[cg*64 + j]
+// with v' = v - sum_c acc[c]; one workgroup per (row group, column group) tile;
+// tiles of the zero triangle are skipped.
 template <typename T, typename TV>
 __global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ Binv, int r0,
                                                          const TV* __restrict__ v,
                                                          const TV* __restrict__ acc, int N,
-                                                         TV* __restrict__ x) {
-  __shared__ TV vs[TB];
+                                                         TV* __restrict__ xpart, int lower) {
+  __shared__ TV vs[TB_CW];
   __shared__ TV red[4][64];
-  const int t = threadIdx.x;
-  for (int j = t; j < TB; j += 256) {
-    TV s = v[r0 + j];
+  const int rg = blockIdx.x, cg = blockIdx.y, t = threadIdx.x;
+  TV* out = xpart + (size_t)cg * N + r0 + rg * 64;
+  if (lower ? cg > rg : cg < rg) {
+    if (t < 64) out[t] = TV(0);
+    return;
+  }
+  if (t < TB_CW) {
+    const int g = r0 + cg * TB_CW + t;
+    TV s = v[g];
 #pragma unroll
-    for (int c = 0; c < TB_CH; ++c) s -= acc[(size_t)c * N + r0 + j];
-    vs[j] = s;
+    for (int c = 0; c < TB_CH; ++c) s -= acc[(size_t)c * N + g];
+    vs[t] = s;
   }
   __syncthreads();
   const int r = t & 63, p = t >> 6;
-  const int row = blockIdx.x * 64 + r;
+  const T* col = Binv + rg * 64 + r + (size_t)(cg * TB_CW) * TB;
   TV a = TV(0);
-  for (int j = p; j < TB; j += 4) a = __builtin_fma((TV)Binv[row + (size_t)j * TB], vs[j], a);
+#pragma unroll 4
+  for (int j = p; j < TB_CW; j += 4) a = __builtin_fma((TV)col[(size_t)j * TB], vs[j], a);
   red[p][r] = a;
   __syncthreads();
-  if (t < 64) x[r0 + blockIdx.x * 64 + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+  if (t < 64) out[t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
 }
 
+// Completes x for columns [c0, c0 + TB) from the column-group partials, and
+// adds the products of those columns into the accumulators of rows
+// [rbeg, rend). Workgroups with blockIdx.x == 0 store the finished x values.
 template <typename T, typename TV>
 __global__ __launch_bounds__(256) void tri_acc_kernel(const T* __restrict__ M, int64_t ldm,
                                                       int rbeg, int rend, int c0,
-                                                      const TV* __restrict__ x,
-                                                      TV* __restrict__ acc, int N) {
-  constexpr int CW = TB / TB_CH;
-  __shared__ TV xs[CW];
+                                                      const TV* __restrict__ xpart,
+                                                      TV* __restrict__ x, TV* __restrict__ acc,
+                                                      int N) {
+  __shared__ TV xs[TB_CW];
   const int c = blockIdx.y;
-  const int cb = c0 + c * CW;
-  if (threadIdx.x < CW) xs[threadIdx.x] = x[cb + threadIdx.x];
+  const int cb = c0 + c * TB_CW;
+  if (threadIdx.x < TB_CW) {
+    TV s = TV(0);
+#pragma unroll
+    for (int g = 0; g < TB_CH; ++g) s += xpart[(size_t)g * N + cb + threadIdx.x];
+    xs[threadIdx.x] = s;
+    if (blockIdx.x == 0) x[cb + threadIdx.x] = s;
+  }
   __syncthreads();
   const int r = rbeg + blockIdx.x * 256 + threadIdx.x;
   if (r >= rend) return;
   TV a = TV(0);
 #pragma unroll 8
-  for (int j = 0; j < CW; ++j) a = __builtin_fma((TV)M[r + (int64_t)(cb + j) * ldm], xs[j], a);
+  for (int j = 0; j < TB_CW; ++j) a = __builtin_fma((TV)M[r + (int64_t)(cb + j) * ldm], xs[j], a);
   acc[(size_t)c * N + r] += a;
 }
 
@@ -542,23 +714,25 @@ int trsv_block_inverse(hipStream_t s, const T* LU, int64_t ld, int N, const T* L
                        const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc) {
   MPDOK_REQUIRE(N % TB == 0, "trsv_block_inverse: N must be a multiple of %d", TB);
   const int nb = N / TB;
-  // forward: v = b, solve L y = v  (y -> x)
+  TV* xpart = acc + (size_t)TB_CH * N;  // second half of the accumulator buffer
+  const dim3 dgrid(TB / 64, TB_CH);
+  // forward: L y = b, y -> v
   MPDOK_HIP(hipMemsetAsync(acc, 0, sizeof(TV) * (size_t)TB_CH * N, s));
   for (int i = 0; i < nb; ++i) {
     const int r0 = i * TB;
-    diag_apply_kernel<T, TV><<<TB / 64, 256, 0, s>>>(Lbinv + (size_t)i * TB * TB, r0, b, acc, N, v);
-    if (r0 + TB < N)
-      tri_acc_kernel<T, TV><<<dim3((N - r0 - TB + 255) / 256, TB_CH), 256, 0, s>>>(
-          LU, ld, r0 + TB, N, r0, v, acc, N);
+    diag_apply_kernel<T, TV><<<dgrid, 256, 0, s>>>(Lbinv + (size_t)i * TB * TB, r0, b, acc, N,
+                                                   xpart, 1);
+    tri_acc_kernel<T, TV><<<dim3(r0 + TB < N ? (N - r0 - TB + 255) / 256 : 1, TB_CH), 256, 0, s>>>(
+        LU, ld, r0 + TB, N, r0, xpart, v, acc, N);
   }
-  // backward: solve U x = y (y in v)
+  // backward: U x = y
   MPDOK_HIP(hipMemsetAsync(acc, 0, sizeof(TV) * (size_t)TB_CH * N, s));
   for (int i = nb - 1; i >= 0; --i) {
     const int r0 = i * TB;
-    diag_apply_kernel<T, TV><<<TB / 64, 256, 0, s>>>(Ubinv + (size_t)i * TB * TB, r0, v, acc, N, x);
-    if (r0 > 0)
-      tri_acc_kernel<T, TV><<<dim3((r0 + 255) / 256, TB_CH), 256, 0, s>>>(LU, ld, 0, r0, r0, x,
-                                                                         acc, N);
+    diag_apply_kernel<T, TV><<<dgrid, 256, 0, s>>>(Ubinv + (size_t)i * TB * TB, r0, v, acc, N,
+                                                   xpart, 0);
+    tri_acc_kernel<T, TV><<<dim3(r0 > 0 ? (r0 + 255) / 256 : 1, TB_CH), 256, 0, s>>>(
+        LU, ld, 0, r0, r0, xpart, x, acc, N);
   }
   MPDOK_HIP(hipGetLastError());
   return OK;

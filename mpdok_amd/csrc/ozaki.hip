@@ -139,22 +139,40 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
 }
 
 // C (m x n, column-major, ldc) -= sliced product. mp, np: padded row/column
-// counts of the slice arrays (multiples of the block tile), Kp multiple of 64.
-template <int S>
-__global__ __launch_bounds__(256) void ozaki_gemm_kernel(int m, int n, int mp, int np, int Kp,
-                                                         const signed char* __restrict__ As,
-                                                         const signed char* __restrict__ Bs,
-                                                         const int* __restrict__ ea,
-                                                         const int* __restrict__ eb,
-                                                         double* __restrict__ C, int64_t ldc) {
-  constexpr int PADB = 16;  // bytes of padding per LDS row
-  __shared__ __attribute__((aligned(16))) signed char sA[S][OZ_BM][OZ_BK + PADB];
-  __shared__ __attribute__((aligned(16))) signed char sB[S][OZ_BN][OZ_BK + PADB];
+// counts of the slice arrays (multiples of 128), Kp multiple of 64.
+// 4 wavefronts (2 x 2), block tile 128 x 64. PREFETCH: stage the next K step
+// through registers while the current one is multiplied (more registers,
+// one workgroup per CU); without it the kernel is lean enough for two
+// workgroups per CU at S <= 5, which overlap each other's staging.
+// LDS rows hold the 64 bytes of one K step of one row (column) of a slice.
+// SWZ: 64-byte rows with the four 16-byte chunks permuted by
+// q ^ (((row >> 1) ^ (row >> 2)) & 3), so that the 8 (or 16) lanes served
+// together by the LDS, which read or write consecutive rows, always hit
+// distinct bank groups; otherwise 80-byte rows (16 bytes of padding).
+template <bool SWZ>
+__device__ __forceinline__ int lds_off(int row, int q) {
+  return SWZ ? row * 64 + 16 * (q ^ (((row >> 1) ^ (row >> 2)) & 3)) : row * 80 + 16 * q;
+}
+
+// Columns [col0, col0 + ncols) of the product (col0 a multiple of 64).
+template <int S, bool PREFETCH, bool SWZ>
+__global__ __launch_bounds__(256, (PREFETCH || S > 5) ? 1 : 2) void ozaki_gemm_kernel(int m, int n, int mp, int np, int Kp,
+                                                             int col0, int ncols,
+                                                             const signed char* __restrict__ As,
+                                                             const signed char* __restrict__ Bs,
+                                                             const int* __restrict__ ea,
+                                                             const int* __restrict__ eb,
+                                                             double* __restrict__ C, int64_t ldc) {
+  constexpr int WN = 2, NT = 128 * WN, BN = 32 * WN;
+  constexpr int ROWB = SWZ ? 64 : 80;
+  constexpr int NCA = OZ_BM * 4 / NT, NCB = BN * 4 / NT;  // 16-byte chunks per thread
+  __shared__ __attribute__((aligned(16))) signed char sA[S][OZ_BM * ROWB];
+  __shared__ __attribute__((aligned(16))) signed char sB[S][BN * ROWB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
   int ti, tj;
-  if (!tile_of(blockIdx.x, mp / OZ_BM, np / OZ_BN, ti, tj)) return;
-  const int bm = ti * OZ_BM, bn = tj * OZ_BN;
+  if (!tile_of(blockIdx.x, mp / OZ_BM, (ncols + BN - 1) / BN, ti, tj)) return;
+  const int bm = ti * OZ_BM, bn = col0 + tj * BN;
   const size_t slabA = (size_t)mp * Kp, slabB = (size_t)np * Kp;
 
   v4i acc[S][4][2];
@@ -165,30 +183,51 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(int m, int n, int mp, i
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[l][a][b] = v4i(0);
 
-  // register prefetch of one K step: per thread 2 A chunks and 1 B chunk per slice
-  v4i pa[S][2], pb[S];
-  const int arow0 = tid >> 2, akq = (tid & 3) * 16;  // A chunks: rows arow0 and arow0 + 64
+  // register prefetch of one K step (16-byte chunks: row = chunk / 4, k = 16 (chunk % 4))
+  v4i pa[PREFETCH ? S : 1][NCA], pb[PREFETCH ? S : 1][NCB];
   auto gload = [&](int k0) {
+    if (!PREFETCH) return;
 #pragma unroll
     for (int q = 0; q < S; ++q) {
-      const signed char* ap = As + q * slabA + (size_t)(bm + arow0) * Kp + k0 + akq;
-      pa[q][0] = *reinterpret_cast<const v4i*>(ap);
-      pa[q][1] = *reinterpret_cast<const v4i*>(ap + (size_t)64 * Kp);
-      pb[q] = *reinterpret_cast<const v4i*>(Bs + q * slabB + (size_t)(bn + arow0) * Kp + k0 + akq);
+#pragma unroll
+      for (int u = 0; u < NCA; ++u) {
+        const int c = tid + u * NT;
+        pa[PREFETCH ? q : 0][u] = *reinterpret_cast<const v4i*>(As + q * slabA + (size_t)(bm + (c >> 2)) * Kp + k0 +
+                                                 (c & 3) * 16);
+      }
+#pragma unroll
+      for (int u = 0; u < NCB; ++u) {
+        const int c = tid + u * NT;
+        pb[PREFETCH ? q : 0][u] = *reinterpret_cast<const v4i*>(Bs + q * slabB + (size_t)(bn + (c >> 2)) * Kp + k0 +
+                                                 (c & 3) * 16);
+      }
     }
   };
-  auto sstore = [&]() {
+  auto sstore = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < S; ++q) {
-      *reinterpret_cast<v4i*>(&sA[q][arow0][akq]) = pa[q][0];
-      *reinterpret_cast<v4i*>(&sA[q][arow0 + 64][akq]) = pa[q][1];
-      *reinterpret_cast<v4i*>(&sB[q][arow0][akq]) = pb[q];
+#pragma unroll
+      for (int u = 0; u < NCA; ++u) {
+        const int c = tid + u * NT;
+        *reinterpret_cast<v4i*>(&sA[q][lds_off<SWZ>(c >> 2, c & 3)]) =
+            PREFETCH ? pa[PREFETCH ? q : 0][u]
+                     : *reinterpret_cast<const v4i*>(As + q * slabA + (size_t)(bm + (c >> 2)) * Kp +
+                                                     k0 + (c & 3) * 16);
+      }
+#pragma unroll
+      for (int u = 0; u < NCB; ++u) {
+        const int c = tid + u * NT;
+        *reinterpret_cast<v4i*>(&sB[q][lds_off<SWZ>(c >> 2, c & 3)]) =
+            PREFETCH ? pb[PREFETCH ? q : 0][u]
+                     : *reinterpret_cast<const v4i*>(Bs + q * slabB + (size_t)(bn + (c >> 2)) * Kp +
+                                                     k0 + (c & 3) * 16);
+      }
     }
   };
-  const int fr = lane & 15, fk = (lane >> 4) * 16;
+  const int fr = lane & 15, fq = lane >> 4;
   gload(0);
   for (int k0 = 0; k0 < Kp; k0 += OZ_BK) {
-    sstore();
+    sstore(k0);
     __syncthreads();
     if (k0 + OZ_BK < Kp) gload(k0 + OZ_BK);
     // all B-slice fragments of this K step stay in registers across the i loop
@@ -196,12 +235,14 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(int m, int n, int mp, i
 #pragma unroll
     for (int j = 0; j < S; ++j)
 #pragma unroll
-      for (int tb = 0; tb < 2; ++tb) b[j][tb] = *reinterpret_cast<const v4i*>(&sB[j][wn + 16 * tb + fr][fk]);
+      for (int tb = 0; tb < 2; ++tb)
+        b[j][tb] = *reinterpret_cast<const v4i*>(&sB[j][lds_off<SWZ>(wn + 16 * tb + fr, fq)]);
 #pragma unroll
     for (int i = 0; i < S; ++i) {
       v4i a[4];
 #pragma unroll
-      for (int ta = 0; ta < 4; ++ta) a[ta] = *reinterpret_cast<const v4i*>(&sA[i][wm + 16 * ta + fr][fk]);
+      for (int ta = 0; ta < 4; ++ta)
+        a[ta] = *reinterpret_cast<const v4i*>(&sA[i][lds_off<SWZ>(wm + 16 * ta + fr, fq)]);
 #pragma unroll
       for (int j = 0; j < S - i; ++j)  // 0-based: level index i + j = l - 2
 #pragma unroll
@@ -234,42 +275,95 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(int m, int n, int mp, i
     }
 }
 
+constexpr int OZ_PAD = 128;  // row and column padding of the slice arrays
+
 size_t ozaki_workspace_bytes(int m, int n, int K, int S) {
-  const int mp = (m + OZ_BM - 1) / OZ_BM * OZ_BM;
-  const int np = (n + OZ_BN - 1) / OZ_BN * OZ_BN;
+  const int mp = (m + OZ_PAD - 1) / OZ_PAD * OZ_PAD;
+  const int np = (n + OZ_PAD - 1) / OZ_PAD * OZ_PAD;
   const int Kp = (K + OZ_BK - 1) / OZ_BK * OZ_BK;
   return (size_t)S * Kp * ((size_t)mp + np) + sizeof(int) * ((size_t)mp + np) + 256;
 }
 
+struct OzLayout {
+  int mp, np, Kp;
+  signed char *As, *Bs;
+  int *ea, *eb;
+};
+static OzLayout oz_layout(int S, int m, int n, int K, void* work) {
+  OzLayout L;
+  L.mp = (m + OZ_PAD - 1) / OZ_PAD * OZ_PAD;
+  L.np = (n + OZ_PAD - 1) / OZ_PAD * OZ_PAD;
+  L.Kp = (K + OZ_BK - 1) / OZ_BK * OZ_BK;
+  L.As = static_cast<signed char*>(work);
+  L.Bs = L.As + (size_t)S * L.mp * L.Kp;
+  L.ea = reinterpret_cast<int*>(L.Bs + (size_t)S * L.np * L.Kp);
+  L.eb = L.ea + L.mp;
+  return L;
+}
+
 template <int S>
-static int ozaki_gemm_s(hipStream_t s, int m, int n, int K, const double* A, int64_t lda,
-                        const double* B, int64_t ldb, double* C, int64_t ldc, void* work) {
-  const int mp = (m + OZ_BM - 1) / OZ_BM * OZ_BM;
-  const int np = (n + OZ_BN - 1) / OZ_BN * OZ_BN;
-  const int Kp = (K + OZ_BK - 1) / OZ_BK * OZ_BK;
-  signed char* As = static_cast<signed char*>(work);
-  signed char* Bs = As + (size_t)S * mp * Kp;
-  int* ea = reinterpret_cast<int*>(Bs + (size_t)S * np * Kp);
-  int* eb = ea + mp;
-  slice_rows_kernel<S><<<mp / 64, 256, 0, s>>>(A, lda, m, K, mp, Kp, As, ea);
-  slice_cols_kernel<S><<<np / 4, 256, 0, s>>>(B, ldb, n, K, np, Kp, Bs, eb);
-  ozaki_gemm_kernel<S><<<tiles_launch(mp / OZ_BM, np / OZ_BN), 256, 0, s>>>(m, n, mp, np, Kp, As,
-                                                                              Bs, ea, eb, C, ldc);
+static int ozaki_slice_s(hipStream_t s, int m, int n, int K, const double* A, int64_t lda,
+                         const double* B, int64_t ldb, void* work) {
+  const OzLayout L = oz_layout(S, m, n, K, work);
+  slice_rows_kernel<S><<<L.mp / 64, 256, 0, s>>>(A, lda, m, K, L.mp, L.Kp, L.As, L.ea);
+  slice_cols_kernel<S><<<L.np / 4, 256, 0, s>>>(B, ldb, n, K, L.np, L.Kp, L.Bs, L.eb);
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
 
-int ozaki_gemm(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
-               const double* B, int64_t ldb, double* C, int64_t ldc, void* work) {
+template <int S>
+static int ozaki_apply_s(hipStream_t s, int m, int n, int K, int col0, int ncols, double* C,
+                         int64_t ldc, void* work) {
+  const OzLayout L = oz_layout(S, m, n, K, work);
+  const int grid = tiles_launch(L.mp / OZ_BM, (ncols + 63) / 64);
+  const int pf = tuning_get(TUNE_OZAKI_PREFETCH), swz = tuning_get(TUNE_OZAKI_SWIZZLE);
+#define OZ_LAUNCH(PF, SW)                                                                  \
+  ozaki_gemm_kernel<S, PF, SW><<<grid, 256, 0, s>>>(m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, \
+                                                    L.Bs, L.ea, L.eb, C, ldc)
+  if (pf && swz) OZ_LAUNCH(true, true);
+  else if (pf) OZ_LAUNCH(true, false);
+  else if (swz) OZ_LAUNCH(false, true);
+  else OZ_LAUNCH(false, false);
+#undef OZ_LAUNCH
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+// Slice A (m x K) by rows and B (K x n) by columns into the workspace.
+int ozaki_slice(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
+                const double* B, int64_t ldb, void* work) {
   if (m <= 0 || n <= 0 || K <= 0) return OK;
   switch (S) {
-    case 3: return ozaki_gemm_s<3>(s, m, n, K, A, lda, B, ldb, C, ldc, work);
-    case 4: return ozaki_gemm_s<4>(s, m, n, K, A, lda, B, ldb, C, ldc, work);
-    case 5: return ozaki_gemm_s<5>(s, m, n, K, A, lda, B, ldb, C, ldc, work);
-    case 6: return ozaki_gemm_s<6>(s, m, n, K, A, lda, B, ldb, C, ldc, work);
-    case 7: return ozaki_gemm_s<7>(s, m, n, K, A, lda, B, ldb, C, ldc, work);
-    default: MPDOK_REQUIRE(false, "ozaki_gemm: slices must be in [3, 7], got %d", S);
+    case 3: return ozaki_slice_s<3>(s, m, n, K, A, lda, B, ldb, work);
+    case 4: return ozaki_slice_s<4>(s, m, n, K, A, lda, B, ldb, work);
+    case 5: return ozaki_slice_s<5>(s, m, n, K, A, lda, B, ldb, work);
+    case 6: return ozaki_slice_s<6>(s, m, n, K, A, lda, B, ldb, work);
+    case 7: return ozaki_slice_s<7>(s, m, n, K, A, lda, B, ldb, work);
+    default: MPDOK_REQUIRE(false, "ozaki_slice: slices must be in [3, 7], got %d", S);
   }
+}
+
+// C[:, col0 : col0 + ncols] -= the sliced product of the last ozaki_slice call
+// with the same (S, m, n, K, work); col0 a multiple of 64, C the base of the
+// full m x n block.
+int ozaki_apply(hipStream_t s, int S, int m, int n, int K, int col0, int ncols, double* C,
+                int64_t ldc, void* work) {
+  if (m <= 0 || ncols <= 0 || K <= 0) return OK;
+  MPDOK_REQUIRE(col0 % 64 == 0 && col0 + ncols <= n, "ozaki_apply: bad column range");
+  switch (S) {
+    case 3: return ozaki_apply_s<3>(s, m, n, K, col0, ncols, C, ldc, work);
+    case 4: return ozaki_apply_s<4>(s, m, n, K, col0, ncols, C, ldc, work);
+    case 5: return ozaki_apply_s<5>(s, m, n, K, col0, ncols, C, ldc, work);
+    case 6: return ozaki_apply_s<6>(s, m, n, K, col0, ncols, C, ldc, work);
+    case 7: return ozaki_apply_s<7>(s, m, n, K, col0, ncols, C, ldc, work);
+    default: MPDOK_REQUIRE(false, "ozaki_apply: slices must be in [3, 7], got %d", S);
+  }
+}
+
+int ozaki_gemm(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
+               const double* B, int64_t ldb, double* C, int64_t ldc, void* work) {
+  MPDOK_TRY(ozaki_slice(s, S, m, n, K, A, lda, B, ldb, work));
+  return ozaki_apply(s, S, m, n, K, 0, n, C, ldc, work);
 }
 
 }  // namespace mpdok
