@@ -67,8 +67,10 @@ void py_lu_free_factored(void);
 const char* mpdok_last_error(void);
 /* Stream used by the reference-compatible symbols (NULL = null stream). */
 int mpdok_set_stream(hipStream_t s);
-/* Kernel-shape choices picked by the tuner (key 0: register prefetch in the
- * int8-sliced GEMM, 0 or 1). No reference counterpart. */
+/* Kernel-shape choices picked by the tuner. Key 0: block shape of the
+ * int8-sliced GEMM (0 = 128 x 64 tiles, 2 workgroups per CU (default);
+ * 1 = the same, 1 per CU; 2 = 128 x 128; 3 = four pipeline stages).
+ * No reference counterpart. */
 int mpdok_set_tuning(int key, int value);
 int mpdok_get_tuning(int key);
 size_t mpdok_reduce_workspace_bytes(int n);
@@ -116,6 +118,31 @@ int mpdok_rbf_assemble(hipStream_t s, const double* coords, int N, int D, int ki
                        void* out, int out_dtype, int64_t ld, int order);
 /* out[0] = mean clamped d2 over the first n points (rbf_kernel.py:57-68). */
 int mpdok_rbf_mean_d2(hipStream_t s, const double* coords, int n, int D, double* out);
+/* Matrix-free y = K[row0:row0+rows, :] x, kernel values recomputed, never
+ * stored. elem_dtype F64: FP64 element, nugget in FP64 (replaces
+ * MPDOKOOCSolver._tiled_dgemv_fp64, mpdok_ooc.py:205-227); F32: the stored FP32
+ * tile's element (cast, then nugget in FP32; the matrix-free form of
+ * _tiled_sgemv, mpdok_ooc.py:39-46,158-201). x, y both xy_dtype (F64 or F32). */
+int mpdok_rbf_matvec(hipStream_t s, const double* coords, int N, int D, int kind, double p,
+                     double reg, int elem_dtype, const void* x, void* y, int xy_dtype, int row0,
+                     int rows);
+
+/* Out-of-core products (mpdok_ooc.py:158-201). y[0:M] = A_host x for a
+ * row-major FP32 A (M x N, leading dimension lda) in host memory (pinned for
+ * full overlap), streamed in tiles of tile_rows rows through two device
+ * buffers in `work` (mpdok_stream_gemv_workspace_bytes), copy of tile t+1
+ * overlapping the product of tile t. Replaces _tiled_sgemv_ram/_ssd's per-tile
+ * cp.asarray + matmul. */
+size_t mpdok_stream_gemv_workspace_bytes(int tile_rows, int N);
+int mpdok_stream_gemv_f32(hipStream_t s, const float* A_host, int64_t lda, int M, int N,
+                          int tile_rows, const float* x, float* y, void* work, size_t work_bytes);
+/* A_host[0:rows, :] = FP32 kernel rows [row0, row0+rows) (nugget after the
+ * cast), assembled on the GPU per tile and copied to host memory, the copy of
+ * one tile overlapping the assembly of the next. Replaces MPDOKOOCSolver.build's
+ * tile loop (mpdok_ooc.py:116-146). Same workspace as mpdok_stream_gemv_f32. */
+int mpdok_rbf_build_host(hipStream_t s, const double* coords, int N, int D, int kind, double p,
+                         double reg, int row0, int rows, int tile_rows, float* A_host,
+                         int64_t ldh, void* work, size_t work_bytes);
 
 /* One Arnoldi orthogonalisation of w against V_0..V_k (vectors of length n,
  * stride ldv): passes = 1 (dense_krylov.py:289-303) or 2 (DGKS,

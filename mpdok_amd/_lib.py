@@ -54,6 +54,11 @@ SIGNATURES = {
     # RBF assembly
     "mpdok_rbf_assemble": (I, [P, P, I, I, I, D, D, I, I, I, I, I, P, I, I64, I]),
     "mpdok_rbf_mean_d2": (I, [P, P, I, I, P]),
+    "mpdok_rbf_matvec": (I, [P, P, I, I, I, D, D, I, P, P, I, I, I]),
+    # out-of-core streaming
+    "mpdok_stream_gemv_workspace_bytes": (SZ, [I, I]),
+    "mpdok_stream_gemv_f32": (I, [P, P, I64, I, I, I, P, P, P, SZ]),
+    "mpdok_rbf_build_host": (I, [P, P, I, I, I, D, D, I, I, I, P, I64, P, SZ]),
     # Krylov primitives
     "mpdok_arnoldi_orth_f32": (I, [P, P, I64, I, P, I, P, I, D, I, P, P, SZ]),
     "mpdok_arnoldi_orth_f64": (I, [P, P, I64, I, P, I, P, I, D, I, P, P, SZ]),

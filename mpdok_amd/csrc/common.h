@@ -27,8 +27,7 @@ void set_error(const char* fmt, ...);
 // Process-wide kernel-shape choices (set by the tuner through
 // mpdok_set_tuning; read at launch time).
 enum TuneKey : int {
-  TUNE_OZAKI_PREFETCH = 0,  // int8-sliced GEMM: 1 = register prefetch of the next K step
-  TUNE_OZAKI_SWIZZLE = 1,   // int8-sliced GEMM: 1 = XOR-swizzled 64-byte LDS rows
+  TUNE_OZAKI_SHAPE = 0,  // int8-sliced GEMM block shape (waves M x N, stages, workgroups per CU)
   TUNE_COUNT = 8,
 };
 int tuning_get(int key);

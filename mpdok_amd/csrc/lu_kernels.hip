@@ -8,7 +8,7 @@
 //    BF16/FP16 matrix-core product, used by the precision ladder.
 //  * diag_factor_kernel: LU without pivoting of one nb x nb (nb <= 64)
 //    diagonal block in LDS, plus the explicit inverses of its unit-lower and
-//    upper triangles (used by the blocked triangular solves).
+//    upper triangles (used by the panel and by the blocked triangular solves).
 //  * panel kernels for partial pivoting: pivot search with a last-block
 //    reduction, row swap, column scaling and rank-1 update of the sub-panel.
 //  * blocked triangular solves (forward unit-lower, backward upper) with 64-row
@@ -196,7 +196,16 @@ template int gemm_batched<float>(hipStream_t, int, int, int, float, const float*
                                  int, int);
 
 // ---------------------------------------------------------------------------
-// Diagonal block: LU without pivoting, then inv(unit L) and inv(U).
+// Diagonal block: LU without pivoting, then inv(unit L) and inv(U), all in LDS.
+//
+// Thread t owns row r = t % 64 and the columns c = t / 64 (mod 4). Elimination
+// step j: every thread forms its row's multiplier l = a[r][j] / a[j][j] itself
+// and updates its columns c > j; the multipliers are kept apart in lo[][], so
+// that column j is never rewritten while it is read and one barrier per step
+// suffices. The inverses then run side by side, threads 0..127 on inv(L)
+// (forward), 128..255 on inv(U) (backward), by substitution on the identity.
+// The loops run on LDS with small code: a single workgroup executes this once
+// per 64 columns, so its instructions come cold from memory each time.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int64_t lda, int nb,
@@ -204,67 +213,66 @@ __global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int
                                                           T* __restrict__ Uinv, int ldi,
                                                           int* __restrict__ info, int col0,
                                                           int factor) {
-  __shared__ T s[64][65];
-  __shared__ T sl[64][65];
-  __shared__ T su[64][65];
-  const int t = threadIdx.x;
-  for (int e = t; e < nb * nb; e += 256) {
-    const int i = e % nb, j = e / nb;
-    s[i][j] = A[i + (int64_t)j * lda];
+  __shared__ T s[64][65];   // U (upper triangle) during and after the elimination
+  __shared__ T lo[64][65];  // strict lower triangle: multipliers
+  __shared__ T x[2][64][65];  // inv(L), inv(U)
+  const int t = threadIdx.x, r = t & 63, q = t >> 6;
+  for (int c = q; c < nb; c += 4) {
+    if (r < nb) {
+      const T v = A[r + (int64_t)c * lda];
+      s[r][c] = v;
+      lo[r][c] = v;
+    }
   }
   __syncthreads();
   if (factor) {
     for (int j = 0; j < nb; ++j) {
       const T piv = s[j][j];
       if (t == 0 && piv == T(0)) atomicCAS(info, 0, col0 + j + 1);
-      for (int i = j + 1 + t; i < nb; i += 256) s[i][j] = s[i][j] / piv;
-      __syncthreads();
-      const int w = nb - j - 1;
-      for (int e = t; e < w * w; e += 256) {
-        const int i = j + 1 + e % w, c = j + 1 + e / w;
-        s[i][c] = __builtin_fma(-s[i][j], s[j][c], s[i][c]);
+      if (r > j && r < nb) {
+        const T l = s[r][j] / piv;
+        if (q == 0) lo[r][j] = l;
+#pragma unroll 4
+        for (int c = j + 1 + q; c < nb; c += 4) s[r][c] = __builtin_fma(-l, s[j][c], s[r][c]);
       }
       __syncthreads();
     }
-    for (int e = t; e < nb * nb; e += 256) {
-      const int i = e % nb, j = e / nb;
-      A[i + (int64_t)j * lda] = s[i][j];
-    }
+    for (int c = q; c < nb; c += 4)
+      if (r < nb) A[r + (int64_t)c * lda] = r > c ? lo[r][c] : s[r][c];
   }
-  // Inverses by right-looking substitution on the identity, one step per
-  // row with all (row, column) updates of that step in parallel:
-  //   inv(L): for k = 0..nb-1:   X[i][c] -= L[i][k] X[k][c]  (i > k, c <= k)
-  //   inv(U): for k = nb-1..0:   X[k][c] /= U[k][k] (c >= k), then
-  //                              X[i][c] -= U[i][k] X[k][c]  (i < k, c >= k)
-  for (int e = t; e < nb * nb; e += 256) {
-    const int i = e % nb, c = e / nb;
-    sl[i][c] = i == c ? T(1) : T(0);
-    su[i][c] = i == c ? T(1) : T(0);
+  // identity start for both inverses
+  for (int c = q; c < 64; c += 4) {
+    x[0][r][c] = r == c ? T(1) : T(0);
+    x[1][r][c] = r == c ? T(1) : T(0);
   }
   __syncthreads();
-  for (int k = 0; k < nb; ++k) {
-    const int rows = nb - 1 - k, cols = k + 1;
-    for (int e = t; e < rows * cols; e += 256) {
-      const int i = k + 1 + e % rows, c = e / rows;
-      sl[i][c] = __builtin_fma(-s[i][k], sl[k][c], sl[i][c]);
+  // inv(L) (threads with h == 0), ascending k:  X[i][c] -= L[i][k] X[k][c],  i > k, c <= k
+  // inv(U) (h == 1), descending k:  X[k][c] /= U[k][k] (c >= k), then
+  //                                 X[i][c] -= U[i][k] X[k][c],  i < k, c >= k
+  const int h = q >> 1, p = q & 1;
+  for (int st = 0; st < nb; ++st) {
+    const int kU = nb - 1 - st;
+    if (h == 1 && p == 0 && r >= kU && r < nb) x[1][kU][r] = x[1][kU][r] / s[kU][kU];
+    __syncthreads();
+    if (h == 0) {
+      const int k = st;
+      if (r > k && r < nb) {
+        const T l = lo[r][k];
+#pragma unroll 4
+        for (int c = p; c <= k; c += 2) x[0][r][c] = __builtin_fma(-l, x[0][k][c], x[0][r][c]);
+      }
+    } else if (r < kU) {
+      const T u = s[r][kU];
+#pragma unroll 4
+      for (int c = kU + p; c < nb; c += 2) x[1][r][c] = __builtin_fma(-u, x[1][kU][c], x[1][r][c]);
     }
     __syncthreads();
   }
-  for (int k = nb - 1; k >= 0; --k) {
-    const T d = s[k][k];
-    for (int c = k + t; c < nb; c += 256) su[k][c] = su[k][c] / d;
-    __syncthreads();
-    const int rows = k, cols = nb - k;
-    for (int e = t; e < rows * cols; e += 256) {
-      const int i = e % rows, c = k + e / rows;
-      su[i][c] = __builtin_fma(-s[i][k], su[k][c], su[i][c]);
+  for (int c = q; c < nb; c += 4) {
+    if (r < nb) {
+      Linv[r + c * ldi] = x[0][r][c];
+      Uinv[r + c * ldi] = x[1][r][c];
     }
-    __syncthreads();
-  }
-  for (int e = t; e < nb * nb; e += 256) {
-    const int i = e % nb, j = e / nb;
-    Linv[i + j * ldi] = sl[i][j];
-    Uinv[i + j * ldi] = su[i][j];
   }
 }
 
@@ -277,134 +285,6 @@ int diag_factor(hipStream_t s, T* A, int64_t lda, int nb, T* Linv, T* Uinv, int 
   return OK;
 }
 
-// ---------------------------------------------------------------------------
-// No-pivot panel path: one wavefront factors the 64 x 64 diagonal block with
-// every row in registers (lane i owns row i; the pivot row is broadcast with
-// v_readlane, so no barrier is needed), and the off-diagonal blocks are solved
-// in place by substitution, one row (L21) or one column (U12) per thread.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double bcast_lane(double v, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ float bcast_lane(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
-template <typename T>
-__global__ __launch_bounds__(64) void diag_lu_wave_kernel(T* __restrict__ A, int64_t lda, int nb,
-                                                          int* __restrict__ info, int col0) {
-  const int i = threadIdx.x;
-  T a[64];
-#pragma unroll
-  for (int c = 0; c < 64; ++c)
-    a[c] = (i < nb && c < nb) ? A[i + (int64_t)c * lda] : T(i == c ? 1 : 0);  // identity padding
-#pragma unroll
-  for (int j = 0; j < 64; ++j) {
-    const T piv = bcast_lane(a[j], j);
-    if (i == 0 && j < nb && piv == T(0)) atomicCAS(info, 0, col0 + j + 1);
-    const T l = i > j ? a[j] / piv : T(0);
-    if (i > j) a[j] = l;
-#pragma unroll
-    for (int c = j + 1; c < 64; ++c) a[c] = __builtin_fma(-l, bcast_lane(a[c], j), a[c]);
-  }
-  if (i < nb) {
-#pragma unroll
-    for (int c = 0; c < 64; ++c)
-      if (c < nb) A[i + (int64_t)c * lda] = a[c];
-  }
-}
-
-// Rows [0, m) of B (m x nb, ld ldb): B := B inv(U), U the upper triangle of the
-// nb x nb block D (ld ldd). One thread per row, U staged in LDS.
-template <typename T>
-__global__ __launch_bounds__(256) void trsm_rows_upper_kernel(const T* __restrict__ D, int64_t ldd,
-                                                              int nb, T* __restrict__ B,
-                                                              int64_t ldb, int m) {
-  __shared__ T u[64][65];
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int r = e & 63, c = e >> 6;
-    u[r][c] = (r < nb && c < nb) ? D[r + (int64_t)c * ldd] : T(r == c ? 1 : 0);
-  }
-  __syncthreads();
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= m) return;
-  T x[64];
-#pragma unroll
-  for (int c = 0; c < 64; ++c) x[c] = c < nb ? B[r + (int64_t)c * ldb] : T(0);
-#pragma unroll
-  for (int c = 0; c < 64; ++c) {
-    T v = x[c];
-#pragma unroll
-    for (int k = 0; k < c; ++k) v = __builtin_fma(-x[k], u[k][c], v);
-    x[c] = v / u[c][c];
-  }
-#pragma unroll
-  for (int c = 0; c < 64; ++c)
-    if (c < nb) B[r + (int64_t)c * ldb] = x[c];
-}
-
-// Columns [0, n) of B (nb x n, ld ldb): B := inv(L) B, L the unit lower triangle
-// of D. One thread per column.
-template <typename T>
-__global__ __launch_bounds__(256) void trsm_cols_unit_lower_kernel(const T* __restrict__ D,
-                                                                   int64_t ldd, int nb,
-                                                                   T* __restrict__ B, int64_t ldb,
-                                                                   int n) {
-  __shared__ T l[64][65];
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int r = e & 63, c = e >> 6;
-    l[r][c] = (r < nb && c < nb) ? D[r + (int64_t)c * ldd] : T(0);
-  }
-  __syncthreads();
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= n) return;
-  T* col = B + (int64_t)c * ldb;
-  T y[64];
-#pragma unroll
-  for (int i = 0; i < 64; ++i) y[i] = i < nb ? col[i] : T(0);
-#pragma unroll
-  for (int i = 1; i < 64; ++i) {
-    T v = y[i];
-#pragma unroll
-    for (int k = 0; k < i; ++k) v = __builtin_fma(-l[i][k], y[k], v);
-    y[i] = v;
-  }
-#pragma unroll
-  for (int i = 0; i < 64; ++i)
-    if (i < nb) col[i] = y[i];
-}
-
-template <typename T>
-int diag_lu_wave(hipStream_t s, T* A, int64_t lda, int nb, int* info, int col0) {
-  MPDOK_REQUIRE(nb >= 1 && nb <= 64, "diag_lu_wave: nb=%d", nb);
-  diag_lu_wave_kernel<T><<<1, 64, 0, s>>>(A, lda, nb, info, col0);
-  MPDOK_HIP(hipGetLastError());
-  return OK;
-}
-template <typename T>
-int trsm_rows_upper(hipStream_t s, const T* D, int64_t ldd, int nb, T* B, int64_t ldb, int m) {
-  if (m <= 0) return OK;
-  trsm_rows_upper_kernel<T><<<(m + 255) / 256, 256, 0, s>>>(D, ldd, nb, B, ldb, m);
-  MPDOK_HIP(hipGetLastError());
-  return OK;
-}
-template <typename T>
-int trsm_cols_unit_lower(hipStream_t s, const T* D, int64_t ldd, int nb, T* B, int64_t ldb, int n) {
-  if (n <= 0) return OK;
-  trsm_cols_unit_lower_kernel<T><<<(n + 255) / 256, 256, 0, s>>>(D, ldd, nb, B, ldb, n);
-  MPDOK_HIP(hipGetLastError());
-  return OK;
-}
-template int diag_lu_wave<double>(hipStream_t, double*, int64_t, int, int*, int);
-template int diag_lu_wave<float>(hipStream_t, float*, int64_t, int, int*, int);
-template int trsm_rows_upper<double>(hipStream_t, const double*, int64_t, int, double*, int64_t, int);
-template int trsm_rows_upper<float>(hipStream_t, const float*, int64_t, int, float*, int64_t, int);
-template int trsm_cols_unit_lower<double>(hipStream_t, const double*, int64_t, int, double*, int64_t,
-                                          int);
-template int trsm_cols_unit_lower<float>(hipStream_t, const float*, int64_t, int, float*, int64_t,
-                                         int);
 template int diag_factor<double>(hipStream_t, double*, int64_t, int, double*, double*, int, int*,
                                  int, int);
 template int diag_factor<float>(hipStream_t, float*, int64_t, int, float*, float*, int, int*, int,

@@ -8,16 +8,26 @@
 //     a = 2^e * sum_q d_q 2^(-7 q)      (truncation < 2^(e - 7 S)).
 // Product: for each level l = i + j in 2..S+1 the exact integer
 //     acc_l = sum_{i+j=l} A_i B_j       (int32; |acc_l| <= S K 127^2)
-// comes from v_mfma_i32_16x16x64_i8; pairs with i + j > S + 1 are dropped. The
+// comes from v_mfma_i32_32x32x32_i8; pairs with i + j > S + 1 are dropped. The
 // epilogue folds c = sum_l acc_l 2^(-7 l) in FP64 for l ascending from 2 and
 // subtracts 2^(eA_i + eB_j) c from C. Slicing, integer sums, the fold order and
 // the power-of-two scalings are all deterministic, so the result is bit for bit
 // the one of the oracle's ozaki_matmul.
 //
-// Kernel shape: 256 threads = 2x2 wavefronts, wavefront tile 64 x 32 (4 x 2
-// MFMA tiles of 16 x 16), block tile 128 x 64, K step 64. For one K step the S
-// slices of both operands are staged in LDS once and all S(S+1)/2 slice pairs
-// are multiplied from there, with one int32 accumulator set per level.
+// Layout of the slices (K-blocked): As[q][k / 32][row][k % 32], one 32-byte
+// record per (row, 32-wide K block), so that the bytes one K step of one block
+// tile needs are a single contiguous run per slice, copied straight into LDS.
+//
+// Kernel shape (default): 256 threads = 2 x 2 wavefronts, two workgroups per
+// CU; wavefront tile 64 x 32 (two v_mfma_i32_32x32x32_i8 tiles), block tile
+// 128 x 64, K step 32. One K step of all S slices of both operands is a
+// pipeline stage in LDS, filled by global_load_lds_dwordx4 (global memory to
+// LDS without registers) one stage ahead of its use; a stage is waited for
+// with s_waitcnt vmcnt and a plain s_barrier, so the copies of the next stage
+// stay in flight while the matrix cores work on the current one. All S(S+1)/2
+// slice pairs of a stage are multiplied from LDS, with one int32 accumulator
+// set per level (32 S accumulator registers per lane). Other block shapes
+// (more wavefronts, stages, one workgroup per CU) are selectable for tuning.
 #include "common.h"
 #include "lu.h"
 
@@ -25,7 +35,7 @@ namespace mpdok {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int OZ_BM = 128, OZ_BN = 64, OZ_BK = 64;
+constexpr int OZ_BM = 128, OZ_BK = 32;  // block rows, K step (bytes of one record)
 
 // Digits of v (|v| < 2^e) into d[0..S).
 template <int S>
@@ -41,8 +51,14 @@ __device__ __forceinline__ void make_digits(double v, int e, signed char* d) {
   }
 }
 
-// Rows of A (m x K, column-major, lda) -> As[q][row][k] (row stride Kp),
-// rows padded to mp, k padded to Kp with zeros; ea[row] = exponent.
+// Byte offset of digit (row, k) inside one slice of the K-blocked layout with
+// `rows` padded rows.
+__device__ __forceinline__ size_t oz_rec(int rows, int row, int k) {
+  return (size_t)(k / OZ_BK) * rows * OZ_BK + (size_t)row * OZ_BK + (k % OZ_BK);
+}
+
+// Rows of A (m x K, column-major, lda) -> As[q] (K-blocked), rows padded to
+// mp, k padded to Kp with zeros; ea[row] = exponent.
 template <int S>
 __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restrict__ A, int64_t lda,
                                                          int m, int K, int mp, int Kp,
@@ -79,7 +95,7 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
     // thread -> (row rr, 16 consecutive k)
     const int rr = t >> 2, kq = (t & 3) * 16;
     const int ii = row0 + rr;
-    if (ii < mp) {
+    if (ii < mp && k0 + kq < Kp) {
       signed char dig[S][16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
@@ -92,14 +108,14 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
       for (int q = 0; q < S; ++q) {
         v4i w;
         __builtin_memcpy(&w, dig[q], 16);
-        *reinterpret_cast<v4i*>(As + q * slab + (size_t)ii * Kp + k0 + kq) = w;
+        *reinterpret_cast<v4i*>(As + q * slab + oz_rec(mp, ii, k0 + kq)) = w;
       }
     }
     __syncthreads();
   }
 }
 
-// Columns of B (K x n, column-major, ldb) -> Bs[q][col][k]; eb[col].
+// Columns of B (K x n, column-major, ldb) -> Bs[q] (K-blocked); eb[col].
 template <int S>
 __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restrict__ B, int64_t ldb,
                                                          int n, int K, int np, int Kp,
@@ -133,155 +149,219 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
     for (int q = 0; q < S; ++q) {
       v4i w;
       __builtin_memcpy(&w, dig[q], 16);
-      *reinterpret_cast<v4i*>(Bs + q * slab + (size_t)j * Kp + k0) = w;
+      *reinterpret_cast<v4i*>(Bs + q * slab + oz_rec(np, j, k0)) = w;
     }
   }
 }
 
-// C (m x n, column-major, ldc) -= sliced product. mp, np: padded row/column
-// counts of the slice arrays (multiples of 128), Kp multiple of 64.
-// 4 wavefronts (2 x 2), block tile 128 x 64. PREFETCH: stage the next K step
-// through registers while the current one is multiplied (more registers,
-// one workgroup per CU); without it the kernel is lean enough for two
-// workgroups per CU at S <= 5, which overlap each other's staging.
-// LDS rows hold the 64 bytes of one K step of one row (column) of a slice.
-// SWZ: 64-byte rows with the four 16-byte chunks permuted by
-// q ^ (((row >> 1) ^ (row >> 2)) & 3), so that the 8 (or 16) lanes served
-// together by the LDS, which read or write consecutive rows, always hit
-// distinct bank groups; otherwise 80-byte rows (16 bytes of padding).
-template <bool SWZ>
-__device__ __forceinline__ int lds_off(int row, int q) {
-  return SWZ ? row * 64 + 16 * (q ^ (((row >> 1) ^ (row >> 2)) & 3)) : row * 80 + 16 * q;
+// C (m x n, column-major, ldc) -= sliced product, columns [col0, col0 + ncols).
+// mp, np: padded row / column counts of the slice arrays; Kp a multiple of 128.
+//
+// MFMA operands: lane l takes row (column) l % 32 and bytes [16 (l / 32),
+// 16 (l / 32) + 16) of its 32-byte record, identically for A and B (the order
+// of k inside a product does not matter, only that both operands agree);
+// D[i][j] sits in lane j + 32 u, register r with i = 8 (r / 4) + 4 u + r % 4.
+constexpr int oz_vmcnt(int n) {  // s_waitcnt immediate: vmcnt = n, lgkmcnt / expcnt not waited
+  return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8);
+}
+constexpr int OZ_LGKM0 = 15 | (3 << 14) | (7 << 4);  // s_waitcnt immediate: lgkmcnt = 0 only
+
+// Block shape: WM x WN wavefronts, each owning 64 rows x 32 columns of the
+// block tile (two 32 x 32 MFMA tiles), so BM = 64 WM, BN = 32 WN; NST
+// pipeline stages; OCC workgroups per CU (launch bound).
+template <int S, int WM, int WN, int NST, int OCC>
+struct OzCfg {
+  static constexpr int NT = 64 * WM * WN;  // threads
+  static constexpr int BM = 64 * WM, BN = 32 * WN;
+  // Copies of an operand with R records per slice: 2 R chunks of 16 bytes per
+  // slice, NT / (2 R) slices per round, every thread busy in every round (a
+  // round past the last slice repeats it into a spare slot that is never read).
+  static constexpr int SPR_A = NT / (2 * BM), SPR_B = NT / (2 * BN);
+  static constexpr int RND_A = (S + SPR_A - 1) / SPR_A, RND_B = (S + SPR_B - 1) / SPR_B;
+  static constexpr int ABYTES = RND_A * SPR_A * BM * OZ_BK;
+  static constexpr int BBYTES = RND_B * SPR_B * BN * OZ_BK;
+  static constexpr int STAGE = ABYTES + BBYTES;
+  static constexpr int COPIES = RND_A + RND_B;  // per thread and stage
+  // epilogue staging: per wavefront 32 rows x 32 columns of FP64 (column stride 33)
+  static constexpr int WAVES_PER_BUF = (WM * WN + NST - 1) / NST;
+  static constexpr int EPI = WAVES_PER_BUF * 32 * 33 * 8;
+  static constexpr int BUF = STAGE > EPI ? STAGE : EPI;
+  static constexpr bool FITS = NST * BUF <= 160 * 1024;  // LDS of one workgroup
+  static_assert(NT % (2 * BM) == 0 && NT % (2 * BN) == 0, "copy split");
+};
+
+// One operand of one K step: R records (32 bytes each) of each of S slices.
+template <int R, int SPR, int RND, int S>
+__device__ __forceinline__ void oz_issue_op(signed char* __restrict__ dst, const signed char* src,
+                                            size_t slab, int wave) {
+  const int sub = (64 * wave) / (2 * R);         // slice within the round (wave-uniform)
+  const int off = ((64 * wave) % (2 * R)) * 16;  // byte offset of this wavefront's 64 chunks
+#pragma unroll
+  for (int rd = 0; rd < RND; ++rd) {
+    const int q = rd * SPR + sub;      // LDS slot
+    const int qs = q < S ? q : S - 1;  // source slice
+    __builtin_amdgcn_global_load_lds(src + qs * slab, dst + q * R * OZ_BK + off, 16, 0, 0);
+  }
 }
 
-// Columns [col0, col0 + ncols) of the product (col0 a multiple of 64).
-template <int S, bool PREFETCH, bool SWZ>
-__global__ __launch_bounds__(256, (PREFETCH || S > 5) ? 1 : 2) void ozaki_gemm_kernel(int m, int n, int mp, int np, int Kp,
-                                                             int col0, int ncols,
-                                                             const signed char* __restrict__ As,
-                                                             const signed char* __restrict__ Bs,
-                                                             const int* __restrict__ ea,
-                                                             const int* __restrict__ eb,
-                                                             double* __restrict__ C, int64_t ldc) {
-  constexpr int WN = 2, NT = 128 * WN, BN = 32 * WN;
-  constexpr int ROWB = SWZ ? 64 : 80;
-  constexpr int NCA = OZ_BM * 4 / NT, NCB = BN * 4 / NT;  // 16-byte chunks per thread
-  __shared__ __attribute__((aligned(16))) signed char sA[S][OZ_BM * ROWB];
-  __shared__ __attribute__((aligned(16))) signed char sB[S][BN * ROWB];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
-  int ti, tj;
-  if (!tile_of(blockIdx.x, mp / OZ_BM, (ncols + BN - 1) / BN, ti, tj)) return;
-  const int bm = ti * OZ_BM, bn = col0 + tj * BN;
-  const size_t slabA = (size_t)mp * Kp, slabB = (size_t)np * Kp;
+// One K step (32 bytes of every row / column of all S slices) into a stage.
+template <class Cfg, int S>
+__device__ __forceinline__ void oz_issue(signed char* __restrict__ st, const signed char* a,
+                                         const signed char* b, size_t slabA, size_t slabB,
+                                         int wave) {
+  oz_issue_op<Cfg::BM, Cfg::SPR_A, Cfg::RND_A, S>(st, a, slabA, wave);
+  oz_issue_op<Cfg::BN, Cfg::SPR_B, Cfg::RND_B, S>(st + Cfg::ABYTES, b, slabB, wave);
+}
 
-  v4i acc[S][4][2];
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// 2^(-7 (l + 2)): weight of level l (0-based) in the fold; exact power of two.
+__device__ __forceinline__ constexpr double oz_level_scale(int l) {
+  double w = 1.0;
+  for (int i = 0; i < 7 * (l + 2); ++i) w *= 0.5;
+  return w;
+}
+
+template <class Cfg, int S>
+__device__ __forceinline__ void oz_compute(const signed char* __restrict__ st, v16i (&acc)[S][2],
+                                           int wm, int wn, int lane) {
+  const int off = (lane & 31) * OZ_BK + 16 * (lane >> 5);
+  v4i b[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+    b[j] = *reinterpret_cast<const v4i*>(st + Cfg::ABYTES + j * Cfg::BN * OZ_BK + wn * OZ_BK + off);
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    v4i a[2];
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+      a[ta] = *reinterpret_cast<const v4i*>(st + i * Cfg::BM * OZ_BK + (wm + 32 * ta) * OZ_BK + off);
+#pragma unroll
+    for (int j = 0; j < S - i; ++j)  // level index i + j (0-based)
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+        acc[i + j][ta] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ta], b[j], acc[i + j][ta], 0, 0, 0);
+  }
+}
+
+// One pipeline step on stage `st`: wait for its copies (those of the NST - 1
+// later stages stay in flight), multiply, free it, refill it.
+template <class Cfg, int S, int NST>
+__device__ __forceinline__ void oz_step(signed char* __restrict__ st, v16i (&acc)[S][2],
+                                        const signed char* a, const signed char* b, size_t slabA,
+                                        size_t slabB, int wm, int wn, int lane, int wave) {
+  __builtin_amdgcn_s_waitcnt(oz_vmcnt((NST - 1) * Cfg::COPIES));  // this thread's copies landed
+  __builtin_amdgcn_s_barrier();                                   // ... and every thread's
+  oz_compute<Cfg, S>(st, acc, wm, wn, lane);
+  __builtin_amdgcn_s_waitcnt(OZ_LGKM0);  // this thread's reads of the stage are done
+  __builtin_amdgcn_s_barrier();          // stage free
+  oz_issue<Cfg, S>(st, a, b, slabA, slabB, wave);
+}
+
+// Kp: a multiple of 32 NST, so that the K steps come in whole rounds of stages.
+template <int S, int WM, int WN, int NST, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemm_kernel(
+    int m, int n, int mp, int np, int Kp, int col0, int ncols, const signed char* __restrict__ As,
+    const signed char* __restrict__ Bs, const int* __restrict__ ea, const int* __restrict__ eb,
+    double* __restrict__ C, int64_t ldc) {
+  typedef OzCfg<S, WM, WN, NST, OCC> Cfg;
+  // one array per pipeline stage: the compiler then tells the stages apart
+  // and waits only for the copies into the stage being read
+  __shared__ __attribute__((aligned(16))) signed char st0[Cfg::BUF];
+  __shared__ __attribute__((aligned(16))) signed char st1[Cfg::BUF];
+  __shared__ __attribute__((aligned(16))) signed char st2[NST > 2 ? Cfg::BUF : 16];
+  __shared__ __attribute__((aligned(16))) signed char st3[NST > 3 ? Cfg::BUF : 16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave % WM) * 64, wn = (wave / WM) * 32;
+  int ti, tj;
+  if (!tile_of(blockIdx.x, mp / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN, ti, tj)) return;
+  const int bm = ti * Cfg::BM, bn = col0 + tj * Cfg::BN;
+  const size_t slabA = (size_t)mp * Kp, slabB = (size_t)np * Kp;
+  const size_t kstepA = (size_t)mp * OZ_BK, kstepB = (size_t)np * OZ_BK;
+  const signed char* Ab = As + (size_t)bm * OZ_BK + 16 * (tid % (2 * Cfg::BM));
+  const signed char* Bb = Bs + (size_t)bn * OZ_BK + 16 * (tid % (2 * Cfg::BN));
+
+  v16i acc[S][2];
 #pragma unroll
   for (int l = 0; l < S; ++l)
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) acc[l][a][b] = v4i(0);
+    for (int a = 0; a < 2; ++a) acc[l][a] = v16i(0);
 
-  // register prefetch of one K step (16-byte chunks: row = chunk / 4, k = 16 (chunk % 4))
-  v4i pa[PREFETCH ? S : 1][NCA], pb[PREFETCH ? S : 1][NCB];
-  auto gload = [&](int k0) {
-    if (!PREFETCH) return;
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-#pragma unroll
-      for (int u = 0; u < NCA; ++u) {
-        const int c = tid + u * NT;
-        pa[PREFETCH ? q : 0][u] = *reinterpret_cast<const v4i*>(As + q * slabA + (size_t)(bm + (c >> 2)) * Kp + k0 +
-                                                 (c & 3) * 16);
-      }
-#pragma unroll
-      for (int u = 0; u < NCB; ++u) {
-        const int c = tid + u * NT;
-        pb[PREFETCH ? q : 0][u] = *reinterpret_cast<const v4i*>(Bs + q * slabB + (size_t)(bn + (c >> 2)) * Kp + k0 +
-                                                 (c & 3) * 16);
-      }
+  // Stage pipeline without branches in the loop body (the accumulators then
+  // stay in place across iterations): every thread has Cfg::COPIES copies per
+  // stage in flight; past the last K step the refills repeat the last step
+  // (harmless, never read) so that the wait counts stay the same.
+  const int nk = Kp / OZ_BK;
+  oz_issue<Cfg, S>(st0, Ab, Bb, slabA, slabB, wave);
+  oz_issue<Cfg, S>(st1, Ab + kstepA, Bb + kstepB, slabA, slabB, wave);
+  if (NST > 2) oz_issue<Cfg, S>(st2, Ab + 2 * kstepA, Bb + 2 * kstepB, slabA, slabB, wave);
+  if (NST > 3) oz_issue<Cfg, S>(st3, Ab + 3 * kstepA, Bb + 3 * kstepB, slabA, slabB, wave);
+  for (int t = 0; t < nk; t += NST) {
+    int k = min(t + NST, nk - 1);
+    oz_step<Cfg, S, NST>(st0, acc, Ab + k * kstepA, Bb + k * kstepB, slabA, slabB, wm, wn, lane, wave);
+    k = min(t + NST + 1, nk - 1);
+    oz_step<Cfg, S, NST>(st1, acc, Ab + k * kstepA, Bb + k * kstepB, slabA, slabB, wm, wn, lane, wave);
+    if (NST > 2) {
+      k = min(t + NST + 2, nk - 1);
+      oz_step<Cfg, S, NST>(st2, acc, Ab + k * kstepA, Bb + k * kstepB, slabA, slabB, wm, wn, lane, wave);
     }
-  };
-  auto sstore = [&](int k0) {
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-#pragma unroll
-      for (int u = 0; u < NCA; ++u) {
-        const int c = tid + u * NT;
-        *reinterpret_cast<v4i*>(&sA[q][lds_off<SWZ>(c >> 2, c & 3)]) =
-            PREFETCH ? pa[PREFETCH ? q : 0][u]
-                     : *reinterpret_cast<const v4i*>(As + q * slabA + (size_t)(bm + (c >> 2)) * Kp +
-                                                     k0 + (c & 3) * 16);
-      }
-#pragma unroll
-      for (int u = 0; u < NCB; ++u) {
-        const int c = tid + u * NT;
-        *reinterpret_cast<v4i*>(&sB[q][lds_off<SWZ>(c >> 2, c & 3)]) =
-            PREFETCH ? pb[PREFETCH ? q : 0][u]
-                     : *reinterpret_cast<const v4i*>(Bs + q * slabB + (size_t)(bn + (c >> 2)) * Kp +
-                                                     k0 + (c & 3) * 16);
-      }
+    if (NST > 3) {
+      k = min(t + NST + 3, nk - 1);
+      oz_step<Cfg, S, NST>(st3, acc, Ab + k * kstepA, Bb + k * kstepB, slabA, slabB, wm, wn, lane, wave);
     }
-  };
-  const int fr = lane & 15, fq = lane >> 4;
-  gload(0);
-  for (int k0 = 0; k0 < Kp; k0 += OZ_BK) {
-    sstore(k0);
-    __syncthreads();
-    if (k0 + OZ_BK < Kp) gload(k0 + OZ_BK);
-    // all B-slice fragments of this K step stay in registers across the i loop
-    v4i b[S][2];
-#pragma unroll
-    for (int j = 0; j < S; ++j)
-#pragma unroll
-      for (int tb = 0; tb < 2; ++tb)
-        b[j][tb] = *reinterpret_cast<const v4i*>(&sB[j][lds_off<SWZ>(wn + 16 * tb + fr, fq)]);
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      v4i a[4];
-#pragma unroll
-      for (int ta = 0; ta < 4; ++ta)
-        a[ta] = *reinterpret_cast<const v4i*>(&sA[i][lds_off<SWZ>(wm + 16 * ta + fr, fq)]);
-#pragma unroll
-      for (int j = 0; j < S - i; ++j)  // 0-based: level index i + j = l - 2
-#pragma unroll
-        for (int ta = 0; ta < 4; ++ta)
-#pragma unroll
-          for (int tb = 0; tb < 2; ++tb)
-            acc[i + j][ta][tb] =
-                __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ta], b[j][tb], acc[i + j][ta][tb], 0, 0, 0);
-    }
-    __syncthreads();
   }
-  // epilogue: D[i = 4*(lane/16) + r][j = lane%16] per 16 x 16 tile
+  __builtin_amdgcn_s_waitcnt(oz_vmcnt(0));
+  __builtin_amdgcn_s_barrier();
+
+  // Epilogue: fold the levels (FP64, l ascending), scale by 2^(ea_i + eb_j),
+  // stage through LDS so that each store covers 32 consecutive rows of a
+  // column, then C -= value.
+  const int wb = wave % NST, ws = wave / NST;
+  signed char* ebuf = wb == 0 ? st0 : wb == 1 ? st1 : wb == 2 ? st2 : st3;
+  double* epi = reinterpret_cast<double*>(ebuf) + ws * (32 * 33);
+  const int jl = lane & 31;
+  const int ej = eb[bn + wn + jl];  // bn + wn + jl < np
 #pragma unroll
-  for (int ta = 0; ta < 4; ++ta)
+  for (int ta = 0; ta < 2; ++ta) {
 #pragma unroll
-    for (int tb = 0; tb < 2; ++tb) {
-      const int j = bn + wn + 16 * tb + (lane & 15);
-      if (j >= n) continue;
-      const int ej = eb[j];
+    for (int r = 0; r < 16; ++r) {
+      const int il = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      double c = 0.0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = bm + wm + 16 * ta + 4 * (lane >> 4) + r;
-        if (i >= m) continue;
-        double c = 0.0;
-#pragma unroll
-        for (int l = 0; l < S; ++l) c += ldexp((double)acc[l][ta][tb][r], -7 * (l + 2));
+      for (int l = 0; l < S; ++l) c += (double)acc[l][ta][r] * oz_level_scale(l);
+      epi[jl * 33 + il] = ldexp(c, ea[bm + wm + 32 * ta + il] + ej);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // lane -> (row lane % 32, every other column)
+    const int i = bm + wm + 32 * ta + (lane & 31);
+#pragma unroll 4
+    for (int c = lane >> 5; c < 32; c += 2) {
+      const int j = bn + wn + c;
+      const double v = epi[c * 33 + (lane & 31)];
+      if (i < m && j < col0 + ncols) {
         double* cp = C + i + (int64_t)j * ldc;
-        *cp = *cp - ldexp(c, ea[i] + ej);
+        *cp = *cp - v;
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
 }
 
-constexpr int OZ_PAD = 128;  // row and column padding of the slice arrays
+// Slice arrays: rows padded to a multiple of 128; columns to a multiple of 128
+// plus 128 more, so that a column range starting at any multiple of 64 is
+// covered by whole tiles of up to 128 columns.
+constexpr int OZ_PAD = 128;
+static int oz_mp(int m) { return (m + OZ_PAD - 1) / OZ_PAD * OZ_PAD; }
+static int oz_np(int n) { return (n + OZ_PAD - 1) / OZ_PAD * OZ_PAD + OZ_PAD; }
+static int oz_kp(int K) { return (K + 4 * OZ_BK - 1) / (4 * OZ_BK) * (4 * OZ_BK); }  // whole rounds of 2 or 4 stages
 
 size_t ozaki_workspace_bytes(int m, int n, int K, int S) {
-  const int mp = (m + OZ_PAD - 1) / OZ_PAD * OZ_PAD;
-  const int np = (n + OZ_PAD - 1) / OZ_PAD * OZ_PAD;
-  const int Kp = (K + OZ_BK - 1) / OZ_BK * OZ_BK;
-  return (size_t)S * Kp * ((size_t)mp + np) + sizeof(int) * ((size_t)mp + np) + 256;
+  const size_t mp = oz_mp(m), np = oz_np(n), Kp = oz_kp(K);
+  return (size_t)S * Kp * (mp + np) + sizeof(int) * (mp + np) + 256;
 }
 
 struct OzLayout {
@@ -291,9 +371,9 @@ struct OzLayout {
 };
 static OzLayout oz_layout(int S, int m, int n, int K, void* work) {
   OzLayout L;
-  L.mp = (m + OZ_PAD - 1) / OZ_PAD * OZ_PAD;
-  L.np = (n + OZ_PAD - 1) / OZ_PAD * OZ_PAD;
-  L.Kp = (K + OZ_BK - 1) / OZ_BK * OZ_BK;
+  L.mp = oz_mp(m);
+  L.np = oz_np(n);
+  L.Kp = oz_kp(K);
   L.As = static_cast<signed char*>(work);
   L.Bs = L.As + (size_t)S * L.mp * L.Kp;
   L.ea = reinterpret_cast<int*>(L.Bs + (size_t)S * L.np * L.Kp);
@@ -311,22 +391,37 @@ static int ozaki_slice_s(hipStream_t s, int m, int n, int K, const double* A, in
   return OK;
 }
 
+template <int S, int WM, int WN, int NST, int OCC>
+static int ozaki_launch(hipStream_t s, const OzLayout& L, int m, int n, int col0, int ncols,
+                        double* C, int64_t ldc) {
+  typedef OzCfg<S, WM, WN, NST, OCC> Cfg;
+  if constexpr (!Cfg::FITS) {
+    return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // stages too large: smallest shape
+  } else {
+    const int grid = tiles_launch(L.mp / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN);
+    ozaki_gemm_kernel<S, WM, WN, NST, OCC><<<grid, Cfg::NT, 0, s>>>(
+        m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc);
+    MPDOK_HIP(hipGetLastError());
+    return OK;
+  }
+}
+
 template <int S>
 static int ozaki_apply_s(hipStream_t s, int m, int n, int K, int col0, int ncols, double* C,
                          int64_t ldc, void* work) {
   const OzLayout L = oz_layout(S, m, n, K, work);
-  const int grid = tiles_launch(L.mp / OZ_BM, (ncols + 63) / 64);
-  const int pf = tuning_get(TUNE_OZAKI_PREFETCH), swz = tuning_get(TUNE_OZAKI_SWIZZLE);
-#define OZ_LAUNCH(PF, SW)                                                                  \
-  ozaki_gemm_kernel<S, PF, SW><<<grid, 256, 0, s>>>(m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, \
-                                                    L.Bs, L.ea, L.eb, C, ldc)
-  if (pf && swz) OZ_LAUNCH(true, true);
-  else if (pf) OZ_LAUNCH(true, false);
-  else if (swz) OZ_LAUNCH(false, true);
-  else OZ_LAUNCH(false, false);
-#undef OZ_LAUNCH
-  MPDOK_HIP(hipGetLastError());
-  return OK;
+  // shape 0 (default): 128 x 64 tiles, two stages, two workgroups per CU, which
+  // overlap each other's epilogue (the FP64 read-modify-write of C) and copies;
+  // measured fastest at S = 5, 6 on 32256^2 x 512 (1.48 / 1.77 int8 POP/s).
+  // S = 7 needs too many registers for two workgroups per CU: one.
+  switch (tuning_get(TUNE_OZAKI_SHAPE)) {
+    case 1: return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 1 per CU
+    case 2: return ozaki_launch<S, 2, 4, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 128 x 128
+    case 3: return ozaki_launch<S, 2, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);  // four stages
+    default:
+      if constexpr (S <= 6) return ozaki_launch<S, 2, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 7 S accumulators
+  }
 }
 
 // Slice A (m x K) by rows and B (K x n) by columns into the workspace.

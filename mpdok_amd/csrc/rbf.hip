@@ -153,7 +153,111 @@ int rbf_mean_d2(hipStream_t st, const double* coords, int n, int D, double* out)
   return OK;
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-free product y = K[row0 : row0 + rows, :] x with every kernel value
+// recomputed from the coordinates and never stored (mpdok_ooc.py:205-227, the
+// FP64 outer residual; and the matrix-free form of the FP32 inner product,
+// mpdok_ooc.py:39-46,158-201, whose elements are the stored tile's values).
+//
+// ELEM = F64: element = kernel value + reg on the diagonal, in FP64 (a25).
+// ELEM = F32: element = FP32 rounding of the kernel value, reg added in FP32
+//             after the cast: bit for bit the element of the stored FP32 tile.
+// Products and sums are FP64; y gets the sum rounded once to TY.
+//
+// Block: 32 rows x 8 column phases (256 threads). Coordinates, squared norms
+// and x of a 256-column chunk are staged in LDS; a wavefront's two half-waves
+// read two columns per step (broadcast reads). The 8 partial sums of a row are
+// added in a fixed order, so results do not depend on timing.
+// ---------------------------------------------------------------------------
+template <int ELEM, typename TX, typename TY>
+__global__ __launch_bounds__(256) void rbf_matvec_kernel(const double* __restrict__ coords, int N,
+                                                         int D, int kind, double p, double reg,
+                                                         int row0, int rows,
+                                                         const TX* __restrict__ x,
+                                                         TY* __restrict__ y) {
+  __shared__ double cj[MAXD][256];
+  __shared__ double sqj[256];
+  __shared__ double xj[256];
+  __shared__ double part[8][33];
+  const int t = threadIdx.x, r = t & 31, ph = t >> 5;
+  const int il = blockIdx.x * 32 + r;
+  const int i = row0 + il;
+  const bool live = il < rows;
+  double ci[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) ci[d] = (live && d < D) ? coords[(int64_t)i * D + d] : 0.0;
+  const double sqi = sqnorm(ci, D);
+  double acc = 0.0;
+  for (int j0 = 0; j0 < N; j0 += 256) {
+    const int j = j0 + t;
+    double c[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) c[d] = (j < N && d < D) ? coords[(int64_t)j * D + d] : 0.0;
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d)
+      if (d < D) cj[d][t] = c[d];
+    sqj[t] = sqnorm(c, D);
+    xj[t] = j < N ? (double)x[j] : 0.0;
+    __syncthreads();
+    const int jn = N - j0 < 256 ? N - j0 : 256;
+    for (int k = ph; k < jn; k += 8) {
+      double cc[MAXD];
+#pragma unroll
+      for (int d = 0; d < MAXD; ++d) cc[d] = d < D ? cj[d][k] : 0.0;
+      const double v = kernel_value(kind, p, gram_d2(ci, sqi, cc, sqj[k], D));
+      const bool diag = i == j0 + k;
+      double e;
+      if (ELEM == F64) {
+        e = diag ? v + reg : v;
+      } else {
+        float f = (float)v;
+        if (diag) f = f + (float)reg;
+        e = (double)f;
+      }
+      acc = __builtin_fma(e, xj[k], acc);
+    }
+    __syncthreads();
+  }
+  part[ph][r] = acc;
+  __syncthreads();
+  if (ph == 0 && live) {
+    const double s = ((part[0][r] + part[1][r]) + (part[2][r] + part[3][r])) +
+                     ((part[4][r] + part[5][r]) + (part[6][r] + part[7][r]));
+    y[il] = (TY)s;
+  }
+}
+
+int rbf_matvec(hipStream_t st, const double* coords, int N, int D, int kind, double p, double reg,
+               int elem_dtype, const void* x, void* y, int xy_dtype, int row0, int rows) {
+  MPDOK_REQUIRE(coords && x && y, "rbf_matvec: null pointer");
+  MPDOK_REQUIRE(D >= 1 && D <= MAXD, "rbf_matvec: D must be in [1, %d], got %d", MAXD, D);
+  MPDOK_REQUIRE(kind == 0 || kind == 1, "rbf_matvec: unknown kernel kind %d", kind);
+  MPDOK_REQUIRE(!(kind == 1 && !(p > 0.0)), "rbf_matvec: Matern length scale must be > 0");
+  MPDOK_REQUIRE(row0 >= 0 && rows >= 0 && row0 + rows <= N, "rbf_matvec: rows outside N");
+  MPDOK_REQUIRE(elem_dtype == F64 || elem_dtype == F32, "rbf_matvec: element dtype %d", elem_dtype);
+  MPDOK_REQUIRE(xy_dtype == F64 || xy_dtype == F32, "rbf_matvec: vector dtype %d", xy_dtype);
+  if (rows == 0) return OK;
+  const int grid = (rows + 31) / 32;
+#define LAUNCH(E, T)                                                                   \
+  rbf_matvec_kernel<E, T, T><<<grid, 256, 0, st>>>(coords, N, D, kind, p, reg, row0, rows, \
+                                                   (const T*)x, (T*)y)
+  if (elem_dtype == F64) {
+    if (xy_dtype == F64) LAUNCH(F64, double); else LAUNCH(F64, float);
+  } else {
+    if (xy_dtype == F64) LAUNCH(F32, double); else LAUNCH(F32, float);
+  }
+#undef LAUNCH
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
 }  // namespace mpdok
+
+MPDOK_API int mpdok_rbf_matvec(hipStream_t st, const double* coords, int N, int D, int kind,
+                               double p, double reg, int elem_dtype, const void* x, void* y,
+                               int xy_dtype, int row0, int rows) {
+  return mpdok::rbf_matvec(st, coords, N, D, kind, p, reg, elem_dtype, x, y, xy_dtype, row0, rows);
+}
 
 MPDOK_API int mpdok_rbf_assemble(hipStream_t st, const double* coords, int N, int D, int kind,
                                  double p, double reg, int reg_mode, int row0, int rows, int col0,

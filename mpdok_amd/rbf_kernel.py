@@ -49,23 +49,25 @@ def _estimate_gamma(coords, max_sample=512):
 
 
 def _build_rbf_cpu_into(coords_np, gamma, reg, A_np, chunk=512, verbose=False):
-    """Host-side build into a Fortran-order host array, column chunk by column
-    chunk (the reference's CPU path for managed buffers larger than 90% of
-    device memory). Runs on the CPU by definition; no GPU path calls it."""
+    """Fill a Fortran-order host array (the reference's path for managed
+    buffers larger than 90% of device memory, rbf_kernel.py:126-155) column
+    chunk by column chunk. Each N x chunk block is assembled on the GPU by the
+    same kernel as build_rbf_kernel (Gaussian, nugget added in FP64) and
+    copied into its columns of A_np; only chunk columns are device-resident at
+    a time."""
     N = coords_np.shape[0]
-    row_norms = np.sum(coords_np ** 2, axis=1)
+    coords = _dev.as_device(np.ascontiguousarray(coords_np, dtype=np.float64), torch.float64)
+    chunk = max(1, min(int(chunk), N))
+    stage = _dev.empty((N, chunk), torch.float64, "F")
     n_chunks = (N + chunk - 1) // chunk
     for ci, j in enumerate(range(0, N, chunk)):
         end = min(j + chunk, N)
-        gram_chunk = coords_np @ coords_np[j:end].T
-        D2 = row_norms[:, None] + row_norms[None, j:end] - 2.0 * gram_chunk
-        np.maximum(D2, 0.0, out=D2)
-        col_data = np.exp(-gamma * D2)
-        for k in range(j, end):
-            col_data[k, k - j] += reg
-        A_np[:, j:end] = col_data
+        blk = stage[:, :end - j]
+        _ops.rbf_assemble(coords, GAUSSIAN, gamma, reg, blk, row0=0, rows=N, col0=j,
+                          cols=end - j)
+        A_np[:, j:end] = blk.cpu().numpy()
         if verbose and (ci % 10 == 0 or ci == n_chunks - 1):
-            print(f"    CPU build: chunk {ci+1}/{n_chunks}  "
+            print(f"    host build: chunk {ci+1}/{n_chunks}  "
                   f"({(ci+1)*100//n_chunks}%)", end='\r', flush=True)
     if verbose:
         print()

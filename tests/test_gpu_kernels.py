@@ -151,6 +151,20 @@ def test_rbf_assembly_matern32(ops):
     assert np.array_equal(A, A.T)
 
 
+def test_host_array_build_matches_device_build(ops):
+    """_build_rbf_cpu_into (rbf_kernel.py:126-155): the host F-order array
+    gets, chunk by chunk, exactly the values of the device build."""
+    from mpdok_amd.rbf_kernel import _build_rbf_cpu_into
+    N, gamma, reg = 301, 25.0, 1e-6
+    X = np.random.default_rng(5).uniform(0, 1, (N, 3))
+    A_np = np.zeros((N, N), order="F")
+    _build_rbf_cpu_into(X, gamma, reg, A_np, chunk=64)
+    full = ops.rbf_assemble(dev(X), 0, gamma, reg, dev(np.zeros((N, N)), torch.float64, "F"))
+    assert np.array_equal(A_np, full.cpu().numpy())
+    ref = orb.build_kernel(X, "gaussian", gamma, reg)
+    assert np.allclose(A_np, ref, rtol=gamma * 4 * 2.0 ** -53 * 6 + 2.0 ** -52, atol=0)
+
+
 def test_estimate_gamma_matches_oracle():
     from mpdok_amd.rbf_kernel import _estimate_gamma
     X = np.random.default_rng(11).uniform(0, 100, (3000, 2))

@@ -91,6 +91,28 @@ def test_ozaki_gemm_bit_exact_vs_oracle(L, S, m, n, K):
     assert np.array_equal(Cd.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("S,m,n,K", [(5, 257, 129, 300), (6, 640, 384, 512), (3, 130, 260, 96)])
+def test_ozaki_gemm_block_shapes_bit_exact(L, shape, S, m, n, K):
+    """Every block shape of the int8-sliced GEMM (tuning key 0) gives the
+    oracle's bits: shapes only change tiling, pipelining and occupancy."""
+    rng = np.random.default_rng(shape * 7 + S)
+    A = rng.standard_normal((m, K)) * np.exp(rng.uniform(-3, 3, (m, 1)))
+    B = rng.standard_normal((K, n)) * np.exp(rng.uniform(-3, 3, (1, n)))
+    C = rng.standard_normal((m, n))
+    ws = torch.empty(L.mpdok_ozaki_workspace_bytes(m, n, K, S), dtype=torch.uint8, device="cuda")
+    Ad, Bd, Cd = dev(A), dev(B), dev(C)
+    assert L.mpdok_set_tuning(0, shape) == 0
+    try:
+        st = L.mpdok_ozaki_gemm_sub(stream(), S, m, n, K, Ad.data_ptr(), m, Bd.data_ptr(), K,
+                                    Cd.data_ptr(), m, ws.data_ptr())
+        torch.cuda.synchronize()
+    finally:
+        L.mpdok_set_tuning(0, 0)
+    assert st == 0, L.mpdok_last_error()
+    assert np.array_equal(Cd.cpu().numpy(), C - olu.ozaki_matmul(A, B, S))
+
+
 def _factor(L, A, precision, pivot, nb=128, slices=6):
     from mpdok_amd.mpdok_ops import _LUHandle
     h = _LUHandle(L, A.shape[0], precision, pivot, nb, slices)
