@@ -126,6 +126,20 @@ int mpdok_rbf_mean_d2(hipStream_t s, const double* coords, int n, int D, double*
 int mpdok_rbf_matvec(hipStream_t s, const double* coords, int N, int D, int kind, double p,
                      double reg, int elem_dtype, const void* x, void* y, int xy_dtype, int row0,
                      int rows);
+/* Cross-kernel product y = K(R, C) x (FP64, no nugget), R: NR points, C: NC
+ * points, both (n, D) row-major: the interpolant at query points
+ * (rbf_demo.ipynb:527, enkf_solver.py:162-200) and K_v alpha / K_v^T g of the
+ * adjoint. No reference C symbol. */
+int mpdok_rbf_cross_matvec(hipStream_t s, const double* R, int NR, const double* C, int NC, int D,
+                           int kind, double p, const double* x, double* y);
+/* out[0] = u^T (dK(R, C)/dp) w with the kernel-parameter derivative
+ * recomputed per element (Gaussian: -d2 k; Matern-3/2: a^2 exp(-a) / p),
+ * deterministic block reduction. The matrix-free dK/dtheta of the adjoint
+ * (north star; no reference counterpart). */
+size_t mpdok_rbf_dbilinear_workspace_bytes(int NR);
+int mpdok_rbf_dbilinear(hipStream_t s, const double* R, int NR, const double* C, int NC, int D,
+                        int kind, double p, const double* u, const double* w, double* out,
+                        void* work, size_t work_bytes);
 
 /* Out-of-core products (mpdok_ooc.py:158-201). y[0:M] = A_host x for a
  * row-major FP32 A (M x N, leading dimension lda) in host memory (pinned for
@@ -195,6 +209,10 @@ int mpdok_lu_factor(void* handle, hipStream_t s, const double* A, int64_t lda, i
  * (benchmark_mpdok.ipynb:990,992). */
 int mpdok_lu_solve(void* handle, hipStream_t s, const double* b, double* x);
 int mpdok_lu_destroy(void* handle);
+/* Use the first n <= N (N as created) rows and columns of the handle's
+ * buffers for the next factorisation: one handle serves systems of many
+ * sizes without reallocating. No reference counterpart. */
+int mpdok_lu_resize(void* handle, int n);
 /* HIP-event timing of the trailing-update launches of later factorisations;
  * mpdok_lu_stats returns the summed time (ms), the algorithmic operations
  * (int8 operations for the sliced variant, flops otherwise) and the launch
@@ -239,6 +257,19 @@ int mpdok_gmres_ir(hipStream_t s, const double* A64, int64_t lda64, int order64,
 int mpdok_lu_ir(hipStream_t s, const double* A64, int64_t lda, int order, void* lu, int N,
                 const double* b, double* x, double tol, int maxiter_outer, int resid_mode,
                 int* converged, double* hist, int hist_cap, int* n_hist);
+
+/* Batched uneven kernel systems (north star's batched configuration; no
+ * reference counterpart). System i: N[i] points coords[i] (device, N x D
+ * row-major), kernel (kind, param[i]) + reg I, nrhs right-hand sides B[i]
+ * (device, N x nrhs column-major) solved into X[i] (same layout) to relres <=
+ * tol by LU (precision 0 FP64 or 4 int8-sliced, no pivoting) + LU-IR with the
+ * Dot2 residual. nstreams workers (host threads with their own streams),
+ * largest systems first, longest-processing-time assignment. converged and
+ * relres (host, nsys x nrhs) receive the per-right-hand-side outcome. */
+int mpdok_batched_rbf_solve(int nsys, const int* N, const double* const* coords, int D, int kind,
+                            const double* param, double reg, int nrhs, const double* const* B,
+                            double* const* X, double tol, int maxiter_outer, int precision,
+                            int slices, int nb, int nstreams, int* converged, double* relres);
 
 #ifdef __cplusplus
 }

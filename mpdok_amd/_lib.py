@@ -55,6 +55,9 @@ SIGNATURES = {
     "mpdok_rbf_assemble": (I, [P, P, I, I, I, D, D, I, I, I, I, I, P, I, I64, I]),
     "mpdok_rbf_mean_d2": (I, [P, P, I, I, P]),
     "mpdok_rbf_matvec": (I, [P, P, I, I, I, D, D, I, P, P, I, I, I]),
+    "mpdok_rbf_cross_matvec": (I, [P, P, I, P, I, I, I, D, P, P]),
+    "mpdok_rbf_dbilinear_workspace_bytes": (SZ, [I]),
+    "mpdok_rbf_dbilinear": (I, [P, P, I, P, I, I, I, D, P, P, P, P, SZ]),
     # out-of-core streaming
     "mpdok_stream_gemv_workspace_bytes": (SZ, [I, I]),
     "mpdok_stream_gemv_f32": (I, [P, P, I64, I, I, I, P, P, P, SZ]),
@@ -75,6 +78,7 @@ SIGNATURES = {
     "mpdok_lu_factor": (I, [P, P, P, I64, I, PI]),
     "mpdok_lu_solve": (I, [P, P, P, P]),
     "mpdok_lu_destroy": (I, [P]),
+    "mpdok_lu_resize": (I, [P, I]),
     "mpdok_lu_factors": (I, [P, PP, PP, PI]),
     "mpdok_lu_set_profiling": (I, [P, I]),
     "mpdok_lu_set_lookahead": (I, [P, I]),
@@ -86,6 +90,7 @@ SIGNATURES = {
     "mpdok_gmres_ir": (I, [P, P, I64, I, P, I64, I, P, I, P, P, D, I, I, D, I, I, I, PI, P, I,
                            PI, PI]),
     "mpdok_lu_ir": (I, [P, P, I64, I, P, I, P, P, D, I, I, PI, P, I, PI]),
+    "mpdok_batched_rbf_solve": (I, [I, P, P, I, I, P, D, I, P, P, D, I, I, I, I, I, P, P]),
 }
 
 F64, F32, BF16, F16 = 0, 1, 2, 3

@@ -24,6 +24,7 @@ int tuning_set(int key, int value) {
   return OK;
 }
 
+const char* error_string() { return g_err; }
 void set_error(const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);

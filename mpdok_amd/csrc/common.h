@@ -23,6 +23,7 @@ enum Status : int {
 
 // Thread-local last-error text, readable through mpdok_last_error().
 void set_error(const char* fmt, ...);
+const char* error_string();  // this thread's last error message
 
 // Process-wide kernel-shape choices (set by the tuner through
 // mpdok_set_tuning; read at launch time).

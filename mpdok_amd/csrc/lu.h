@@ -55,5 +55,8 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
 int lu_solve(LUHandle* h, hipStream_t s, const double* b, double* x);
 void lu_destroy(LUHandle* h);
 int lu_n(const LUHandle* h);
+// Use the first n <= capacity rows / columns of the buffers for the next
+// factorisation (the handle's buffers are sized for the N it was created with).
+int lu_resize(LUHandle* h, int n);
 
 }  // namespace mpdok

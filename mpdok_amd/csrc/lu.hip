@@ -32,6 +32,7 @@ namespace mpdok {
 
 struct LUHandle {
   int N = 0, precision = 0, pivot = 0, nb = 512, slices = 6;
+  int cap = 0;  // largest N the buffers hold (lu_resize)
   bool factored = false;
   size_t es = 8;
   void* LU = nullptr;
@@ -67,6 +68,15 @@ struct LUHandle {
 
 int lu_n(const LUHandle* h) { return h ? h->N : 0; }
 
+int lu_resize(LUHandle* h, int n) {
+  MPDOK_REQUIRE(h != nullptr, "lu_resize: null handle");
+  MPDOK_REQUIRE(n >= 1 && n <= h->cap, "lu_resize: n=%d outside [1, %d]", n, h->cap);
+  h->N = n;
+  h->fast_solve = (n % TRSV_BLOCK) == 0 && h->Lbinv != nullptr;
+  h->factored = false;
+  return OK;
+}
+
 static bool fp64_storage(int precision) { return precision == LU_FP64 || precision == LU_OZAKI; }
 
 void lu_destroy(LUHandle* h) {
@@ -96,6 +106,7 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
                 "lu_create: slices must be in [3, 7]");
   LUHandle* h = new LUHandle();
   h->N = N;
+  h->cap = N;
   h->precision = precision;
   h->pivot = pivot;
   h->nb = nb;
@@ -123,7 +134,7 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   alloc(&h->v2, (size_t)N * 8);
   if (precision == LU_OZAKI) alloc(&h->oz_work, ozaki_workspace_bytes(N, N, nb, slices));
   h->fast_solve = (N % TRSV_BLOCK) == 0;
-  if (h->fast_solve) {
+  if (N >= TRSV_BLOCK) {  // also for a later lu_resize to a multiple of TRSV_BLOCK
     alloc(&h->Lbinv, (size_t)N * TRSV_BLOCK * h->es);
     alloc(&h->Ubinv, (size_t)N * TRSV_BLOCK * h->es);
     alloc(&h->v3, (size_t)N * 8);
@@ -431,6 +442,9 @@ MPDOK_API int mpdok_lu_factor(void* h, hipStream_t s, const double* A, int64_t l
 }
 MPDOK_API int mpdok_lu_solve(void* h, hipStream_t s, const double* b, double* x) {
   return mpdok::lu_solve(static_cast<mpdok::LUHandle*>(h), s, b, x);
+}
+MPDOK_API int mpdok_lu_resize(void* h, int n) {
+  return mpdok::lu_resize(static_cast<mpdok::LUHandle*>(h), n);
 }
 MPDOK_API int mpdok_lu_destroy(void* h) {
   mpdok::lu_destroy(static_cast<mpdok::LUHandle*>(h));

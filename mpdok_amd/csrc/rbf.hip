@@ -154,25 +154,84 @@ int rbf_mean_d2(hipStream_t st, const double* coords, int n, int D, double* out)
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-free product y = K[row0 : row0 + rows, :] x with every kernel value
-// recomputed from the coordinates and never stored (mpdok_ooc.py:205-227, the
-// FP64 outer residual; and the matrix-free form of the FP32 inner product,
-// mpdok_ooc.py:39-46,158-201, whose elements are the stored tile's values).
+// Matrix-free products with kernel values recomputed from the coordinates and
+// never stored.
 //
-// ELEM = F64: element = kernel value + reg on the diagonal, in FP64 (a25).
-// ELEM = F32: element = FP32 rounding of the kernel value, reg added in FP32
-//             after the cast: bit for bit the element of the stored FP32 tile.
-// Products and sums are FP64; y gets the sum rounded once to TY.
+// rbf_matvec_kernel: y_i = sum_j E(k(r_i, c_j)) x_j for row points r_i
+// (i in [row0, row0 + rows) of R) and all column points c_j of C.
+//   ELEM = F64: E = k + reg on the diagonal, in FP64 (the FP64 outer residual
+//               of mpdok_ooc.py:205-227).
+//   ELEM = F32: E = FP32 rounding of k, reg added in FP32 after the cast: bit
+//               for bit the element of the stored FP32 tile (mpdok_ooc.py:39-46,
+//               118-123), i.e. the matrix-free form of the FP32 inner product.
+//   The diagonal (and so the nugget) exists only when R and C are the same
+//   point set (same = 1); a cross product K(Q, X) x has none.
+// DERIV = 1 replaces k by its derivative with respect to the kernel parameter
+// p (Gaussian exp(-p d2): -d2 k; Matern-3/2 with a = sqrt(3 d2) / p:
+// a^2 exp(-a) / p); with no nugget. Products and sums are FP64.
 //
 // Block: 32 rows x 8 column phases (256 threads). Coordinates, squared norms
-// and x of a 256-column chunk are staged in LDS; a wavefront's two half-waves
-// read two columns per step (broadcast reads). The 8 partial sums of a row are
-// added in a fixed order, so results do not depend on timing.
+// and x of a 256-column chunk are staged in LDS; each half-wavefront reads one
+// column per step (broadcast reads). The 8 partial sums of a row are added in
+// a fixed order, so results do not depend on timing.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double kernel_dparam(int kind, double p, double d2) {
+  if (kind == 0) return -d2 * exp(-p * d2);
+  const double a = sqrt(3.0 * d2) / p;
+  return a * a * exp(-a) / p;
+}
+
+template <int ELEM, int DERIV, typename TX>
+__device__ __forceinline__ double rbf_row_sum(const double* __restrict__ ci, double sqi, int i,
+                                              const double* __restrict__ C, int NC, int D,
+                                              int kind, double p, double reg, int same,
+                                              const TX* __restrict__ x, double (*cj)[256],
+                                              double* sqj, double* xj, int t, int ph) {
+  double acc = 0.0;
+  for (int j0 = 0; j0 < NC; j0 += 256) {
+    const int j = j0 + t;
+    double c[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) c[d] = (j < NC && d < D) ? C[(int64_t)j * D + d] : 0.0;
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d)
+      if (d < D) cj[d][t] = c[d];
+    sqj[t] = sqnorm(c, D);
+    xj[t] = j < NC ? (double)x[j] : 0.0;
+    __syncthreads();
+    const int jn = NC - j0 < 256 ? NC - j0 : 256;
+    for (int k = ph; k < jn; k += 8) {
+      double cc[MAXD];
+#pragma unroll
+      for (int d = 0; d < MAXD; ++d) cc[d] = d < D ? cj[d][k] : 0.0;
+      const double d2 = gram_d2(ci, sqi, cc, sqj[k], D);
+      double e;
+      if (DERIV) {
+        e = kernel_dparam(kind, p, d2);
+      } else {
+        const double v = kernel_value(kind, p, d2);
+        const bool diag = same && i == j0 + k;
+        if (ELEM == F64) {
+          e = diag ? v + reg : v;
+        } else {
+          float f = (float)v;
+          if (diag) f = f + (float)reg;
+          e = (double)f;
+        }
+      }
+      acc = __builtin_fma(e, xj[k], acc);
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
+// y[i - row0] = sum_j E_ij x_j  (y in TY).
 template <int ELEM, typename TX, typename TY>
-__global__ __launch_bounds__(256) void rbf_matvec_kernel(const double* __restrict__ coords, int N,
+__global__ __launch_bounds__(256) void rbf_matvec_kernel(const double* __restrict__ R,
+                                                         const double* __restrict__ C, int NC,
                                                          int D, int kind, double p, double reg,
-                                                         int row0, int rows,
+                                                         int same, int row0, int rows,
                                                          const TX* __restrict__ x,
                                                          TY* __restrict__ y) {
   __shared__ double cj[MAXD][256];
@@ -185,39 +244,9 @@ __global__ __launch_bounds__(256) void rbf_matvec_kernel(const double* __restric
   const bool live = il < rows;
   double ci[MAXD];
 #pragma unroll
-  for (int d = 0; d < MAXD; ++d) ci[d] = (live && d < D) ? coords[(int64_t)i * D + d] : 0.0;
-  const double sqi = sqnorm(ci, D);
-  double acc = 0.0;
-  for (int j0 = 0; j0 < N; j0 += 256) {
-    const int j = j0 + t;
-    double c[MAXD];
-#pragma unroll
-    for (int d = 0; d < MAXD; ++d) c[d] = (j < N && d < D) ? coords[(int64_t)j * D + d] : 0.0;
-#pragma unroll
-    for (int d = 0; d < MAXD; ++d)
-      if (d < D) cj[d][t] = c[d];
-    sqj[t] = sqnorm(c, D);
-    xj[t] = j < N ? (double)x[j] : 0.0;
-    __syncthreads();
-    const int jn = N - j0 < 256 ? N - j0 : 256;
-    for (int k = ph; k < jn; k += 8) {
-      double cc[MAXD];
-#pragma unroll
-      for (int d = 0; d < MAXD; ++d) cc[d] = d < D ? cj[d][k] : 0.0;
-      const double v = kernel_value(kind, p, gram_d2(ci, sqi, cc, sqj[k], D));
-      const bool diag = i == j0 + k;
-      double e;
-      if (ELEM == F64) {
-        e = diag ? v + reg : v;
-      } else {
-        float f = (float)v;
-        if (diag) f = f + (float)reg;
-        e = (double)f;
-      }
-      acc = __builtin_fma(e, xj[k], acc);
-    }
-    __syncthreads();
-  }
+  for (int d = 0; d < MAXD; ++d) ci[d] = (live && d < D) ? R[(int64_t)i * D + d] : 0.0;
+  const double acc = rbf_row_sum<ELEM, 0, TX>(ci, sqnorm(ci, D), i, C, NC, D, kind, p, reg, same,
+                                             x, cj, sqj, xj, t, ph);
   part[ph][r] = acc;
   __syncthreads();
   if (ph == 0 && live) {
@@ -227,19 +256,75 @@ __global__ __launch_bounds__(256) void rbf_matvec_kernel(const double* __restric
   }
 }
 
-int rbf_matvec(hipStream_t st, const double* coords, int N, int D, int kind, double p, double reg,
-               int elem_dtype, const void* x, void* y, int xy_dtype, int row0, int rows) {
-  MPDOK_REQUIRE(coords && x && y, "rbf_matvec: null pointer");
-  MPDOK_REQUIRE(D >= 1 && D <= MAXD, "rbf_matvec: D must be in [1, %d], got %d", MAXD, D);
-  MPDOK_REQUIRE(kind == 0 || kind == 1, "rbf_matvec: unknown kernel kind %d", kind);
-  MPDOK_REQUIRE(!(kind == 1 && !(p > 0.0)), "rbf_matvec: Matern length scale must be > 0");
-  MPDOK_REQUIRE(row0 >= 0 && rows >= 0 && row0 + rows <= N, "rbf_matvec: rows outside N");
+// Bilinear form u^T (dK/dp) w over rows [0, NR) of R and all points of C:
+// per-block partial sums into part_out[blockIdx.x], reduced in a fixed order
+// by bilinear_reduce_kernel.
+__global__ __launch_bounds__(256) void rbf_dbilinear_kernel(const double* __restrict__ R, int NR,
+                                                            const double* __restrict__ C, int NC,
+                                                            int D, int kind, double p,
+                                                            const double* __restrict__ u,
+                                                            const double* __restrict__ w,
+                                                            double* __restrict__ part_out) {
+  __shared__ double cj[MAXD][256];
+  __shared__ double sqj[256];
+  __shared__ double xj[256];
+  __shared__ double part[8][33];
+  const int t = threadIdx.x, r = t & 31, ph = t >> 5;
+  const int i = blockIdx.x * 32 + r;
+  const bool live = i < NR;
+  double ci[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) ci[d] = (live && d < D) ? R[(int64_t)i * D + d] : 0.0;
+  const double acc = rbf_row_sum<F64, 1, double>(ci, sqnorm(ci, D), i, C, NC, D, kind, p, 0.0, 0,
+                                                 w, cj, sqj, xj, t, ph);
+  part[ph][r] = acc;
+  __syncthreads();
+  if (ph == 0) {
+    double s = ((part[0][r] + part[1][r]) + (part[2][r] + part[3][r])) +
+               ((part[4][r] + part[5][r]) + (part[6][r] + part[7][r]));
+    s = live ? s * u[i] : 0.0;
+    // the 32 rows of this block, in a fixed order
+    for (int off = 16; off > 0; off >>= 1) s += __shfl_down(s, off, 32);
+    if (r == 0) part_out[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void bilinear_reduce_kernel(const double* __restrict__ part,
+                                                              int n, double* __restrict__ out) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) acc += part[k];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+static int rbf_check(const double* R, const double* C, int D, int kind, double p) {
+  MPDOK_REQUIRE(R && C, "rbf: null coordinates");
+  MPDOK_REQUIRE(D >= 1 && D <= MAXD, "rbf: D must be in [1, %d], got %d", MAXD, D);
+  MPDOK_REQUIRE(kind == 0 || kind == 1, "rbf: unknown kernel kind %d", kind);
+  MPDOK_REQUIRE(!(kind == 1 && !(p > 0.0)), "rbf: Matern length scale must be > 0");
+  return OK;
+}
+
+int rbf_cross_matvec(hipStream_t st, const double* R, int NR, const double* C, int NC, int D,
+                     int kind, double p, double reg, int same, int elem_dtype, const void* x,
+                     void* y, int xy_dtype, int row0, int rows) {
+  MPDOK_TRY(rbf_check(R, C, D, kind, p));
+  MPDOK_REQUIRE(x && y, "rbf_matvec: null vector");
+  MPDOK_REQUIRE(NR >= 0 && NC >= 0 && row0 >= 0 && rows >= 0 && row0 + rows <= NR,
+                "rbf_matvec: rows outside the row set");
+  MPDOK_REQUIRE(!same || NR == NC, "rbf_matvec: same point set needs NR == NC");
   MPDOK_REQUIRE(elem_dtype == F64 || elem_dtype == F32, "rbf_matvec: element dtype %d", elem_dtype);
   MPDOK_REQUIRE(xy_dtype == F64 || xy_dtype == F32, "rbf_matvec: vector dtype %d", xy_dtype);
   if (rows == 0) return OK;
   const int grid = (rows + 31) / 32;
-#define LAUNCH(E, T)                                                                   \
-  rbf_matvec_kernel<E, T, T><<<grid, 256, 0, st>>>(coords, N, D, kind, p, reg, row0, rows, \
+#define LAUNCH(E, T)                                                                          \
+  rbf_matvec_kernel<E, T, T><<<grid, 256, 0, st>>>(R, C, NC, D, kind, p, reg, same, row0, rows, \
                                                    (const T*)x, (T*)y)
   if (elem_dtype == F64) {
     if (xy_dtype == F64) LAUNCH(F64, double); else LAUNCH(F64, float);
@@ -251,12 +336,49 @@ int rbf_matvec(hipStream_t st, const double* coords, int N, int D, int kind, dou
   return OK;
 }
 
+int rbf_matvec(hipStream_t st, const double* coords, int N, int D, int kind, double p, double reg,
+               int elem_dtype, const void* x, void* y, int xy_dtype, int row0, int rows) {
+  return rbf_cross_matvec(st, coords, N, coords, N, D, kind, p, reg, 1, elem_dtype, x, y,
+                          xy_dtype, row0, rows);
+}
+
+size_t rbf_dbilinear_workspace_bytes(int NR) { return (size_t)((NR + 31) / 32 + 1) * 8; }
+
+int rbf_dbilinear(hipStream_t st, const double* R, int NR, const double* C, int NC, int D,
+                  int kind, double p, const double* u, const double* w, double* out, void* work,
+                  size_t work_bytes) {
+  MPDOK_TRY(rbf_check(R, C, D, kind, p));
+  MPDOK_REQUIRE(u && w && out && work, "rbf_dbilinear: null pointer");
+  MPDOK_REQUIRE(NR >= 1 && NC >= 1, "rbf_dbilinear: empty point set");
+  MPDOK_REQUIRE(work_bytes >= rbf_dbilinear_workspace_bytes(NR), "rbf_dbilinear: workspace");
+  const int nb = (NR + 31) / 32;
+  double* part = static_cast<double*>(work);
+  rbf_dbilinear_kernel<<<nb, 256, 0, st>>>(R, NR, C, NC, D, kind, p, u, w, part);
+  bilinear_reduce_kernel<<<1, 256, 0, st>>>(part, nb, out);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
 }  // namespace mpdok
 
 MPDOK_API int mpdok_rbf_matvec(hipStream_t st, const double* coords, int N, int D, int kind,
                                double p, double reg, int elem_dtype, const void* x, void* y,
                                int xy_dtype, int row0, int rows) {
   return mpdok::rbf_matvec(st, coords, N, D, kind, p, reg, elem_dtype, x, y, xy_dtype, row0, rows);
+}
+MPDOK_API int mpdok_rbf_cross_matvec(hipStream_t st, const double* R, int NR, const double* C,
+                                     int NC, int D, int kind, double p, const double* x,
+                                     double* y) {
+  return mpdok::rbf_cross_matvec(st, R, NR, C, NC, D, kind, p, 0.0, 0, mpdok::F64, x, y,
+                                 mpdok::F64, 0, NR);
+}
+MPDOK_API size_t mpdok_rbf_dbilinear_workspace_bytes(int NR) {
+  return mpdok::rbf_dbilinear_workspace_bytes(NR);
+}
+MPDOK_API int mpdok_rbf_dbilinear(hipStream_t st, const double* R, int NR, const double* C, int NC,
+                                  int D, int kind, double p, const double* u, const double* w,
+                                  double* out, void* work, size_t work_bytes) {
+  return mpdok::rbf_dbilinear(st, R, NR, C, NC, D, kind, p, u, w, out, work, work_bytes);
 }
 
 MPDOK_API int mpdok_rbf_assemble(hipStream_t st, const double* coords, int N, int D, int kind,

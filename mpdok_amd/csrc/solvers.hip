@@ -13,12 +13,16 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 #include "common.h"
 #include "gemv.h"
 #include "krylov.h"
 #include "lu.h"
+#include "rbf.h"
 
 namespace mpdok {
 
@@ -429,4 +433,100 @@ MPDOK_API void py_lu_free_factored(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   lu_destroy(g_lu);
   g_lu = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Batched uneven kernel systems (north star's batched configuration; no
+// reference counterpart). System i: N[i] points coords[i] (N x D, row-major),
+// kernel (kind, param[i]) + reg I, nrhs right-hand sides B[i] (N x nrhs,
+// column-major) -> X[i]. Each system: assemble K in FP64, LU (precision: FP64
+// or int8-sliced; no pivoting, K is SPD), then LU-IR with the Dot2 residual per
+// right-hand side until relres <= tol.
+//
+// Scheduling: systems go to nstreams workers, largest N first, each to the
+// worker with the least N^3 so far (longest-processing-time rule). A worker is
+// a host thread with its own stream, one FP64 matrix buffer and one LU handle
+// sized for its largest system (lu_resize for the others), so the
+// host-synchronous convergence tests of one system never stall another
+// worker's GPU work and nothing is allocated between systems.
+// ---------------------------------------------------------------------------
+MPDOK_API int mpdok_batched_rbf_solve(int nsys, const int* N, const double* const* coords, int D,
+                                      int kind, const double* param, double reg, int nrhs,
+                                      const double* const* B, double* const* X, double tol,
+                                      int maxiter_outer, int precision, int slices, int nb,
+                                      int nstreams, int* converged, double* relres) {
+  MPDOK_REQUIRE(nsys >= 0 && nrhs >= 1 && D >= 1, "batched: bad sizes");
+  if (nsys == 0) return OK;
+  MPDOK_REQUIRE(N && coords && param && B && X && converged && relres, "batched: null pointer");
+  MPDOK_REQUIRE(nstreams >= 1 && nstreams <= 64, "batched: nstreams must be in [1, 64]");
+  MPDOK_REQUIRE(precision == LU_FP64 || precision == LU_OZAKI, "batched: precision %d", precision);
+  for (int i = 0; i < nsys; ++i) MPDOK_REQUIRE(N[i] >= 1, "batched: N[%d] = %d", i, N[i]);
+  int dev = 0;
+  MPDOK_HIP(hipGetDevice(&dev));
+  std::vector<int> order(nsys);
+  for (int i = 0; i < nsys; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return N[a] > N[b]; });
+  const int nw = std::min(nstreams, nsys);
+  std::vector<std::vector<int>> work(nw);
+  std::vector<double> load(nw, 0.0);
+  for (int i : order) {
+    const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    work[w].push_back(i);
+    load[w] += (double)N[i] * N[i] * N[i];
+  }
+  std::vector<int> status(nw, OK);
+  std::vector<std::string> msg(nw);
+  auto worker = [&](int w) {
+    int st = OK;
+    hipStream_t s = nullptr;
+    double* A = nullptr;
+    LUHandle* h = nullptr;
+    const int cap = N[work[w][0]];  // largest first
+    auto run = [&]() -> int {
+      MPDOK_HIP(hipSetDevice(dev));
+      MPDOK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      MPDOK_HIP(hipMalloc(&A, (size_t)cap * cap * sizeof(double)));
+      MPDOK_TRY(lu_create(cap, precision, PIV_NONE, nb, slices, &h));
+      for (int i : work[w]) {
+        const int n = N[i];
+        MPDOK_TRY(rbf_assemble(s, coords[i], n, D, kind, param[i], reg, 0, 0, n, 0, n, A, F64, n,
+                               COL_MAJOR));
+        MPDOK_TRY(lu_resize(h, n));
+        int info = 0;
+        MPDOK_TRY(lu_factor(h, s, A, n, COL_MAJOR, &info));
+        for (int r = 0; r < nrhs; ++r) {
+          const int k = i * nrhs + r;
+          if (info != 0) {
+            converged[k] = 0;
+            relres[k] = NAN;
+            continue;
+          }
+          IRResult res;
+          MPDOK_TRY(lu_ir_impl(s, A, n, COL_MAJOR, h, n, B[i] + (size_t)r * n,
+                               X[i] + (size_t)r * n, tol, maxiter_outer, 1, res));
+          converged[k] = res.converged;
+          relres[k] = res.hist.empty() ? NAN : res.hist.back();
+        }
+      }
+      MPDOK_HIP(hipStreamSynchronize(s));
+      return OK;
+    };
+    st = run();
+    if (st != OK) msg[w] = error_string();
+    if (s) (void)hipStreamSynchronize(s);
+    lu_destroy(h);
+    if (A) (void)hipFree(A);
+    if (s) (void)hipStreamDestroy(s);
+    status[w] = st;
+  };
+  std::vector<std::thread> threads;
+  for (int w = 0; w < nw; ++w) threads.emplace_back(worker, w);
+  for (auto& t : threads) t.join();
+  for (int w = 0; w < nw; ++w) {
+    if (status[w] != OK) {
+      set_error("batched (worker %d): %s", w, msg[w].c_str());
+      return status[w];
+    }
+  }
+  return OK;
 }

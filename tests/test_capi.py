@@ -55,3 +55,17 @@ def test_product_never_imports_oracle():
             if f.endswith(".py"):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in re.findall(r"^\s*(?:from|import)\s+(\w+)", src, flags=re.M), f
+
+
+def test_ooc_solver_host_validation():
+    """MPDOKOOCSolver argument checks happen before any device work
+    (mpdok_ooc.py:130-146, :342-343): bad store, missing path, solve first."""
+    from mpdok_amd.mpdok_ooc import MPDOKOOCSolver
+    s = MPDOKOOCSolver()
+    assert s.tile_rows == 4096 and s.N is None and s._store is None
+    with pytest.raises(ValueError, match="store must be"):
+        s.build([[0.0, 0.0]], gamma=1.0, store="tape", verbose=False)
+    with pytest.raises(ValueError, match="requires a path"):
+        s.build([[0.0, 0.0]], gamma=1.0, store="ssd", path=None, verbose=False)
+    with pytest.raises(RuntimeError, match="build"):
+        s.solve([1.0])

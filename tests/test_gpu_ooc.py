@@ -1,0 +1,151 @@
+"""Out-of-core solver (mpdok_ooc.py) on the GPU against the CPU oracle:
+tile build (RAM and on-disk stores), streamed and matrix-free products, the
+DGKS inner GMRES and the outer refinement."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import krylov as okr
+from oracle import rbf as orb
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(tile_rows):
+    from mpdok_amd.mpdok_ops import _dev  # noqa: F401  (loads the library first)
+    from mpdok_amd.mpdok_ooc import MPDOKOOCSolver
+    return MPDOKOOCSolver(tile_rows=tile_rows)
+
+
+def _pts(N, D, seed):
+    return np.random.default_rng(seed).uniform(0.0, 1.0, (N, D))
+
+
+@pytest.mark.parametrize("N,tile_rows", [(1000, 256), (777, 1024), (513, 100)])
+def test_ram_build_matches_oracle_tiles(N, tile_rows):
+    """FP32 tiles (nugget after the cast, mpdok_ooc.py:118-123) within one
+    FP32 ulp of the oracle's; ragged last tile included."""
+    X = _pts(N, 2, N)
+    s = _solver(tile_rows)
+    g = s.build(X, gamma=30.0, reg=1e-3, store="ram", verbose=False)
+    assert g == 30.0 and s.N == N and s._ram_buf.shape == (N, N)
+    assert s._ram_buf.dtype == np.float32 and s._ram_buf.flags["C_CONTIGUOUS"]
+    ref = orb.build_fp32_tile(X, 0, N, "gaussian", 30.0, 1e-3)
+    ulp = np.spacing(np.abs(ref))
+    assert np.all(np.abs(s._ram_buf - ref) <= ulp)
+    s.free()
+
+
+def test_ssd_store_is_raw_row_major_fp32(tmp_path):
+    """The on-disk store is raw FP32, row-major, no header (mpdok_ooc.py:142),
+    byte-identical to the RAM store."""
+    N = 600
+    X = _pts(N, 3, 7)
+    ram = _solver(128)
+    ram.build(X, gamma=20.0, reg=1e-4, store="ram", verbose=False)
+    path = str(tmp_path / "A_fp32.bin")
+    ssd = _solver(128)
+    ssd.build(X, gamma=20.0, reg=1e-4, store="ssd", path=path, verbose=False)
+    assert os.path.getsize(path) == N * N * 4
+    disk = np.fromfile(path, dtype=np.float32).reshape(N, N)
+    assert np.array_equal(disk, ram._ram_buf)
+    v = np.random.default_rng(1).standard_normal(N).astype(np.float32)
+    y_ram = ram._tiled_sgemv(torch.as_tensor(v).cuda()).cpu().numpy()
+    y_ssd = ssd._tiled_sgemv(torch.as_tensor(v).cuda()).cpu().numpy()
+    assert np.array_equal(y_ram, y_ssd)       # same tiles, same kernel, same order
+    ref = ram._ram_buf.astype(np.float64) @ v.astype(np.float64)
+    assert np.allclose(y_ram, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+    ram.free()
+    ssd.free()
+
+
+def test_bring_your_own_matrix_recipe():
+    """The README recipe (RBF_README.md:229-248): attributes set by hand, a
+    plain (pageable) NumPy matrix, FP64 residual from the coordinates."""
+    N = 700
+    X = _pts(N, 2, 3)
+    A32 = orb.build_fp32_tile(X, 0, N, "gaussian", 15.0, 1e-2)
+    s = _solver(200)
+    s.N, s.gamma, s.reg = N, 15.0, 1e-2
+    s._store = "ram"
+    s._ram_buf = np.ascontiguousarray(A32)
+    s._coords = torch.as_tensor(X).cuda()
+    s._sq = (s._coords ** 2).sum(dim=1)
+    v = np.random.default_rng(2).standard_normal(N).astype(np.float32)
+    y = s._tiled_sgemv(torch.as_tensor(v).cuda()).cpu().numpy()
+    ref = A32.astype(np.float64) @ v.astype(np.float64)
+    assert np.allclose(y, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+    b = np.sin(3 * X[:, 0]) + X[:, 1]
+    x = s.solve(b, tol=1e-9, maxiter_outer=20, restart=50, verbose=False).cpu().numpy()
+    A64 = orb.build_kernel(X, "gaussian", 15.0, 1e-2)
+    assert np.linalg.norm(b - A64 @ x) / np.linalg.norm(b) < 1e-9
+
+
+@pytest.mark.parametrize("kind,param", [("gaussian", 25.0), ("matern32", 0.1)])
+def test_matrix_free_products_match_oracle(kind, param):
+    """FP64 product (mpdok_ooc.py:205-227) and the FP32-element product agree
+    with the oracle within summation-order bounds."""
+    N = 900
+    X = _pts(N, 3, 11)
+    s = _solver(256)
+    if kind == "gaussian":
+        s.build(X, gamma=param, reg=1e-4, store="free", verbose=False)
+    else:
+        s.build(X, reg=1e-4, store="free", verbose=False, kernel="matern32", length_scale=param)
+    v = np.random.default_rng(4).standard_normal(N)
+    y64 = s._tiled_dgemv_fp64(torch.as_tensor(v).cuda()).cpu().numpy()
+    ref64 = orb.matvec_fp64(X, v, kind, param, 1e-4)
+    absA = np.abs(orb.build_kernel(X, kind, param, 1e-4)) @ np.abs(v)
+    # element error <= 4 u gamma 2 D (Gram identity) + 1 ulp (exp); sum error N u
+    atol = (4 * 2.0 ** -53 * (param if kind == "gaussian" else 1e3) * 6 + (N + 2) * 2.0 ** -53)
+    assert np.all(np.abs(y64 - ref64) <= atol * absA + 1e-300)
+    y32 = s._tiled_sgemv(torch.as_tensor(v.astype(np.float32)).cuda()).cpu().numpy()
+    T = orb.build_fp32_tile(X, 0, N, kind, param, 1e-4).astype(np.float64)
+    ref32 = T @ v.astype(np.float32).astype(np.float64)
+    assert np.all(np.abs(y32 - ref32) <= 2.0 ** -23 * np.abs(T) @ np.abs(v) * 4 + 1e-30)
+    s.free()
+
+
+def test_inner_gmres_matches_oracle_and_solve_converges():
+    # unpreconditioned FP32 inner GMRES(50), as the reference's: a system it
+    # can refine to 1e-10 (kappa ~ 1e3; the reference's default tol is 1e-5)
+    N = 1500
+    X = _pts(N, 2, 21)
+    gamma, reg = 40.0, 5e-2
+    s = _solver(512)
+    s.build(X, gamma=gamma, reg=reg, store="ram", verbose=False)
+    A32 = s._ram_buf.copy()
+    A64 = orb.build_kernel(X, "gaussian", gamma, reg)
+    b = np.cos(4 * X[:, 0]) * X[:, 1] + 0.1
+    # one inner solve from the exact FP64 residual of x = 0
+    e = s._inner_gmres(torch.as_tensor(b).cuda(), 50).cpu().numpy()
+    e_ref = okr.inner_gmres(lambda v: A32 @ v, b, 50)
+    assert np.linalg.norm(e - e_ref) <= 1e-3 * np.linalg.norm(e_ref)
+    calls = []
+    orig = s._tiled_dgemv_fp64
+
+    def counted(v):
+        calls.append(1)
+        return orig(v)
+    s._tiled_dgemv_fp64 = counted          # instance patch, called once per outer step
+    x = s.solve(b, tol=1e-10, maxiter_outer=20, restart=50, verbose=False).cpu().numpy()
+    rel = np.linalg.norm(b - A64 @ x) / np.linalg.norm(b)
+    assert rel < 1e-10
+    x_ref = np.linalg.solve(A64, b)
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-6
+    assert 2 <= len(calls) <= 20
+    s.free()
+
+
+def test_zero_rhs_and_not_converged_warning():
+    N = 300
+    X = _pts(N, 2, 5)
+    s = _solver(128)
+    s.build(X, gamma=10.0, reg=1e-2, store="ram", verbose=False)
+    x = s.solve(np.zeros(N), verbose=False)
+    assert torch.count_nonzero(x).item() == 0 and x.dtype == torch.float64
+    with pytest.warns(RuntimeWarning, match="did not converge"):
+        s.solve(np.ones(N), tol=1e-30, maxiter_outer=2, restart=5, verbose=False)
+    s.free()
