@@ -8,7 +8,8 @@
 //    BF16/FP16 matrix-core product, used by the precision ladder.
 //  * diag_factor_kernel: LU without pivoting of one nb x nb (nb <= 64)
 //    diagonal block in LDS, plus the explicit inverses of its unit-lower and
-//    upper triangles (used by the panel and by the blocked triangular solves).
+//    upper triangles (used by the panel and by the blocked triangular solves),
+//    all in one 64 x 64 array.
 //  * panel kernels for partial pivoting: pivot search with a last-block
 //    reduction, row swap, column scaling and rank-1 update of the sub-panel.
 //  * blocked triangular solves (forward unit-lower, backward upper) with 64-row
@@ -196,16 +197,24 @@ template int gemm_batched<float>(hipStream_t, int, int, int, float, const float*
                                  int, int);
 
 // ---------------------------------------------------------------------------
-// Diagonal block: LU without pivoting, then inv(unit L) and inv(U), all in LDS.
+// Diagonal block: LU without pivoting, then inv(unit L) and inv(U), in place
+// in one 64 x 64 LDS array (34 KB with the two staging columns). The small
+// footprint matters: this kernel runs on the lookahead stream next to the
+// trailing update, and it can take a CU as soon as one workgroup of the
+// update leaves it instead of waiting for the whole CU's LDS.
 //
-// Thread t owns row r = t % 64 and the columns c = t / 64 (mod 4). Elimination
-// step j: every thread forms its row's multiplier l = a[r][j] / a[j][j] itself
-// and updates its columns c > j; the multipliers are kept apart in lo[][], so
-// that column j is never rewritten while it is read and one barrier per step
-// suffices. The inverses then run side by side, threads 0..127 on inv(L)
-// (forward), 128..255 on inv(U) (backward), by substitution on the identity.
-// The loops run on LDS with small code: a single workgroup executes this once
-// per 64 columns, so its instructions come cold from memory each time.
+// Elimination: thread t owns row r = t % 64 and the columns c = t / 64
+// (mod 4). At step j every thread of row r forms the multiplier
+// l = a[r][j] / a[j][j] itself and updates its columns c > j; the multiplier
+// is stored into a[r][j] one step later, when no thread reads column j any
+// more, so one barrier per step suffices. The arithmetic (one division per
+// multiplier, one fused multiply-add per update) is the textbook order.
+//
+// Inversion (LAPACK's unblocked in-place algorithm): U column by column,
+// ascending, u_jj <- 1 / u_jj, u_ij <- -u_jj * sum_{k=i}^{j-1} inv(U)_ik u_kj;
+// unit-lower L column by column, descending, l_ij <- -(l_ij + sum_{k=j+1}^{i-1}
+// inv(L)_ik l_kj). Threads 0..63 do U, 64..127 L, side by side; each column's
+// old values are staged first, so the in-place update reads no value it writes.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int64_t lda, int nb,
@@ -213,65 +222,57 @@ __global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int
                                                           T* __restrict__ Uinv, int ldi,
                                                           int* __restrict__ info, int col0,
                                                           int factor) {
-  __shared__ T s[64][65];   // U (upper triangle) during and after the elimination
-  __shared__ T lo[64][65];  // strict lower triangle: multipliers
-  __shared__ T x[2][64][65];  // inv(L), inv(U)
+  __shared__ T s[64][65];
+  __shared__ T tcol[2][65];  // staged old column: [0] U (rows <= j), [1] L (rows > j)
   const int t = threadIdx.x, r = t & 63, q = t >> 6;
-  for (int c = q; c < nb; c += 4) {
-    if (r < nb) {
-      const T v = A[r + (int64_t)c * lda];
-      s[r][c] = v;
-      lo[r][c] = v;
-    }
-  }
+  for (int c = q; c < nb; c += 4)
+    if (r < nb) s[r][c] = A[r + (int64_t)c * lda];
   __syncthreads();
   if (factor) {
+    T lprev = T(0);
     for (int j = 0; j < nb; ++j) {
+      // multiplier of step j - 1 into column j - 1: every read of it is done
+      if (j > 0 && q == 0 && r > j - 1 && r < nb) s[r][j - 1] = lprev;
       const T piv = s[j][j];
       if (t == 0 && piv == T(0)) atomicCAS(info, 0, col0 + j + 1);
+      T l = T(0);
       if (r > j && r < nb) {
-        const T l = s[r][j] / piv;
-        if (q == 0) lo[r][j] = l;
+        l = s[r][j] / piv;
 #pragma unroll 4
         for (int c = j + 1 + q; c < nb; c += 4) s[r][c] = __builtin_fma(-l, s[j][c], s[r][c]);
       }
+      lprev = l;
       __syncthreads();
     }
     for (int c = q; c < nb; c += 4)
-      if (r < nb) A[r + (int64_t)c * lda] = r > c ? lo[r][c] : s[r][c];
-  }
-  // identity start for both inverses
-  for (int c = q; c < 64; c += 4) {
-    x[0][r][c] = r == c ? T(1) : T(0);
-    x[1][r][c] = r == c ? T(1) : T(0);
-  }
-  __syncthreads();
-  // inv(L) (threads with h == 0), ascending k:  X[i][c] -= L[i][k] X[k][c],  i > k, c <= k
-  // inv(U) (h == 1), descending k:  X[k][c] /= U[k][k] (c >= k), then
-  //                                 X[i][c] -= U[i][k] X[k][c],  i < k, c >= k
-  const int h = q >> 1, p = q & 1;
-  for (int st = 0; st < nb; ++st) {
-    const int kU = nb - 1 - st;
-    if (h == 1 && p == 0 && r >= kU && r < nb) x[1][kU][r] = x[1][kU][r] / s[kU][kU];
+      if (r < nb) A[r + (int64_t)c * lda] = s[r][c];
     __syncthreads();
-    if (h == 0) {
-      const int k = st;
-      if (r > k && r < nb) {
-        const T l = lo[r][k];
-#pragma unroll 4
-        for (int c = p; c <= k; c += 2) x[0][r][c] = __builtin_fma(-l, x[0][k][c], x[0][r][c]);
+  }
+  for (int st = 0; st < nb; ++st) {
+    const int ju = st, jl = nb - 1 - st;
+    if (q == 0 && r <= ju) tcol[0][r] = s[r][ju];
+    if (q == 1 && r > jl && r < nb) tcol[1][r] = s[r][jl];
+    __syncthreads();
+    if (q == 0 && r <= ju) {
+      const T d = T(1) / tcol[0][ju];
+      if (r == ju) {
+        s[ju][ju] = d;
+      } else {
+        T acc = T(0);
+        for (int k = r; k < ju; ++k) acc = __builtin_fma(s[r][k], tcol[0][k], acc);
+        s[r][ju] = -d * acc;
       }
-    } else if (r < kU) {
-      const T u = s[r][kU];
-#pragma unroll 4
-      for (int c = kU + p; c < nb; c += 2) x[1][r][c] = __builtin_fma(-u, x[1][kU][c], x[1][r][c]);
+    } else if (q == 1 && r > jl && r < nb) {
+      T acc = tcol[1][r];
+      for (int k = jl + 1; k < r; ++k) acc = __builtin_fma(s[r][k], tcol[1][k], acc);
+      s[r][jl] = -acc;
     }
     __syncthreads();
   }
   for (int c = q; c < nb; c += 4) {
     if (r < nb) {
-      Linv[r + c * ldi] = x[0][r][c];
-      Uinv[r + c * ldi] = x[1][r][c];
+      Uinv[r + c * ldi] = r <= c ? s[r][c] : T(0);
+      Linv[r + c * ldi] = r > c ? s[r][c] : (r == c ? T(1) : T(0));
     }
   }
 }
