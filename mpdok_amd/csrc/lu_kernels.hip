@@ -2,7 +2,7 @@
 //
 //  * gemm_kernel<T>: C = alpha A B + beta C, column-major, on the matrix cores
 //    (v_mfma_f64_16x16x4f64 for FP64, v_mfma_f32_16x16x4f32 for FP32). 128x128
-//    block tile, 2x2 wavefronts of 64x64, BK = 16, LDS double buffer with
+//    (or 64x64) block tile, 2x2 wavefronts, BK = 16, LDS double buffer with
 //    register prefetch of the next tile. ROUND = 1/2 rounds both operands to
 //    BF16/FP16 on their way into LDS (FP32 accumulation): the precision of a
 //    BF16/FP16 matrix-core product, used by the precision ladder.
@@ -56,68 +56,77 @@ __device__ __forceinline__ double round_low(double v) {
   return v;
 }
 
-constexpr int GBM = 128, GBN = 128, GBK = 16, GPAD = 16;
+constexpr int GBK = 16, GPAD = 16;
 
 // Lane layout of the 16x16x4 MFMAs (one value per lane for A and B, four for
 // the 16x16 result): A[i = l%16][k = l/16], B[k = l/16][j = l%16],
 // D[i = Mfma<T>::row(l, r)][j = l%16], r = 0..3.
-template <typename T, int ROUND>
+//
+// TB x TB block tile (TB = 128 or 64), 2 x 2 wavefronts of (TB/2) x (TB/2).
+// TB = 64 is for the skinny products of the panel and the block-row solve: it
+// makes twice the workgroups per row of tiles and needs 38 KB of LDS (FP64)
+// instead of 72 KB, so it fits next to a trailing update's workgroup on a CU.
+template <typename T, int ROUND, int TB>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alpha,
                                                    const T* __restrict__ A, int64_t lda,
                                                    const T* __restrict__ B, int64_t ldb, T beta,
                                                    T* __restrict__ C, int64_t ldc, int64_t sA_,
                                                    int64_t sB_, int64_t sC_) {
-  __shared__ T sA[2][GBK][GBM + GPAD];
+  constexpr int KPH = 256 / TB;      // k phases of the A loads
+  constexpr int NA = GBK / KPH;      // A elements per thread
+  constexpr int NBL = TB / 16;       // B elements per thread
+  constexpr int QT = TB / 32;        // 16 x 16 MFMA tiles per wavefront side
+  __shared__ T sA[2][GBK][TB + GPAD];
   // B tile stored k-contiguous per column (+1 padding): the 16 lanes that
   // store 16 consecutive k of one column, and the 16 lanes that read one k of
   // 16 consecutive columns, both fall into distinct banks.
-  __shared__ T sB[2][GBN][GBK + 1];
+  __shared__ T sB[2][TB][GBK + 1];
   typedef typename Mfma<T>::acc_t acc_t;
   // batched use: blockIdx.y selects the problem; blockIdx.x the tile
   A += blockIdx.y * sA_;
   B += blockIdx.y * sB_;
   C += blockIdx.y * sC_;
   int ti, tj;
-  if (!tile_of(blockIdx.x, (m + GBM - 1) / GBM, (n + GBN - 1) / GBN, ti, tj)) return;
+  if (!tile_of(blockIdx.x, (m + TB - 1) / TB, (n + TB - 1) / TB, ti, tj)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-  const int bm = ti * GBM, bn = tj * GBN;
-  acc_t acc[4][4];
+  const int wm = (wave & 1) * (TB / 2), wn = (wave >> 1) * (TB / 2);
+  const int bm = ti * TB, bn = tj * TB;
+  acc_t acc[QT][QT];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < QT; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = acc_t(0);
+    for (int b = 0; b < QT; ++b) acc[a][b] = acc_t(0);
 
-  T ra[8], rb[8];
-  const int ai = bm + (tid & 127), ak = tid >> 7;        // A: row, k phase (0..1)
+  T ra[NA], rb[NBL];
+  const int ai = bm + (tid % TB), ak = tid / TB;         // A: row, k phase
   const int bk = tid & 15, bj = bn + (tid >> 4);         // B: k, column
-  const bool mn_inside = bm + GBM <= m && bn + GBN <= n;
+  const bool mn_inside = bm + TB <= m && bn + TB <= n;
   const T* pa = A + ai + (int64_t)ak * lda;
   const T* pb = B + bk + (int64_t)bj * ldb;
   auto gload = [&](int k0) {
     if (mn_inside && k0 + GBK <= k) {  // interior tile: no bounds checks
 #pragma unroll
-      for (int r = 0; r < 8; ++r) ra[r] = pa[(int64_t)(k0 + 2 * r) * lda];
+      for (int r = 0; r < NA; ++r) ra[r] = pa[(int64_t)(k0 + KPH * r) * lda];
 #pragma unroll
-      for (int r = 0; r < 8; ++r) rb[r] = pb[k0 + (int64_t)(16 * r) * ldb];
+      for (int r = 0; r < NBL; ++r) rb[r] = pb[k0 + (int64_t)(16 * r) * ldb];
       return;
     }
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int kk = k0 + ak + 2 * r;
+    for (int r = 0; r < NA; ++r) {
+      const int kk = k0 + ak + KPH * r;
       ra[r] = (ai < m && kk < k) ? A[ai + (int64_t)kk * lda] : T(0);
     }
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < NBL; ++r) {
       const int kk = k0 + bk, j = bj + 16 * r;
       rb[r] = (kk < k && j < n) ? B[kk + (int64_t)j * ldb] : T(0);
     }
   };
   auto sstore = [&](int buf) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) sA[buf][ak + 2 * r][tid & 127] = round_low<ROUND>(ra[r]);
+    for (int r = 0; r < NA; ++r) sA[buf][ak + KPH * r][tid % TB] = round_low<ROUND>(ra[r]);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) sB[buf][(tid >> 4) + 16 * r][bk] = round_low<ROUND>(rb[r]);
+    for (int r = 0; r < NBL; ++r) sB[buf][(tid >> 4) + 16 * r][bk] = round_low<ROUND>(rb[r]);
   };
 
   const int nk = (k + GBK - 1) / GBK;
@@ -129,24 +138,24 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alp
     if (t + 1 < nk) gload((t + 1) * GBK);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
-      T a[4], b[4];
+      T a[QT], b[QT];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < QT; ++q) {
         a[q] = sA[buf][kk + (lane >> 4)][wm + 16 * q + (lane & 15)];
         b[q] = sB[buf][wn + 16 * q + (lane & 15)][kk + (lane >> 4)];
       }
 #pragma unroll
-      for (int qa = 0; qa < 4; ++qa)
+      for (int qa = 0; qa < QT; ++qa)
 #pragma unroll
-        for (int qb = 0; qb < 4; ++qb) acc[qa][qb] = Mfma<T>::mma(a[qa], b[qb], acc[qa][qb]);
+        for (int qb = 0; qb < QT; ++qb) acc[qa][qb] = Mfma<T>::mma(a[qa], b[qb], acc[qa][qb]);
     }
     if (t + 1 < nk) sstore(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
-  for (int qa = 0; qa < 4; ++qa)
+  for (int qa = 0; qa < QT; ++qa)
 #pragma unroll
-    for (int qb = 0; qb < 4; ++qb) {
+    for (int qb = 0; qb < QT; ++qb) {
       const int j = bn + wn + 16 * qb + (lane & 15);
       if (j >= n) continue;
 #pragma unroll
@@ -162,20 +171,37 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alp
     }
 }
 
+template <typename T, int TB>
+static void gemm_launch(hipStream_t s, dim3 grid, int round, int m, int n, int k, T alpha,
+                        const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB,
+                        T beta, T* C, int64_t ldc, int64_t sC) {
+  if (round == 0)
+    gemm_kernel<T, 0, TB><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
+  else if (round == 1)
+    gemm_kernel<T, 1, TB><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
+  else
+    gemm_kernel<T, 2, TB><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
+}
+
+// Tile choice: 64 x 64 tiles for the skinny products of the panel and of the
+// block-row solve (a side or the inner dimension at most 64), or when
+// 128 x 128 tiles would not give every CU two workgroups; 128 x 128 otherwise.
 template <typename T>
 int gemm_batched(hipStream_t s, int m, int n, int k, T alpha, const T* A, int64_t lda,
                  int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta, T* C, int64_t ldc,
                  int64_t sC, int batch, int round) {
   if (m <= 0 || n <= 0 || batch <= 0) return OK;
-  MPDOK_REQUIRE(batch <= 65535 && (int64_t)((m + GBM - 1) / GBM) * ((n + GBN - 1) / GBN) < (1 << 30),
-                "gemm: grid too large (m=%d n=%d batch=%d)", m, n, batch);
-  dim3 grid(tiles_launch((m + GBM - 1) / GBM, (n + GBN - 1) / GBN), batch);
-  if (round == 0)
-    gemm_kernel<T, 0><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
-  else if (round == 1)
-    gemm_kernel<T, 1><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
+  const int64_t t128 = (int64_t)((m + 127) / 128) * ((n + 127) / 128);
+  const bool small = m <= 64 || n <= 64 || k <= 64 || t128 * batch < 512;
+  const int TB = small ? 64 : 128;
+  const int64_t tiles = (int64_t)((m + TB - 1) / TB) * ((n + TB - 1) / TB);
+  MPDOK_REQUIRE(batch <= 65535 && tiles < (1 << 30), "gemm: grid too large (m=%d n=%d batch=%d)",
+                m, n, batch);
+  dim3 grid(tiles_launch((m + TB - 1) / TB, (n + TB - 1) / TB), batch);
+  if (small)
+    gemm_launch<T, 64>(s, grid, round, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
   else
-    gemm_kernel<T, 2><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
+    gemm_launch<T, 128>(s, grid, round, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
@@ -213,8 +239,9 @@ template int gemm_batched<float>(hipStream_t, int, int, int, float, const float*
 // Inversion (LAPACK's unblocked in-place algorithm): U column by column,
 // ascending, u_jj <- 1 / u_jj, u_ij <- -u_jj * sum_{k=i}^{j-1} inv(U)_ik u_kj;
 // unit-lower L column by column, descending, l_ij <- -(l_ij + sum_{k=j+1}^{i-1}
-// inv(L)_ik l_kj). Threads 0..63 do U, 64..127 L, side by side; each column's
-// old values are staged first, so the in-place update reads no value it writes.
+// inv(L)_ik l_kj). Threads 0..127 do U, 128..255 L, side by side, two lanes
+// per row; each column's old values are staged first, so the in-place update
+// reads no value it writes.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int64_t lda, int nb,
@@ -248,24 +275,47 @@ __global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int
       if (r < nb) A[r + (int64_t)c * lda] = s[r][c];
     __syncthreads();
   }
+  // Inversion: threads 0..127 on U, 128..255 on L; within a group, wavefront
+  // w & 1 holds rows 32 (w & 1) + (lane % 32), and lanes l, l ^ 32 share a row,
+  // each summing every other term (four accumulators each), then combine.
+  const int tt = t & 127, half = (tt >> 5) & 1, row = (tt & 31) + 32 * (tt >> 6);
+  const bool onU = t < 128;
   for (int st = 0; st < nb; ++st) {
     const int ju = st, jl = nb - 1 - st;
-    if (q == 0 && r <= ju) tcol[0][r] = s[r][ju];
-    if (q == 1 && r > jl && r < nb) tcol[1][r] = s[r][jl];
+    if (half == 0) {
+      if (onU && row <= ju) tcol[0][row] = s[row][ju];
+      if (!onU && row > jl && row < nb) tcol[1][row] = s[row][jl];
+    }
     __syncthreads();
-    if (q == 0 && r <= ju) {
-      const T d = T(1) / tcol[0][ju];
-      if (r == ju) {
-        s[ju][ju] = d;
-      } else {
-        T acc = T(0);
-        for (int k = r; k < ju; ++k) acc = __builtin_fma(s[r][k], tcol[0][k], acc);
-        s[r][ju] = -d * acc;
+    if (onU) {
+      if (row <= ju) {  // both lanes of a row take the same branch
+        T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+        int k = row + half;
+        for (; k + 6 < ju; k += 8) {
+          a0 = __builtin_fma(s[row][k], tcol[0][k], a0);
+          a1 = __builtin_fma(s[row][k + 2], tcol[0][k + 2], a1);
+          a2 = __builtin_fma(s[row][k + 4], tcol[0][k + 4], a2);
+          a3 = __builtin_fma(s[row][k + 6], tcol[0][k + 6], a3);
+        }
+        for (; k < ju; k += 2) a0 = __builtin_fma(s[row][k], tcol[0][k], a0);
+        T acc = (a0 + a1) + (a2 + a3);
+        acc += __shfl_xor(acc, 32, 64);
+        const T d = T(1) / tcol[0][ju];
+        if (half == 0) s[row][ju] = row == ju ? d : -d * acc;
       }
-    } else if (q == 1 && r > jl && r < nb) {
-      T acc = tcol[1][r];
-      for (int k = jl + 1; k < r; ++k) acc = __builtin_fma(s[r][k], tcol[1][k], acc);
-      s[r][jl] = -acc;
+    } else if (row > jl && row < nb) {
+      T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+      int k = jl + 1 + half;
+      for (; k + 6 < row; k += 8) {
+        a0 = __builtin_fma(s[row][k], tcol[1][k], a0);
+        a1 = __builtin_fma(s[row][k + 2], tcol[1][k + 2], a1);
+        a2 = __builtin_fma(s[row][k + 4], tcol[1][k + 4], a2);
+        a3 = __builtin_fma(s[row][k + 6], tcol[1][k + 6], a3);
+      }
+      for (; k < row; k += 2) a0 = __builtin_fma(s[row][k], tcol[1][k], a0);
+      T acc = (a0 + a1) + (a2 + a3);
+      acc += __shfl_xor(acc, 32, 64);
+      if (half == 0) s[row][jl] = -(tcol[1][row] + acc);
     }
     __syncthreads();
   }
