@@ -158,8 +158,9 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
 // mp, np: padded row / column counts of the slice arrays; Kp a multiple of 128.
 //
 // MFMA operands: lane l takes row (column) l % 32 and bytes [16 (l / 32),
-// 16 (l / 32) + 16) of its 32-byte record, identically for A and B (the order
-// of k inside a product does not matter, only that both operands agree);
+// 16 (l / 32) + 16) of its 32-byte record (in LDS: half l / 32 of the record),
+// identically for A and B (the order of k inside a product does not matter,
+// only that both operands agree);
 // D[i][j] sits in lane j + 32 u, register r with i = 8 (r / 4) + 4 u + r % 4.
 constexpr int oz_vmcnt(int n) {  // s_waitcnt immediate: vmcnt = n, lgkmcnt / expcnt not waited
   return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8);
@@ -175,11 +176,11 @@ struct OzCfg {
   static constexpr int BM = 64 * WM, BN = 32 * WN;
   // Copies of an operand with R records per slice: 2 R chunks of 16 bytes per
   // slice, NT / (2 R) slices per round, every thread busy in every round (a
-  // round past the last slice repeats it into a spare slot that is never read).
+  // round past the last slice copies that slice once more, see oz_issue_op).
   static constexpr int SPR_A = NT / (2 * BM), SPR_B = NT / (2 * BN);
   static constexpr int RND_A = (S + SPR_A - 1) / SPR_A, RND_B = (S + SPR_B - 1) / SPR_B;
-  static constexpr int ABYTES = RND_A * SPR_A * BM * OZ_BK;
-  static constexpr int BBYTES = RND_B * SPR_B * BN * OZ_BK;
+  static constexpr int ABYTES = S * BM * OZ_BK;
+  static constexpr int BBYTES = S * BN * OZ_BK;
   static constexpr int STAGE = ABYTES + BBYTES;
   static constexpr int COPIES = RND_A + RND_B;  // per thread and stage
   // epilogue staging: per wavefront 32 rows x 32 columns of FP64 (column stride 33)
@@ -191,6 +192,11 @@ struct OzCfg {
 };
 
 // One operand of one K step: R records (32 bytes each) of each of S slices.
+// In LDS a slice is stored as [k half][record][16 bytes]: the 16-byte chunk
+// at position c (c = 0 .. 2R - 1) is half c / R of record c % R. The copy
+// writes position = 16 * lane (global_load_lds), so each lane fetches the
+// chunk that belongs there; the fragment reads then cover contiguous 512-byte
+// runs per half wavefront (no bank conflicts).
 template <int R, int SPR, int RND, int S>
 __device__ __forceinline__ void oz_issue_op(signed char* __restrict__ dst, const signed char* src,
                                             size_t slab, int wave) {
@@ -198,10 +204,19 @@ __device__ __forceinline__ void oz_issue_op(signed char* __restrict__ dst, const
   const int off = ((64 * wave) % (2 * R)) * 16;  // byte offset of this wavefront's 64 chunks
 #pragma unroll
   for (int rd = 0; rd < RND; ++rd) {
-    const int q = rd * SPR + sub;      // LDS slot
-    const int qs = q < S ? q : S - 1;  // source slice
-    __builtin_amdgcn_global_load_lds(src + qs * slab, dst + q * R * OZ_BK + off, 16, 0, 0);
+    // past the last slice (odd S with two slices per round) the copy repeats
+    // slice S - 1 into its own slot: the same bytes twice, so whichever lands
+    // last leaves them unchanged, and both land before the stage is read
+    const int q = rd * SPR + sub < S ? rd * SPR + sub : S - 1;
+    __builtin_amdgcn_global_load_lds(src + q * slab, dst + q * R * OZ_BK + off, 16, 0, 0);
   }
+}
+// Source offset (bytes, within one slice's K step of the tile) of the chunk
+// that thread `tid` copies for an operand of R records.
+template <int R>
+__device__ __forceinline__ int oz_src_off(int tid) {
+  const int c = tid % (2 * R);
+  return (c % R) * OZ_BK + (c / R) * 16;
 }
 
 // One K step (32 bytes of every row / column of all S slices) into a stage.
@@ -225,17 +240,19 @@ __device__ __forceinline__ constexpr double oz_level_scale(int l) {
 template <class Cfg, int S>
 __device__ __forceinline__ void oz_compute(const signed char* __restrict__ st, v16i (&acc)[S][2],
                                            int wm, int wn, int lane) {
-  const int off = (lane & 31) * OZ_BK + 16 * (lane >> 5);
+  const int r32 = lane & 31, hf = lane >> 5;  // record within a 32-tile, k half
   v4i b[S];
 #pragma unroll
   for (int j = 0; j < S; ++j)
-    b[j] = *reinterpret_cast<const v4i*>(st + Cfg::ABYTES + j * Cfg::BN * OZ_BK + wn * OZ_BK + off);
+    b[j] = *reinterpret_cast<const v4i*>(st + Cfg::ABYTES + j * Cfg::BN * OZ_BK +
+                                         hf * Cfg::BN * 16 + (wn + r32) * 16);
 #pragma unroll
   for (int i = 0; i < S; ++i) {
     v4i a[2];
 #pragma unroll
     for (int ta = 0; ta < 2; ++ta)
-      a[ta] = *reinterpret_cast<const v4i*>(st + i * Cfg::BM * OZ_BK + (wm + 32 * ta) * OZ_BK + off);
+      a[ta] = *reinterpret_cast<const v4i*>(st + i * Cfg::BM * OZ_BK + hf * Cfg::BM * 16 +
+                                            (wm + 32 * ta + r32) * 16);
 #pragma unroll
     for (int j = 0; j < S - i; ++j)  // level index i + j (0-based)
 #pragma unroll
@@ -278,8 +295,8 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemm_kernel(
   const int bm = ti * Cfg::BM, bn = col0 + tj * Cfg::BN;
   const size_t slabA = (size_t)mp * Kp, slabB = (size_t)np * Kp;
   const size_t kstepA = (size_t)mp * OZ_BK, kstepB = (size_t)np * OZ_BK;
-  const signed char* Ab = As + (size_t)bm * OZ_BK + 16 * (tid % (2 * Cfg::BM));
-  const signed char* Bb = Bs + (size_t)bn * OZ_BK + 16 * (tid % (2 * Cfg::BN));
+  const signed char* Ab = As + (size_t)bm * OZ_BK + oz_src_off<Cfg::BM>(tid);
+  const signed char* Bb = Bs + (size_t)bn * OZ_BK + oz_src_off<Cfg::BN>(tid);
 
   v16i acc[S][2];
 #pragma unroll
@@ -418,6 +435,7 @@ static int ozaki_apply_s(hipStream_t s, int m, int n, int K, int col0, int ncols
     case 1: return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 1 per CU
     case 2: return ozaki_launch<S, 2, 4, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 128 x 128
     case 3: return ozaki_launch<S, 2, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);  // four stages
+    case 4: return ozaki_launch<S, 2, 4, 4, 1>(s, L, m, n, col0, ncols, C, ldc);  // 128 x 128, four stages
     default:
       if constexpr (S <= 6) return ozaki_launch<S, 2, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
       else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 7 S accumulators

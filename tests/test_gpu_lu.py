@@ -91,7 +91,7 @@ def test_ozaki_gemm_bit_exact_vs_oracle(L, S, m, n, K):
     assert np.array_equal(Cd.cpu().numpy(), ref)
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("S,m,n,K", [(5, 257, 129, 300), (6, 640, 384, 512), (3, 130, 260, 96)])
 def test_ozaki_gemm_block_shapes_bit_exact(L, shape, S, m, n, K):
     """Every block shape of the int8-sliced GEMM (tuning key 0) gives the
