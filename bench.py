@@ -27,6 +27,8 @@ import numpy as np
 
 METRIC = ("Time to relres≤1e-12: mixed-precision GMRES-IR vs plain FP64 on GPU, N=32768")
 PEAK_INT8_TOPS = 5000.0     # MI355X dense int8, tera-operations/s
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "ozaki_gemm_traffic_n32768.md")
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -38,6 +40,19 @@ def log(*a):
 def rhs_of(X, seed=43):
     return (np.sin(2 * np.pi * X[:, 0]) * np.cos(2 * np.pi * X[:, 1]) * np.cos(2 * np.pi * X[:, 2])
             + 1e-4 * np.random.default_rng(seed).standard_normal(X.shape[0]))
+
+
+def measured_traffic(N, slices):
+    """HBM bytes per launch of the int8-sliced trailing update, from the
+    committed FETCH_SIZE / WRITE_SIZE counter summary (its own run; the
+    counters cannot be read inside a timed run). Only for the workload the
+    summary was measured on."""
+    if N != 32768 or slices != 5 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(N_full, n_sample):
@@ -65,7 +80,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
-    ap.add_argument("--slices", type=int, default=6)
+    ap.add_argument("--slices", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--nb", type=int, default=512)
     ap.add_argument("--cpu-sample", type=int, default=12288)
@@ -191,10 +206,13 @@ def main():
         achieved = st["trailing_ops"] / (st["trailing_ms"] * 1e-3) / 1e12
         roofline = {"bound": "mfma", "kernel": f"ozaki_gemm_kernel<{args.slices}>",
                     "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
-                    "frac": achieved / PEAK_INT8_TOPS, "traffic": None,
+                    "frac": achieved / PEAK_INT8_TOPS,
+                    "traffic": measured_traffic(N, args.slices),
                     "note": ("int8 tera-operations/s of the trailing updates: 2 m n k "
                              f"x {args.slices * (args.slices + 1) // 2} slice pairs per launch, "
-                             f"{st['launches']} launches, avg {avg_ms:.3f} ms (HIP events)")}
+                             f"{st['launches']} launches, avg {avg_ms:.3f} ms (HIP events); "
+                             "traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                             "(profiles/ozaki_gemm_traffic_n32768.md)")}
     plain_roof = None
     if plain_stats and plain_stats.get("launches"):
         a = plain_stats["trailing_ops"] / (plain_stats["trailing_ms"] * 1e-3) / 1e12

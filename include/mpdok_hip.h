@@ -70,7 +70,8 @@ int mpdok_set_stream(hipStream_t s);
 /* Kernel-shape choices picked by the tuner. Key 0: block shape of the
  * int8-sliced GEMM (0 = 128 x 64 tiles, 2 workgroups per CU (default);
  * 1 = the same, 1 per CU; 2 = 128 x 128; 3 = four pipeline stages;
- * 4 = 128 x 128 with four stages). No reference counterpart. */
+ * 4 = 128 x 128 with four stages). Key 1: row tiles per group of its tile
+ * order (0 = 8). No reference counterpart. */
 int mpdok_set_tuning(int key, int value);
 int mpdok_get_tuning(int key);
 size_t mpdok_reduce_workspace_bytes(int n);

@@ -18,6 +18,8 @@ static int g_tuning[TUNE_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
 int tuning_get(int key) { return key >= 0 && key < TUNE_COUNT ? g_tuning[key] : 0; }
 int tuning_set(int key, int value) {
   MPDOK_REQUIRE(key >= 0 && key < TUNE_COUNT, "tuning: unknown key %d", key);
+  MPDOK_REQUIRE(key != TUNE_OZAKI_GROUP || (value >= 0 && value <= 256),
+                "tuning: group %d out of range", value);
   MPDOK_REQUIRE(key != TUNE_OZAKI_SHAPE || (value >= 0 && value <= 4),
                 "tuning: key %d out of range", key);
   g_tuning[key] = value;

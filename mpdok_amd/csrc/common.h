@@ -29,6 +29,7 @@ const char* error_string();  // this thread's last error message
 // mpdok_set_tuning; read at launch time).
 enum TuneKey : int {
   TUNE_OZAKI_SHAPE = 0,  // int8-sliced GEMM block shape (waves M x N, stages, workgroups per CU)
+  TUNE_OZAKI_GROUP = 1,  // int8-sliced GEMM: row tiles per group of the tile order (0: 8)
   TUNE_COUNT = 8,
 };
 int tuning_get(int key);
@@ -133,14 +134,15 @@ __device__ __forceinline__ T nt_load(const T* p) {
 // the next column block, so a band of A rows stays in that XCD's L2. Launch
 // with tiles_launch(tm, tn) workgroups; surplus workgroups get `false`.
 constexpr int NXCD = 8, GROUP_M = 8;
-__device__ __forceinline__ bool tile_of(int b, int tm, int tn, int& ti, int& tj) {
+__device__ __forceinline__ bool tile_of(int b, int tm, int tn, int& ti, int& tj,
+                                        int group = GROUP_M) {
   const int T = tm * tn;
   const int per = (T + NXCD - 1) / NXCD;
   const int t = (b % NXCD) * per + b / NXCD;
   if (b / NXCD >= per || t >= T) return false;
-  const int width = GROUP_M * tn;
-  const int g = t / width, first = g * GROUP_M;
-  const int gsz = min(GROUP_M, tm - first);
+  const int width = group * tn;
+  const int g = t / width, first = g * group;
+  const int gsz = min(group, tm - first);
   const int r = t % width;
   ti = first + r % gsz;
   tj = r / gsz;

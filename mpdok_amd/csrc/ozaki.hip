@@ -280,7 +280,7 @@ template <int S, int WM, int WN, int NST, int OCC>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemm_kernel(
     int m, int n, int mp, int np, int Kp, int col0, int ncols, const signed char* __restrict__ As,
     const signed char* __restrict__ Bs, const int* __restrict__ ea, const int* __restrict__ eb,
-    double* __restrict__ C, int64_t ldc) {
+    double* __restrict__ C, int64_t ldc, int group) {
   typedef OzCfg<S, WM, WN, NST, OCC> Cfg;
   // one array per pipeline stage: the compiler then tells the stages apart
   // and waits only for the copies into the stage being read
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemm_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave % WM) * 64, wn = (wave / WM) * 32;
   int ti, tj;
-  if (!tile_of(blockIdx.x, mp / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN, ti, tj)) return;
+  if (!tile_of(blockIdx.x, mp / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN, ti, tj, group)) return;
   const int bm = ti * Cfg::BM, bn = col0 + tj * Cfg::BN;
   const size_t slabA = (size_t)mp * Kp, slabB = (size_t)np * Kp;
   const size_t kstepA = (size_t)mp * OZ_BK, kstepB = (size_t)np * OZ_BK;
@@ -416,8 +416,9 @@ static int ozaki_launch(hipStream_t s, const OzLayout& L, int m, int n, int col0
     return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // stages too large: smallest shape
   } else {
     const int grid = tiles_launch(L.mp / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN);
+    const int g = tuning_get(TUNE_OZAKI_GROUP);
     ozaki_gemm_kernel<S, WM, WN, NST, OCC><<<grid, Cfg::NT, 0, s>>>(
-        m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc);
+        m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, g > 0 ? g : GROUP_M);
     MPDOK_HIP(hipGetLastError());
     return OK;
   }
