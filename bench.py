@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=12288)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip torch solve and plain GMRES")
+    ap.add_argument("--configs", action="store_true",
+                    help="also measure the other configurations (CPU-runnable N = 4096 case, "
+                         "out-of-core stream, batched systems, adjoint) into 'configs'")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU layout is "
@@ -243,8 +246,134 @@ def main():
         "assembly_ms": t_assemble * 1e3,
         "roofline": roofline, "cpu_baseline": cpu, **extra,
     }
+    if args.configs:
+        torch.cuda.empty_cache()
+        out["configs"] = other_configs(log)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def other_configs(log):
+    """Secondary configurations, one measurement each (seconds, relres, rates)."""
+    import warnings
+    import torch
+    from mpdok_amd import _dev, _ops
+    from mpdok_amd.mpdok_ops import LUIRSolver, MPDOKSolver
+    res = {}
+    warnings.simplefilter("ignore")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        v = fn()
+        torch.cuda.synchronize()
+        return v, time.perf_counter() - t0
+
+    # CPU-runnable case: N = 4096 in [0,1]^2, eps = 8, nugget 1e-8
+    N = 4096
+    X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 2))
+    b = (np.sin(2 * np.pi * X[:, 0]) * np.cos(2 * np.pi * X[:, 1])
+         + 1e-4 * np.random.default_rng(43).standard_normal(N))
+    A = _dev.empty((N, N), torch.float64, "F")
+    _ops.rbf_assemble(_dev.as_device(X, torch.float64), 0, 64.0, 1e-8, A)
+    bd = _dev.as_device(b, torch.float64)
+    rel = lambda x: _ops.norm(_ops.residual(A, x, bd, mode="dot2")) / _ops.norm(bd)
+    s = MPDOKSolver()
+    kw = dict(tol=1e-12, maxiter_outer=20, precond="lu", factor_precision="ozaki", slices=6)
+    s.solve(A, bd, **kw)
+    x, t = timed(lambda: s.solve(A, bd, **kw))
+    lu = LUIRSolver()
+    x2, t2 = timed(lambda: lu.solve(A, bd, tol=1e-12, maxiter_outer=10))
+    res["n4096_2d"] = {"gmres_ir_int8_lu": {"s": t, "relres": rel(x)},
+                       "lu_ir_fp32_lu_reference_symbol": {"s": t2, "relres": rel(x2)}}
+    log(f"[configs] n4096 {res['n4096_2d']}")
+    del A, s, lu
+    torch.cuda.empty_cache()
+
+    # out-of-core: Matern-3/2, length scale 0.1, nugget 1e-4, N = 65536 (the
+    # 262144-point FP32 operator, 275 GB, exceeds this process's host memory)
+    from mpdok_amd.mpdok_ooc import MPDOKOOCSolver
+    N = 65536
+    X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
+    o = MPDOKOOCSolver(tile_rows=4096)
+    _, tb = timed(lambda: o.build(X, reg=1e-4, store="ram", verbose=False, kernel="matern32",
+                                  length_scale=0.1))
+    v = torch.rand(N, dtype=torch.float32, device=_dev.device())
+    o._tiled_sgemv(v)
+    _, ts = timed(lambda: o._tiled_sgemv(v))
+    v64 = v.double()
+    _, tf = timed(lambda: o._tiled_dgemv_fp64(v64))
+    res["ooc_matern_n65536"] = {"build_s": tb, "stream_product_s": ts,
+                                "stream_GBps": N * N * 4 / ts / 1e9,
+                                "matrix_free_fp64_product_s": tf}
+    log(f"[configs] ooc {res['ooc_matern_n65536']}")
+    o.free()
+    del o
+    torch.cuda.empty_cache()
+
+    # batched uneven systems: 256 systems, N log-uniform in [256, 8192], 4 RHS
+    from mpdok_amd.batched import BatchTuner, c4_workload
+    sysw = c4_workload(seed=0)
+    tuner = BatchTuner()
+    choice = tuner.choose(sysw)
+    _, info = tuner.solve(sysw)
+    Ns = np.array([len(q[0]) for q in sysw], dtype=float)
+    res["batched_256"] = {"s": info["seconds"], "config": choice,
+                          "all_converged": bool(info["converged"].all()),
+                          "max_relres": float(np.nanmax(info["relres"])),
+                          "lu_flop": float((2.0 / 3.0 * Ns ** 3).sum())}
+    log(f"[configs] batched {res['batched_256']}")
+    torch.cuda.empty_cache()
+
+    # adjoint: N = 16384 training, 4096 validation, eps 6, log nugget -12
+    from mpdok_amd.adjoint import rbf_validation_loss
+    rng = np.random.default_rng(7)
+    Xt, Xv = rng.uniform(0, 1, (16384, 2)), rng.uniform(0, 1, (4096, 2))
+    f = lambda P: np.sin(3 * np.pi * P[:, 0]) * np.exp(-P[:, 1])
+    yt = f(Xt) + 1e-3 * rng.standard_normal(16384)
+    yv = f(Xv) + 1e-3 * rng.standard_normal(4096)
+    e = torch.tensor(6.0, dtype=torch.float64, requires_grad=True)
+    ls = torch.tensor(-12.0, dtype=torch.float64, requires_grad=True)
+
+    def fwd_bwd():
+        L = rbf_validation_loss(e, ls, Xt, yt, Xv, yv, tol=1e-12, maxiter_outer=30)
+        L.backward()
+        return L
+    L, tadj = timed(fwd_bwd)
+    h = 1e-5
+    Lf = lambda a, c: float(rbf_validation_loss(a, c, Xt, yt, Xv, yv, tol=1e-12,
+                                                maxiter_outer=30))
+    fd_e = (Lf(6.0 + h, -12.0) - Lf(6.0 - h, -12.0)) / (2 * h)
+    fd_s = (Lf(6.0, -12.0 + h) - Lf(6.0, -12.0 - h)) / (2 * h)
+    # dense FP64 reference gradient (closed form, rocSOLVER FP64 solves through torch)
+    dev = _dev.device()
+    Xt_d, Xv_d = (torch.as_tensor(Xt, device=dev), torch.as_tensor(Xv, device=dev))
+    K = torch.empty((16384, 16384), dtype=torch.float64, device=dev)
+    _ops.rbf_assemble(Xt_d, 0, 36.0, float(np.exp(-12.0)), K)
+    D2v = torch.cdist(Xv_d, Xt_d) ** 2
+    Ev = torch.exp(-36.0 * D2v)
+    yt_d, yv_d = torch.as_tensor(yt, device=dev), torch.as_tensor(yv, device=dev)
+    alpha = torch.linalg.solve(K, yt_d)
+    r = Ev @ alpha - yv_d
+    g = 2.0 * r / 4096
+    lam = torch.linalg.solve(K, Ev.T @ g)
+    K.fill_diagonal_(0.0)                       # K - sigma I = exp(-gamma D2)
+    D2 = torch.cdist(Xt_d, Xt_d) ** 2
+    dgamma = g @ ((-D2v * Ev) @ alpha) - lam @ ((-D2 * K) @ alpha)
+    ref_e, ref_s = float(2 * 6.0 * dgamma), float(-np.exp(-12.0) * (lam @ alpha))
+    del K, D2, D2v, Ev
+    torch.cuda.empty_cache()
+    res["adjoint_n16384"] = {"forward_backward_s": tadj, "loss": float(L),
+                             "dL_deps": float(e.grad), "dense_fp64_deps": ref_e,
+                             "rel_diff_dense_eps": abs(float(e.grad) - ref_e) / abs(ref_e),
+                             "fd_deps": fd_e,
+                             "rel_diff_fd_eps": abs(float(e.grad) - fd_e) / abs(fd_e),
+                             "dL_dlogsigma": float(ls.grad), "dense_fp64_dlogsigma": ref_s,
+                             "rel_diff_dense_logsigma": abs(float(ls.grad) - ref_s) / abs(ref_s),
+                             "fd_dlogsigma": fd_s,
+                             "rel_diff_fd_logsigma": abs(float(ls.grad) - fd_s) / abs(fd_s)}
+    log(f"[configs] adjoint {res['adjoint_n16384']}")
+    return res
 
 
 def _plain_gmres(solver, A, b, tol):

@@ -9,8 +9,8 @@ first by the longest-processing-time rule, so small systems fill the gaps left
 by large ones.
 
 `BatchTuner` picks the number of streams, the LU block size and the factor
-precision by timing candidates on a sample of the batch (largest, median and
-smallest systems), and caches the choice per size profile. This path has no
+precision by timing candidates on a sample of the batch (every k-th system by
+size, about 24), and caches the choice per size profile. This path has no
 counterpart in the reference; it follows the north star's batched
 configuration (256 systems, N log-uniform in [256, 8192], 4 right-hand sides).
 """
@@ -99,7 +99,7 @@ class BatchTuner:
     sample of it; remembers the choice per size profile (sizes rounded to
     powers of two)."""
 
-    CANDIDATES = {"nstreams": (2, 4, 8), "nb": (256, 512), "precision": ("fp64", "ozaki")}
+    CANDIDATES = {"nstreams": (4, 8, 16), "nb": (256, 512), "precision": ("fp64", "ozaki")}
 
     def __init__(self):
         self.cache = {}
@@ -113,10 +113,10 @@ class BatchTuner:
         key = self.profile(systems)
         if key in self.cache:
             return self.cache[key]
+        # every k-th system by size: the sample keeps the batch's size mix
         by_n = sorted(systems, key=lambda s: len(s[0]))
-        k = len(by_n)
-        sample = [by_n[-1], by_n[k // 2], by_n[0]] if k >= 3 else by_n
-        sample = sample * 3   # enough systems to keep several streams busy
+        step = max(1, len(by_n) // 24)
+        sample = by_n[::step]
         best = None
         for ns, nb, prec in itertools.product(*self.CANDIDATES.values()):
             _, info = solve_batched(sample, tol=tol, reg=reg, precision=prec, nb=nb,

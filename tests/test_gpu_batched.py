@@ -72,7 +72,7 @@ def test_c4_workload_shapes_and_tuner_choice():
         assert 2.0 <= eps <= 12.0
     tuner = BatchTuner()
     choice = tuner.choose(sysw)
-    assert choice["nstreams"] in (2, 4, 8) and choice["nb"] in (256, 512)
+    assert choice["nstreams"] in (4, 8, 16) and choice["nb"] in (256, 512)
     assert tuner.choose(sysw) is choice            # cached per size profile
     xs, info = tuner.solve(sysw)
     assert info["converged"].all()
