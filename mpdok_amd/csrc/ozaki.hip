@@ -183,9 +183,9 @@ struct OzCfg {
   static constexpr int BBYTES = S * BN * OZ_BK;
   static constexpr int STAGE = ABYTES + BBYTES;
   static constexpr int COPIES = RND_A + RND_B;  // per thread and stage
-  // epilogue staging: per wavefront 32 rows x 32 columns of FP64 (column stride 33)
+  // epilogue staging: per wavefront 32 x 32 FP64 (column stride 33) and 32 exponents
   static constexpr int WAVES_PER_BUF = (WM * WN + NST - 1) / NST;
-  static constexpr int EPI = WAVES_PER_BUF * 32 * 33 * 8;
+  static constexpr int EPI = WAVES_PER_BUF * (32 * 33 + 32) * 8;
   static constexpr int BUF = STAGE > EPI ? STAGE : EPI;
   static constexpr bool FITS = NST * BUF <= 160 * 1024;  // LDS of one workgroup
   static_assert(NT % (2 * BM) == 0 && NT % (2 * BN) == 0, "copy split");
@@ -330,42 +330,64 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemm_kernel(
   __builtin_amdgcn_s_waitcnt(oz_vmcnt(0));
   __builtin_amdgcn_s_barrier();
 
-  // Epilogue: fold the levels (FP64, l ascending), scale by 2^(ea_i + eb_j),
-  // stage through LDS so that each store covers 32 consecutive rows of a
-  // column, then C -= value.
+  // Epilogue, per 32-row half `ta` of the wavefront's 64 x 32 C tile: the 16
+  // C values of a lane are loaded together (nothing stored in between, so
+  // they are all in flight at once; a load after a store to the same array
+  // would wait for it), the next half's loads are issued as soon as this
+  // half's accumulators are folded, then the stores follow. The levels are
+  // folded (FP64, l ascending) in the MFMA layout and staged through LDS
+  // unscaled, with each column's exponent beside them, so that after the
+  // transpose lane (row r, columns c) scales by 2^(ea_i + eb_j) in one ldexp
+  // and stores C -= value over 32 consecutive rows of a column.
   const int wb = wave % NST, ws = wave / NST;
   signed char* ebuf = wb == 0 ? st0 : wb == 1 ? st1 : wb == 2 ? st2 : st3;
-  double* epi = reinterpret_cast<double*>(ebuf) + ws * (32 * 33);
-  const int jl = lane & 31;
-  const int ej = eb[bn + wn + jl];  // bn + wn + jl < np
+  double* epi = reinterpret_cast<double*>(ebuf) + ws * (32 * 33 + 32);
+  int* ecol = reinterpret_cast<int*>(epi + 32 * 33);
+  const int rl = lane & 31, cph = lane >> 5;
+  if (cph == 0) ecol[rl] = eb[bn + wn + rl];  // bn + wn + rl < np
+  double cv[2][16];
+  auto load_c = [&](int ta) {
+    const int i = bm + wm + 32 * ta + rl;
 #pragma unroll
-  for (int ta = 0; ta < 2; ++ta) {
+    for (int c = 0; c < 16; ++c) {
+      const int j = bn + wn + cph + 2 * c;
+      cv[ta][c] = (i < m && j < col0 + ncols) ? C[i + (int64_t)j * ldc] : 0.0;
+    }
+  };
+  auto fold = [&](int ta) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int il = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      const int il = 8 * (r >> 2) + 4 * cph + (r & 3);
       double c = 0.0;
 #pragma unroll
       for (int l = 0; l < S; ++l) c += (double)acc[l][ta][r] * oz_level_scale(l);
-      epi[jl * 33 + il] = ldexp(c, ea[bm + wm + 32 * ta + il] + ej);
+      epi[rl * 33 + il] = c;
     }
+  };
+  auto store_c = [&](int ta) {
+    const int i = bm + wm + 32 * ta + rl;
+    const int ei = ea[i];  // i < mp
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int jc = cph + 2 * c, j = bn + wn + jc;
+      const double v = ldexp(epi[jc * 33 + rl], ei + ecol[jc]);
+      if (i < m && j < col0 + ncols) C[i + (int64_t)j * ldc] = cv[ta][c] - v;
+    }
+  };
+  auto lds_sync = [] {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // lane -> (row lane % 32, every other column)
-    const int i = bm + wm + 32 * ta + (lane & 31);
-#pragma unroll 4
-    for (int c = lane >> 5; c < 32; c += 2) {
-      const int j = bn + wn + c;
-      const double v = epi[c * 33 + (lane & 31)];
-      if (i < m && j < col0 + ncols) {
-        double* cp = C + i + (int64_t)j * ldc;
-        *cp = *cp - v;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
+  };
+  load_c(0);
+  fold(0);
+  load_c(1);
+  lds_sync();
+  store_c(0);
+  lds_sync();
+  fold(1);
+  lds_sync();
+  store_c(1);
 }
 
 // Slice arrays: rows padded to a multiple of 128; columns to a multiple of 128
