@@ -223,6 +223,17 @@ int mpdok_lu_set_profiling(void* handle, int on);
  * current step finishes (default on; no-pivot handles only). */
 int mpdok_lu_set_lookahead(void* handle, int on);
 int mpdok_lu_stats(void* handle, double* trailing_ms, double* trailing_ops, int* launches);
+/* Adaptive slice counts for the int8-sliced variant (default on when
+ * slices <= 5): every panel pair is cut into `slices` digits, and each
+ * trailing update multiplies only the leading S of them, S the fewest in
+ * [smin, slices] whose error bound 2^(E - (8 S - 1)) (E = max row exponent
+ * of L21 + max column exponent of U12) stays within the first update's
+ * minus `delta` bits. Decided on the device; no host synchronisation. No
+ * reference counterpart (the reference factors in FP32). */
+int mpdok_lu_set_adaptive(void* handle, int on, int smin, int delta);
+/* Slice counts of the last factorisation, one per outer step (*nsteps of
+ * them; at most n written). */
+int mpdok_lu_slice_counts(void* handle, int* out, int n, int* nsteps);
 /* Device pointers to the packed factors (column-major, ld N) and the row
  * permutation ((P b)[i] = b[perm[i]], NULL without pivoting). */
 int mpdok_lu_factors(void* handle, void** LU, int** perm, int* elem_size);
@@ -238,6 +249,16 @@ size_t mpdok_ozaki_workspace_bytes(int m, int n, int K, int slices);
 int mpdok_ozaki_gemm_sub(hipStream_t s, int slices, int m, int n, int K, const double* A,
                          int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
                          void* work);
+/* The adaptive pieces of the factorisation, for parity tests: cut A and B
+ * into smax (3..5) digits, then C -= the product of the leading S digits,
+ * S = s_fixed (>= 2) or, for s_fixed = 0, the decision rule of
+ * mpdok_lu_set_adaptive against the reference exponent sum e0_in (INT_MIN:
+ * this call is the first update and records it). dev_scratch: 2 device
+ * ints. *s_used and *e_ref receive S and the reference sum. Synchronises s. */
+int mpdok_ozaki_gemm_sub_adaptive(hipStream_t s, int smax, int m, int n, int K, const double* A,
+                                  int64_t lda, const double* B, int64_t ldb, double* C,
+                                  int64_t ldc, void* work, int s_fixed, int smin, int delta,
+                                  int e0_in, int* dev_scratch, int* s_used, int* e_ref);
 
 /* ---- refinement drivers -----------------------------------------------------
  * GMRES-IR: FP64 outer refinement (residual mode 0 FP64 / 1 Dot2) around one

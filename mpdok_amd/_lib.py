@@ -83,9 +83,13 @@ SIGNATURES = {
     "mpdok_lu_set_profiling": (I, [P, I]),
     "mpdok_lu_set_lookahead": (I, [P, I]),
     "mpdok_lu_stats": (I, [P, ctypes.POINTER(D), ctypes.POINTER(D), PI]),
+    "mpdok_lu_set_adaptive": (I, [P, I, I, I]),
+    "mpdok_lu_slice_counts": (I, [P, PI, I, PI]),
     "mpdok_gemm": (I, [P, I, I, I, I, D, P, I64, P, I64, D, P, I64, I]),
     "mpdok_ozaki_workspace_bytes": (SZ, [I, I, I, I]),
     "mpdok_ozaki_gemm_sub": (I, [P, I, I, I, I, P, I64, P, I64, P, I64, P]),
+    "mpdok_ozaki_gemm_sub_adaptive": (I, [P, I, I, I, I, P, I64, P, I64, P, I64, P, I, I, I, I, P,
+                                          PI, PI]),
     # refinement drivers
     "mpdok_gmres_ir": (I, [P, P, I64, I, P, I64, I, P, I, P, P, D, I, I, D, I, I, I, PI, P, I,
                            PI, PI]),

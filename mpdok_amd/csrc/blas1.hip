@@ -20,8 +20,10 @@ int tuning_set(int key, int value) {
   MPDOK_REQUIRE(key >= 0 && key < TUNE_COUNT, "tuning: unknown key %d", key);
   MPDOK_REQUIRE(key != TUNE_OZAKI_GROUP || (value >= 0 && value <= 256),
                 "tuning: group %d out of range", value);
-  MPDOK_REQUIRE(key != TUNE_OZAKI_SHAPE || (value >= 0 && value <= 4),
+  MPDOK_REQUIRE(key != TUNE_OZAKI_SHAPE || (value >= 0 && value <= 12),
                 "tuning: key %d out of range", key);
+  MPDOK_REQUIRE(key != TUNE_OZAKI_STAGGER || (value >= 0 && value <= 64),
+                "tuning: stagger %d out of range", value);
   g_tuning[key] = value;
   return OK;
 }

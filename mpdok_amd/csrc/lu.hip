@@ -58,10 +58,18 @@ struct LUHandle {
   hipEvent_t ev_next_cols = nullptr, ev_panel = nullptr;
   void* Wp = nullptr;
   bool lookahead = true;
+  // int8-sliced variant: per-update slice counts picked on the device from the
+  // panels' exponents (adaptive, slices <= 5); d_S[0] is the first update's
+  // exponent sum, d_S[1 + step] the count of step `step`
+  bool adaptive = true;
+  int smin = 2, delta = 0;
+  int* d_S = nullptr;
+  std::vector<int> steps_S;  // host copy after a factorisation
   // optional HIP-event timing of the trailing-update launches
   bool profile = false;
   std::vector<hipEvent_t> ev;
-  std::vector<double> ev_ops;  // algorithmic operations of each timed launch
+  std::vector<double> ev_ops;  // algorithmic operations of each timed launch (per slice pair)
+  std::vector<int> ev_step;    // step of each timed launch
   double trailing_ms = 0.0, trailing_ops = 0.0;
   int trailing_launches = 0;
 };
@@ -87,7 +95,8 @@ void lu_destroy(LUHandle* h) {
   if (h->side) (void)hipStreamDestroy(h->side);
   if (h->Wp) (void)hipFree(h->Wp);
   void* ptrs[] = {h->LU, h->Linv, h->Uinv, h->W, h->ipiv, h->perm, h->d_info,
-                  h->piv_work, h->oz_work, h->v1, h->v2, h->Lbinv, h->Ubinv, h->v3, h->acc};
+                  h->piv_work, h->oz_work, h->v1, h->v2, h->Lbinv, h->Ubinv, h->v3, h->acc,
+                  h->d_S};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -132,7 +141,11 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   alloc(&h->piv_work, panel_gepp_workspace_bytes(N));
   alloc(&h->v1, (size_t)N * 8);
   alloc(&h->v2, (size_t)N * 8);
-  if (precision == LU_OZAKI) alloc(&h->oz_work, ozaki_workspace_bytes(N, N, nb, slices));
+  if (precision == LU_OZAKI) {
+    alloc(&h->oz_work, ozaki_workspace_bytes(N, N, nb, slices));
+    alloc((void**)&h->d_S, sizeof(int) * (N / nb + 3));
+  }
+  h->adaptive = precision == LU_OZAKI && slices <= 5;
   h->fast_solve = (N % TRSV_BLOCK) == 0;
   if (N >= TRSV_BLOCK) {  // also for a later lu_resize to a multiple of TRSV_BLOCK
     alloc(&h->Lbinv, (size_t)N * TRSV_BLOCK * h->es);
@@ -261,6 +274,21 @@ static int block_row_solve(LUHandle* h, hipStream_t s, int k, int kb, int c0, T*
   return OK;
 }
 
+static bool adaptive_on(const LUHandle* h) {
+  return h->precision == LU_OZAKI && h->adaptive && h->slices <= 5 && h->d_S != nullptr;
+}
+
+// Slices of the panel pair of step k (smax digits), and with adaptive slice
+// counts the decision for its trailing update.
+static int slice_step(LUHandle* h, hipStream_t s, int k, int kb, int n3, const double* L21,
+                      const double* U12, int64_t ld) {
+  MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb, L21, ld, U12, ld, h->oz_work));
+  if (adaptive_on(h))
+    MPDOK_TRY(ozaki_decide(s, h->slices, n3, n3, kb, h->oz_work, h->smin, h->delta, k == 0,
+                           h->d_S, h->d_S + 1 + k / h->nb));
+  return OK;
+}
+
 // Trailing update of step k restricted to columns [k+kb+c_off, k+kb+c_off+ncols).
 // For the int8-sliced variant the operands must have been sliced already.
 template <typename T>
@@ -274,8 +302,8 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
   const size_t nev = h->ev_ops.size();
   if (h->profile) {
     // int8-sliced: S(S+1)/2 int8 products of 2 m n k operations each
-    const double pairs = h->precision == LU_OZAKI ? h->slices * (h->slices + 1) / 2.0 : 1.0;
-    h->ev_ops.push_back(2.0 * m3 * ncols * kb * pairs);
+    h->ev_ops.push_back(2.0 * m3 * ncols * kb);
+    h->ev_step.push_back(k / h->nb);
     while (h->ev.size() < 2 * h->ev_ops.size()) {
       hipEvent_t e;
       MPDOK_HIP(hipEventCreate(&e));
@@ -283,7 +311,11 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
     }
     MPDOK_HIP(hipEventRecord(h->ev[2 * nev], s));
   }
-  if (h->precision == LU_OZAKI) {
+  if (adaptive_on(h)) {
+    MPDOK_TRY(ozaki_apply_dyn(s, h->slices, m3, n3, kb, c_off, ncols,
+                              reinterpret_cast<double*>(at(k + kb, k + kb)), ld, h->oz_work,
+                              h->d_S + 1 + k / h->nb));
+  } else if (h->precision == LU_OZAKI) {
     MPDOK_TRY(ozaki_apply(s, h->slices, m3, n3, kb, c_off, ncols,
                           reinterpret_cast<double*>(at(k + kb, k + kb)), ld, h->oz_work));
   } else {
@@ -317,9 +349,8 @@ static int factor_impl(LUHandle* h, hipStream_t s) {
       if (k > 0) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
       MPDOK_TRY(block_row_solve<T>(h, s, k, kb, k + kb, W));
       if (h->precision == LU_OZAKI)
-        MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb,
-                              reinterpret_cast<const double*>(at(k + kb, k)), ld,
-                              reinterpret_cast<const double*>(at(k, k + kb)), ld, h->oz_work));
+        MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k + kb, k)),
+                             reinterpret_cast<const double*>(at(k, k + kb)), ld));
       const int nk = nb < n3 ? nb : n3;
       MPDOK_TRY(trailing_cols<T>(h, s, k, kb, 0, nk));
       MPDOK_HIP(hipEventRecord(h->ev_next_cols, s));
@@ -339,9 +370,8 @@ static int factor_impl(LUHandle* h, hipStream_t s) {
     if (n3 <= 0) continue;
     MPDOK_TRY(block_row_solve<T>(h, s, k, kb, k + kb, W));
     if (h->precision == LU_OZAKI)
-      MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb,
-                            reinterpret_cast<const double*>(at(k + kb, k)), ld,
-                            reinterpret_cast<const double*>(at(k, k + kb)), ld, h->oz_work));
+      MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k + kb, k)),
+                           reinterpret_cast<const double*>(at(k, k + kb)), ld));
     MPDOK_TRY(trailing_cols<T>(h, s, k, kb, 0, n3));
   }
   if (h->pivot == PIV_PARTIAL) MPDOK_TRY(ipiv_to_perm(s, h->ipiv, h->perm, N));
@@ -355,6 +385,7 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
   h->factored = false;
   const int N = h->N;
   h->ev_ops.clear();
+  h->ev_step.clear();
   MPDOK_HIP(hipMemsetAsync(h->d_info, 0, sizeof(int), s));
   // copy (and convert / transpose) A into the handle's column-major storage
   MPDOK_TRY(cast_matrix(s, A, F64, lda, order, h->LU, fp64_storage(h->precision) ? F64 : F32, N,
@@ -365,6 +396,11 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
     MPDOK_TRY(factor_impl<float>(h, s));
   int hinfo = 0;
   MPDOK_HIP(hipMemcpyAsync(&hinfo, h->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+  const int nsteps = (N + h->nb - 1) / h->nb;
+  h->steps_S.assign(nsteps, h->precision == LU_OZAKI ? h->slices : 0);
+  if (adaptive_on(h) && N > h->nb)
+    MPDOK_HIP(hipMemcpyAsync(h->steps_S.data(), h->d_S + 1, sizeof(int) * (nsteps - 1),
+                             hipMemcpyDeviceToHost, s));
   MPDOK_HIP(hipStreamSynchronize(s));
   if (info) *info = hinfo;
   h->factored = (hinfo == 0);
@@ -376,7 +412,8 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
       float ms = 0.f;
       MPDOK_HIP(hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]));
       h->trailing_ms += ms;
-      h->trailing_ops += h->ev_ops[i];
+      const int Sk = h->precision == LU_OZAKI ? h->steps_S[h->ev_step[i]] : 1;
+      h->trailing_ops += h->ev_ops[i] * (h->precision == LU_OZAKI ? Sk * (Sk + 1) / 2.0 : 1.0);
     }
   }
   return OK;
@@ -497,4 +534,30 @@ MPDOK_API int mpdok_gemm(hipStream_t s, int dtype, int m, int n, int k, double a
                                beta, (double*)C, ldc, 0);
   return mpdok::gemm<float>(s, m, n, k, (float)alpha, (const float*)A, lda, (const float*)B, ldb,
                             (float)beta, (float*)C, ldc, round);
+}
+
+// Adaptive slice counts of the int8-sliced variant (default on for slices
+// <= 5): each trailing update takes the fewest slices in [smin, slices] whose
+// error bound stays within the first update's minus `delta` bits.
+MPDOK_API int mpdok_lu_set_adaptive(void* hv, int on, int smin, int delta) {
+  auto* h = static_cast<mpdok::LUHandle*>(hv);
+  MPDOK_REQUIRE(h != nullptr, "lu_set_adaptive: null handle");
+  MPDOK_REQUIRE(!on || (smin >= 2 && smin <= h->slices),
+                "lu_set_adaptive: smin=%d outside [2, %d]", smin, h->slices);
+  MPDOK_REQUIRE(!on || h->slices <= 5, "lu_set_adaptive: needs slices <= 5 (got %d)", h->slices);
+  h->adaptive = on != 0;
+  if (on) {
+    h->smin = smin;
+    h->delta = delta;
+  }
+  return 0;
+}
+// Slice counts the last factorisation used, one per step (n of them at most).
+MPDOK_API int mpdok_lu_slice_counts(void* hv, int* out, int n, int* nsteps) {
+  auto* h = static_cast<mpdok::LUHandle*>(hv);
+  MPDOK_REQUIRE(h != nullptr, "lu_slice_counts: null handle");
+  const int ns = (int)h->steps_S.size();
+  for (int i = 0; i < ns && i < n; ++i) out[i] = h->steps_S[i];
+  if (nsteps) *nsteps = ns;
+  return 0;
 }

@@ -1,18 +1,20 @@
 // Trailing update C -= A B of the blocked LU on the int8 matrix cores, to
-// about 7*S bits, with FP64 storage of C (Ozaki scheme; Ozaki, Ogita, Oishi,
-// Rump, Numer. Algorithms 59, 2012).
+// about 8 S - 1 bits, with FP64 storage of C (Ozaki scheme; Ozaki, Ogita,
+// Oishi, Rump, Numer. Algorithms 59, 2012).
 //
-// Splitting: each row of A (m x K) and each column of B (K x n) is scaled by a
-// power of two 2^e with e = frexp exponent of its max magnitude, then cut into
-// S signed 7-bit digits d_1..d_S (|d| <= 127) with
-//     a = 2^e * sum_q d_q 2^(-7 q)      (truncation < 2^(e - 7 S)).
+// Splitting: each row of A (m x K) and each column of B (K x n) gets a power
+// of two 2^e, the smallest with max|.| * 2^(7 - e) <= 127. With P = 8 S - 1,
+// an element v becomes the integer V = rint(v 2^(P - e)) (|V| <= 127 256^(S-1)),
+// written in balanced base 256 from the least significant digit:
+//     V = sum_q d_q 256^(S - q),  d_q in [-128, 127] (q >= 2), |d_1| <= 127,
+// so v = 2^(e - P) V up to half a unit of the last digit.
 // Product: for each level l = i + j in 2..S+1 the exact integer
-//     acc_l = sum_{i+j=l} A_i B_j       (int32; |acc_l| <= S K 127^2)
-// comes from v_mfma_i32_32x32x32_i8; pairs with i + j > S + 1 are dropped. The
-// epilogue folds c = sum_l acc_l 2^(-7 l) in FP64 for l ascending from 2 and
-// subtracts 2^(eA_i + eB_j) c from C. Slicing, integer sums, the fold order and
-// the power-of-two scalings are all deterministic, so the result is bit for bit
-// the one of the oracle's ozaki_matmul.
+//     acc_l = sum_{i+j=l} A_i B_j       (int32; |acc_l| <= S K 2^14)
+// comes from v_mfma_i32_32x32x32_i8; pairs with i + j > S + 1 are dropped.
+// The epilogue folds c = sum_l acc_l 2^(2 - 8 l) in FP64 for l ascending from
+// 2 and subtracts 2^(eA_i + eB_j) c from C. Slicing, integer sums, the fold
+// order and the power-of-two scalings are all deterministic, so the result is
+// bit for bit the one of the oracle's ozaki_matmul.
 //
 // Layout of the slices (K-blocked): As[q][k / 32][row][k % 32], one 32-byte
 // record per (row, 32-wide K block), so that the bytes one K step of one block
@@ -37,18 +39,30 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int OZ_BM = 128, OZ_BK = 32;  // block rows, K step (bytes of one record)
 
-// Digits of v (|v| < 2^e) into d[0..S).
+// Exponent of a row / column with maximum magnitude mx: the smallest e with
+// mx 2^(7 - e) <= 127 (OZ_ZERO_EXP for mx = 0: its digits are all zero, so
+// the exponent scales nothing, and it stays below every real one).
+constexpr int OZ_ZERO_EXP = -1000;
+__device__ __forceinline__ int oz_exponent(double mx) {
+  int e = OZ_ZERO_EXP;
+  if (mx > 0.0) {
+    frexp(mx, &e);
+    if (ldexp(mx, 7 - e) > 127.0) ++e;
+  }
+  return e;
+}
+
+// Digits of v (|v| <= 127 2^(e - 7)) into d[0..S): d[0] most significant.
 template <int S>
 __device__ __forceinline__ void make_digits(double v, int e, signed char* d) {
-  double r = ldexp(v, -e);
+  long long V = (long long)rint(ldexp(v, 8 * S - 1 - e));
 #pragma unroll
-  for (int q = 0; q < S; ++q) {
-    const double t = r * 128.0;
-    double dq = rint(t);
-    dq = fmin(fmax(dq, -127.0), 127.0);
-    r = t - dq;
-    d[q] = (signed char)(int)dq;
+  for (int q = S - 1; q > 0; --q) {
+    const signed char dq = (signed char)(V & 0xff);  // V mod 256 in [-128, 127]
+    d[q] = dq;
+    V = (V - dq) >> 8;  // exact: V - dq is a multiple of 256
   }
+  d[0] = (signed char)V;
 }
 
 // Byte offset of digit (row, k) inside one slice of the K-blocked layout with
@@ -77,9 +91,7 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
   red[p][r] = mx;
   __syncthreads();
   if (t < 64) {
-    const double v = fmax(fmax(red[0][t], red[1][t]), fmax(red[2][t], red[3][t]));
-    int e = 0;
-    if (v > 0.0) frexp(v, &e);
+    const int e = oz_exponent(fmax(fmax(red[0][t], red[1][t]), fmax(red[2][t], red[3][t])));
     se[t] = e;
     if (row0 + t < mp) ea[row0 + t] = e;
   }
@@ -130,8 +142,7 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
     for (int k = lane; k < K; k += 64) mx = fmax(mx, fabs(col[k]));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-  int e = 0;
-  if (mx > 0.0) frexp(mx, &e);
+  const int e = oz_exponent(mx);
   if (lane == 0) eb[j] = e;
   const size_t slab = (size_t)np * Kp;
   for (int k0 = lane * 16; k0 < Kp; k0 += 64 * 16) {
@@ -230,10 +241,10 @@ __device__ __forceinline__ void oz_issue(signed char* __restrict__ st, const sig
 
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-// 2^(-7 (l + 2)): weight of level l (0-based) in the fold; exact power of two.
+// 2^(2 - 8 (l + 2)): weight of level l (0-based) in the fold; exact power of two.
 __device__ __forceinline__ constexpr double oz_level_scale(int l) {
   double w = 1.0;
-  for (int i = 0; i < 7 * (l + 2); ++i) w *= 0.5;
+  for (int i = 0; i < 8 * (l + 2) - 2; ++i) w *= 0.5;
   return w;
 }
 
@@ -390,6 +401,590 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemm_kernel(
   store_c(1);
 }
 
+// ---------------------------------------------------------------------------
+// Single-barrier pipelined variant (the default). WM x WN wavefronts, each
+// owning TA x TB MFMA tiles of 32 x 32 (wavefront tile 32 TA x 32 TB,
+// 16 S TA TB accumulator registers per lane, held in the accumulation
+// registers across the loop); block tile (32 TA WM) x (32 TB WN); one
+// workgroup per CU; four LDS stages. Per K step (32 bytes of every row and
+// column of all S slices) the wavefronts synchronise once:
+//   multiply the first slice pair of step k (fragments in registers); wait
+//   for the copies of step k + 1 and for this wavefront's reads of step k's
+//   fragments; s_barrier; refill the stage of step k with step k + 4;
+//   multiply the remaining pairs while reading step k + 1's fragments from
+//   LDS into the registers each pair frees.
+// The pair order (level i ascending, j descending) frees B fragment j after
+// its pair with i = S - 1 - j and A fragment i after its group, so the reads
+// of the next step go out as early as the registers allow and their latency
+// hides under the remaining matrix-core work.
+template <int S, int WM, int WN, int TA, int TB, int NST_, int OCC>
+struct OzPCfg {
+  static constexpr int NT = 64 * WM * WN, BM = 32 * TA * WM, BN = 32 * TB * WN, NST = NST_;
+  static constexpr int ABYTES = S * BM * OZ_BK, BBYTES = S * BN * OZ_BK;
+  static constexpr int STAGE = ABYTES + BBYTES;
+  // copy rounds of an operand with R records per slice (2 R chunks of 16 B)
+  static constexpr int rounds(int R) {
+    return NT >= 2 * R ? (S + NT / (2 * R) - 1) / (NT / (2 * R)) : S * (2 * R / NT);
+  }
+  static constexpr int COPIES = rounds(BM) + rounds(BN);  // per thread and stage
+  static constexpr int EPI_WAVE = (32 * 33 + 32) * 8;  // one 32 x 32 FP64 tile + its exponents
+  static_assert(NST == 2 || NST == 4, "stages");
+  static_assert(OCC * NST * STAGE <= 160 * 1024, "LDS");
+  static_assert(NT % (2 * BM) == 0 || (2 * BM) % NT == 0, "copy split (A)");
+  static_assert(NT % (2 * BN) == 0 || (2 * BN) % NT == 0, "copy split (B)");
+  static_assert(2 * COPIES <= 63, "vmcnt");
+};
+
+// One K step of one operand (R records per slice) into a stage: thread tid
+// copies 16-byte chunks; in LDS a slice is [k half][record][16 bytes] and
+// the chunk at position c is half c / R of record c % R, so each wavefront's
+// 64 copies land in 1 KB of consecutive LDS (global_load_lds writes lane l at
+// 16 l) and the fragment reads cover contiguous 512-byte runs per half
+// wavefront (no bank conflicts). src: the tile's first record of slice 0.
+// src and slab are workgroup-uniform and the per-lane part is a 32-bit
+// offset, so the copies use a scalar base address.
+template <int S, int NT, int R>
+__device__ __forceinline__ void ozp_issue_op(signed char* __restrict__ dst, const signed char* src,
+                                             size_t slab, int tid, int wave) {
+  if constexpr (NT >= 2 * R) {
+    constexpr int SPR = NT / (2 * R), RND = (S + SPR - 1) / SPR;
+    const int sub = wave / (2 * R / 64);  // slice within the round (wavefront-uniform)
+    const int c = tid % (2 * R);
+    const unsigned so = (c % R) * OZ_BK + (c / R) * 16;
+    const int lo = ((64 * wave) % (2 * R)) * 16;
+#pragma unroll
+    for (int rd = 0; rd < RND; ++rd) {
+      // a round past the last slice (S not a multiple of SPR) copies slice
+      // S - 1 once more into its own slot: the same bytes twice
+      const int q = rd * SPR + sub < S ? rd * SPR + sub : S - 1;
+      __builtin_amdgcn_global_load_lds(src + q * slab + so, dst + q * R * OZ_BK + lo, 16, 0, 0);
+    }
+  } else {
+    constexpr int RPS = 2 * R / NT;  // rounds per slice
+#pragma unroll
+    for (int rd = 0; rd < S * RPS; ++rd) {
+      const int q = rd / RPS;
+      const unsigned c = (rd % RPS) * NT + tid;
+      __builtin_amdgcn_global_load_lds(src + q * slab + ((c % R) * OZ_BK + (c / R) * 16),
+                                       dst + q * R * OZ_BK + ((rd % RPS) * NT + 64 * wave) * 16,
+                                       16, 0, 0);
+    }
+  }
+}
+
+template <class Cfg, int S>
+__device__ __forceinline__ void ozp_issue(signed char* __restrict__ st, const signed char* a,
+                                          const signed char* b, size_t slabA, size_t slabB,
+                                          int tid) {
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  ozp_issue_op<S, Cfg::NT, Cfg::BM>(st, a, slabA, tid, wave);
+  ozp_issue_op<S, Cfg::NT, Cfg::BN>(st + Cfg::ABYTES, b, slabB, tid, wave);
+}
+
+// One K step: st_cur's stage is refilled with step k + NST (tile pointers a, b
+// at that step); the fragments of step k are in (fa, fb); those of step k + 1
+// are read from st_nxt. sched_barrier keeps the compiler from moving the
+// next step's reads back next to their uses or the barrier ahead of the
+// first pair.
+template <class Cfg, int S, int TA, int TB>
+__device__ __forceinline__ void ozp_step(signed char* __restrict__ st_cur,
+                                         const signed char* __restrict__ st_nxt,
+                                         v16i (&acc)[S][TA][TB], v4i (&fa)[S][TA],
+                                         v4i (&fb)[S][TB], const signed char* a,
+                                         const signed char* b, size_t slabA, size_t slabB,
+                                         int offA, int offB, int tid) {
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+#pragma unroll
+    for (int j = S - 1 - i; j >= 0; --j) {
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+          acc[i + j][ta][tb] =
+              __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][ta], fb[j][tb], acc[i + j][ta][tb], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (i == 0 && j == S - 1) {
+        // copies of step k + 1 landed (those of the later steps stay in
+        // flight), and this wavefront's reads of step k's fragments are done;
+        // after the barrier every wavefront's are, so st_cur is free
+        __builtin_amdgcn_s_waitcnt(oz_vmcnt((Cfg::NST - 2) * Cfg::COPIES) & OZ_LGKM0);
+        __builtin_amdgcn_s_barrier();
+        ozp_issue<Cfg, S>(st_cur, a, b, slabA, slabB, tid);
+      }
+      if (j == S - 1 - i) {  // last use of B fragment j
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+          fb[j][tb] = *reinterpret_cast<const v4i*>(st_nxt + Cfg::ABYTES + j * Cfg::BN * OZ_BK +
+                                                    offB + tb * 32 * 16);
+      }
+      if (j == 0) {  // last use of A fragment i
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta)
+          fa[i][ta] = *reinterpret_cast<const v4i*>(st_nxt + i * Cfg::BM * OZ_BK + offA +
+                                                    ta * 32 * 16);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// Kp: a multiple of 4 K steps (128 bytes). The stage arrays (SB bytes each,
+// at least Cfg::STAGE; one array per stage, so the compiler tells the stages
+// apart and waits only for the copies into the stage being read) belong to
+// the calling kernel, which may size them for more slices than S.
+template <int S, int WM, int WN, int TA, int TB, int NST, int OCC, int SB>
+__device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, int col0, int ncols,
+                                         const signed char* __restrict__ As,
+                                         const signed char* __restrict__ Bs,
+                                         const int* __restrict__ ea, const int* __restrict__ eb,
+                                         double* __restrict__ C, int64_t ldc, int group,
+                                         signed char* __restrict__ st0,
+                                         signed char* __restrict__ st1,
+                                         signed char* __restrict__ st2,
+                                         signed char* __restrict__ st3) {
+  typedef OzPCfg<S, WM, WN, TA, TB, NST, OCC> Cfg;
+  static_assert(SB >= Cfg::STAGE, "stage arrays");
+  constexpr int WPB = SB / Cfg::EPI_WAVE;  // wavefronts staged per stage array in the epilogue
+  static_assert(WPB >= 1 && WM * WN <= NST * WPB, "epilogue staging");
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave % WM) * 32 * TA, wn = (wave / WM) * 32 * TB;
+  int ti, tj;
+  if (!tile_of(blockIdx.x, mp / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN, ti, tj, group)) return;
+  const int bm = ti * Cfg::BM, bn = col0 + tj * Cfg::BN;
+  const size_t slabA = (size_t)mp * Kp, slabB = (size_t)np * Kp;
+  const size_t kstepA = (size_t)mp * OZ_BK, kstepB = (size_t)np * OZ_BK;
+  const signed char* Ab = As + (size_t)bm * OZ_BK;
+  const signed char* Bb = Bs + (size_t)bn * OZ_BK;
+  // fragment reads: lane takes record r32 of its 32-tile, k half hf
+  const int r32 = lane & 31, hf = lane >> 5;
+  const int offA = hf * Cfg::BM * 16 + (wm + r32) * 16;
+  const int offB = hf * Cfg::BN * 16 + (wn + r32) * 16;
+
+  v16i acc[S][TA][TB];
+#pragma unroll
+  for (int l = 0; l < S; ++l)
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb) acc[l][ta][tb] = v16i(0);
+
+  const int nk = Kp / OZ_BK;  // a multiple of 4
+  ozp_issue<Cfg, S>(st0, Ab, Bb, slabA, slabB, tid);
+  ozp_issue<Cfg, S>(st1, Ab + kstepA, Bb + kstepB, slabA, slabB, tid);
+  if constexpr (NST > 2) {
+    ozp_issue<Cfg, S>(st2, Ab + 2 * kstepA, Bb + 2 * kstepB, slabA, slabB, tid);
+    ozp_issue<Cfg, S>(st3, Ab + 3 * kstepA, Bb + 3 * kstepB, slabA, slabB, tid);
+  }
+  __builtin_amdgcn_s_waitcnt(oz_vmcnt((NST - 1) * Cfg::COPIES));
+  __builtin_amdgcn_s_barrier();
+  v4i fa[S][TA], fb[S][TB];
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta)
+      fa[q][ta] = *reinterpret_cast<const v4i*>(st0 + q * Cfg::BM * OZ_BK + offA + ta * 32 * 16);
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb)
+      fb[q][tb] = *reinterpret_cast<const v4i*>(st0 + Cfg::ABYTES + q * Cfg::BN * OZ_BK + offB +
+                                                tb * 32 * 16);
+  }
+  // past the last K step the refills repeat the last step (never read) and
+  // the last step's next-fragment reads fetch a stage that is never used, so
+  // the loop body is the same every time
+  for (int t = 0; t < nk; t += NST) {
+    int k = min(t + NST, nk - 1);
+    ozp_step<Cfg, S, TA, TB>(st0, st1, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA, slabB,
+                             offA, offB, tid);
+    k = min(t + NST + 1, nk - 1);
+    ozp_step<Cfg, S, TA, TB>(st1, NST > 2 ? st2 : st0, acc, fa, fb, Ab + k * kstepA,
+                             Bb + k * kstepB, slabA, slabB, offA, offB, tid);
+    if constexpr (NST > 2) {
+      k = min(t + NST + 2, nk - 1);
+      ozp_step<Cfg, S, TA, TB>(st2, st3, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA,
+                               slabB, offA, offB, tid);
+      k = min(t + NST + 3, nk - 1);
+      ozp_step<Cfg, S, TA, TB>(st3, st0, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA,
+                               slabB, offA, offB, tid);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(oz_vmcnt(0) & OZ_LGKM0);
+  __builtin_amdgcn_s_barrier();
+
+  // Epilogue, one 32 x 32 MFMA tile (ta, tb) at a time: the next tile's C
+  // values are loaded before this tile's stores; the levels are folded (FP64,
+  // l ascending) in the MFMA layout, staged through LDS unscaled with the
+  // tile's column exponents, and after the transpose lane (row r, columns c)
+  // scales by 2^(ea_i + eb_j) and stores C -= value down 32 consecutive rows.
+  const int wb = wave / WPB, ws = wave % WPB;
+  signed char* ebuf = wb == 0 ? st0 : wb == 1 ? st1 : wb == 2 ? st2 : st3;
+  double* epi = reinterpret_cast<double*>(ebuf) + ws * (32 * 33 + 32);
+  int* ecol = reinterpret_cast<int*>(epi + 32 * 33);
+  const int rl = lane & 31, cph = lane >> 5;
+  constexpr int NTILE = TA * TB;
+  double cv[2][16];
+  auto load_c = [&](int t, double(&dst)[16]) {
+    const int ta = t / TB, tb = t % TB;
+    const int i = bm + wm + 32 * ta + rl;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int j = bn + wn + 32 * tb + cph + 2 * c;
+      dst[c] = (i < m && j < col0 + ncols) ? C[i + (int64_t)j * ldc] : 0.0;
+    }
+  };
+  auto lds_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  load_c(0, cv[0]);
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) {
+    const int ta = t / TB, tb = t % TB;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int il = 8 * (r >> 2) + 4 * cph + (r & 3);
+      double c = 0.0;
+#pragma unroll
+      for (int l = 0; l < S; ++l) c += (double)acc[l][ta][tb][r] * oz_level_scale(l);
+      epi[rl * 33 + il] = c;
+    }
+    if (cph == 0) ecol[rl] = eb[bn + wn + 32 * tb + rl];  // < np
+    if (t + 1 < NTILE) load_c(t + 1, cv[(t + 1) & 1]);
+    lds_sync();
+    const int i = bm + wm + 32 * ta + rl;
+    const int ei = ea[i];  // i < mp
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int jc = cph + 2 * c, j = bn + wn + 32 * tb + jc;
+      const double v = ldexp(epi[jc * 33 + rl], ei + ecol[jc]);
+      if (i < m && j < col0 + ncols) C[i + (int64_t)j * ldc] = cv[t & 1][c] - v;
+    }
+    lds_sync();
+  }
+}
+
+template <int S, int WM, int WN, int TA, int TB, int NST, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmp_kernel(
+    int m, int n, int mp, int np, int Kp, int col0, int ncols, const signed char* __restrict__ As,
+    const signed char* __restrict__ Bs, const int* __restrict__ ea, const int* __restrict__ eb,
+    double* __restrict__ C, int64_t ldc, int group) {
+  constexpr int SB = OzPCfg<S, WM, WN, TA, TB, NST, OCC>::STAGE;
+  __shared__ __attribute__((aligned(16))) signed char st0[SB];
+  __shared__ __attribute__((aligned(16))) signed char st1[SB];
+  __shared__ __attribute__((aligned(16))) signed char st2[NST > 2 ? SB : 16];
+  __shared__ __attribute__((aligned(16))) signed char st3[NST > 2 ? SB : 16];
+  ozp_body<S, WM, WN, TA, TB, NST, OCC, SB>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C, ldc,
+                                            group, st0, st1, st2, st3);
+}
+
+// The same with the slice count read on the device (*Sdev, 2 .. SMAX): the
+// slices are those of an SMAX-slice split (only the leading *Sdev digits
+// are used), so one launch serves whatever count the adaptive rule picked.
+template <int SMAX, int WM, int WN, int TA, int TB, int NST, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmd_kernel(
+    int m, int n, int mp, int np, int Kp, int col0, int ncols, const signed char* __restrict__ As,
+    const signed char* __restrict__ Bs, const int* __restrict__ ea, const int* __restrict__ eb,
+    double* __restrict__ C, int64_t ldc, int group, const int* __restrict__ Sdev) {
+  constexpr int SB = OzPCfg<SMAX, WM, WN, TA, TB, NST, OCC>::STAGE;
+  __shared__ __attribute__((aligned(16))) signed char st0[SB];
+  __shared__ __attribute__((aligned(16))) signed char st1[SB];
+  __shared__ __attribute__((aligned(16))) signed char st2[NST > 2 ? SB : 16];
+  __shared__ __attribute__((aligned(16))) signed char st3[NST > 2 ? SB : 16];
+  const int S = __builtin_amdgcn_readfirstlane(*Sdev);
+  if (S <= 2)
+    ozp_body<2, WM, WN, TA, TB, NST, OCC, SB>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
+                                              ldc, group, st0, st1, st2, st3);
+  else if (SMAX > 3 && S == 3)
+    ozp_body<3, WM, WN, TA, TB, NST, OCC, SB>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
+                                              ldc, group, st0, st1, st2, st3);
+  else if (SMAX > 4 && S == 4)
+    ozp_body<(SMAX > 4 ? 4 : SMAX), WM, WN, TA, TB, NST, OCC, SB>(
+        m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C, ldc, group, st0, st1, st2, st3);
+  else
+    ozp_body<SMAX, WM, WN, TA, TB, NST, OCC, SB>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
+                                                 ldc, group, st0, st1, st2, st3);
+}
+
+// Slice count of one trailing update from the exponent maxima of its panels:
+// E = max_i eA_i + max_j eB_j bounds the magnitude of every product term, and
+// the error of an S-slice product is ~2^(E - (8 S - 1)). The first update of
+// a factorisation (first != 0) records E0 and takes SMAX slices; a later one
+// takes the fewest slices S in [smin, SMAX] whose error bound stays within
+// the first update's minus `delta` bits:
+//     E - (8 S - 1) <= E0 - (8 SMAX - 1) - delta,
+// i.e. S = SMAX + ceil((E - E0 + delta) / 8), clamped. Rows and columns that
+// are all zero carry the exponent OZ_ZERO_EXP and never decide the maximum
+// unless everything is zero.
+__global__ __launch_bounds__(256) void oz_decide_kernel(const int* __restrict__ ea, int m,
+                                                        const int* __restrict__ eb, int n,
+                                                        int smax, int smin, int delta, int first,
+                                                        int* __restrict__ e0,
+                                                        int* __restrict__ s_out) {
+  __shared__ int red[2][256];
+  int a = OZ_ZERO_EXP, b = OZ_ZERO_EXP;
+  for (int i = threadIdx.x; i < m; i += 256) a = max(a, ea[i]);
+  for (int j = threadIdx.x; j < n; j += 256) b = max(b, eb[j]);
+  red[0][threadIdx.x] = a;
+  red[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] = max(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+      red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int E = red[0][0] + red[1][0];
+    int S = smax;
+    if (first) {
+      *e0 = E;
+    } else {
+      const int d = E - *e0 + delta;
+      S = smax + (d >= 0 ? (d + 7) / 8 : -((-d) / 8));
+      S = S < smin ? smin : (S > smax ? smax : S);
+    }
+    *s_out = S;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent variant (the default for S <= 5): one workgroup per CU walks a
+// strided sequence of block tiles (tile p, p + G, ... of the XCD-aware order,
+// G = grid size), and its stage pipeline runs on across tile boundaries: the
+// copies of the next tile's first steps are in flight while the current tile
+// is written back, so a tile pays no pipeline fill. The products are taken
+// with the operands swapped (U-panel slices as the MFMA's first operand),
+// which puts C's rows along the lanes: for accumulator register r, lanes 0-31
+// hold 32 consecutive rows of one column and lanes 32-63 of another, so the
+// FP64 read-modify-write of C is coalesced straight from the accumulators
+// (no LDS transpose) and the epilogue leaves LDS to the pipeline.
+//   lane l, register r of MFMA tile (ta, tb): row bm + wm + 32 ta + l % 32,
+//   column bn + wn + 32 tb + 8 (r / 4) + 4 (l / 32) + r % 4.
+// READ_NEXT = false for a tile's last step: the next tile's first fragments
+// are read after the write-back, so they hold no registers during it.
+template <class Cfg, int S, int TA, int TB, bool READ_NEXT = true>
+__device__ __forceinline__ void ozq_step(signed char* __restrict__ st_cur,
+                                         const signed char* __restrict__ st_nxt,
+                                         v16i (&acc)[S][TA][TB], v4i (&fa)[S][TA],
+                                         v4i (&fb)[S][TB], const signed char* a,
+                                         const signed char* b, size_t slabA, size_t slabB,
+                                         int offA, int offB, int tid) {
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+#pragma unroll
+    for (int j = S - 1 - i; j >= 0; --j) {
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+          acc[i + j][ta][tb] =
+              __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[j][tb], fa[i][ta], acc[i + j][ta][tb], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (i == 0 && j == S - 1) {
+        __builtin_amdgcn_s_waitcnt(oz_vmcnt((Cfg::NST - 2) * Cfg::COPIES) & OZ_LGKM0);
+        __builtin_amdgcn_s_barrier();
+        ozp_issue<Cfg, S>(st_cur, a, b, slabA, slabB, tid);
+      }
+      if (!READ_NEXT) {
+        __builtin_amdgcn_sched_barrier(0);
+        continue;
+      }
+      if (j == S - 1 - i) {
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+          fb[j][tb] = *reinterpret_cast<const v4i*>(st_nxt + Cfg::ABYTES + j * Cfg::BN * OZ_BK +
+                                                    offB + tb * 32 * 16);
+      }
+      if (j == 0) {
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta)
+          fa[i][ta] = *reinterpret_cast<const v4i*>(st_nxt + i * Cfg::BM * OZ_BK + offA +
+                                                    ta * 32 * 16);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// Kp: a multiple of 4 K steps (128 bytes). Launch with at most tm * tn
+// workgroups (tiles_launch rounding is not needed: every workgroup id below
+// the grid owns at least one tile when grid <= tiles).
+//
+// Every workgroup's tiles take the same time, so workgroups started together
+// stay in step: all multiply, then all write back at once (matrix cores idle
+// while HBM saturates). `stagger` (units of 8128 cycles of s_sleep) delays
+// workgroup b by ((b / 8) mod 4) units before its first copies, spreading the
+// write-backs over a tile's duration.
+template <int S, int WM, int WN, int TA, int TB, int NST, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmq_kernel(
+    int m, int n, int mp, int np, int Kp, int col0, int ncols, const signed char* __restrict__ As,
+    const signed char* __restrict__ Bs, const int* __restrict__ ea, const int* __restrict__ eb,
+    double* __restrict__ C, int64_t ldc, int group, int stagger) {
+  typedef OzPCfg<S, WM, WN, TA, TB, NST, OCC> Cfg;
+  __shared__ __attribute__((aligned(16))) signed char st0[Cfg::STAGE];
+  __shared__ __attribute__((aligned(16))) signed char st1[Cfg::STAGE];
+  __shared__ __attribute__((aligned(16))) signed char st2[NST > 2 ? Cfg::STAGE : 16];
+  __shared__ __attribute__((aligned(16))) signed char st3[NST > 2 ? Cfg::STAGE : 16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave % WM) * 32 * TA, wn = (wave / WM) * 32 * TB;
+  const int tm = mp / Cfg::BM, tn = (ncols + Cfg::BN - 1) / Cfg::BN;
+  const int G = gridDim.x;
+  // tiles of this workgroup: p = blockIdx.x + s G for s = 0 .. nmine - 1 (ids
+  // past the XCD-aware order's surplus are skipped by tile_of)
+  const int T = (tm * tn + NXCD - 1) / NXCD * NXCD;  // ids of the XCD-aware order
+  const size_t slabA = (size_t)mp * Kp, slabB = (size_t)np * Kp;
+  const size_t kstepA = (size_t)mp * OZ_BK, kstepB = (size_t)np * OZ_BK;
+  const int r32 = lane & 31, hf = lane >> 5;
+  const int offA = hf * Cfg::BM * 16 + (wm + r32) * 16;
+  const int offB = hf * Cfg::BN * 16 + (wn + r32) * 16;
+  const int nk = Kp / OZ_BK;  // a multiple of 4
+
+  // first tile with work
+  int p = blockIdx.x, ti = 0, tj = 0;
+  while (p < T && !tile_of(p, tm, tn, ti, tj, group)) p += G;
+  if (p >= T) return;
+  // next tile with work (or none)
+  auto next_tile = [&](int from, int& nti, int& ntj) {
+    int q = from + G;
+    while (q < T && !tile_of(q, tm, tn, nti, ntj, group)) q += G;
+    return q;
+  };
+  int nti = ti, ntj = tj;
+  int pn = next_tile(p, nti, ntj);
+  const signed char* Ab = As + (size_t)(ti * Cfg::BM) * OZ_BK;
+  const signed char* Bb = Bs + (size_t)(col0 + tj * Cfg::BN) * OZ_BK;
+  // source of the copies past the tile's last step: the next tile's first
+  // steps, or (last tile) the last step again (never read)
+  const signed char* An = pn < T ? As + (size_t)(nti * Cfg::BM) * OZ_BK : Ab + (nk - 1) * kstepA;
+  const signed char* Bn = pn < T ? Bs + (size_t)(col0 + ntj * Cfg::BN) * OZ_BK : Bb + (nk - 1) * kstepB;
+  size_t nstepA = pn < T ? kstepA : 0, nstepB = pn < T ? kstepB : 0;
+
+  v16i acc[S][TA][TB];
+#pragma unroll
+  for (int l = 0; l < S; ++l)
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb) acc[l][ta][tb] = v16i(0);
+
+  for (int z = ((blockIdx.x >> 3) & 3) * stagger; z > 0; --z) __builtin_amdgcn_s_sleep(127);
+  ozp_issue<Cfg, S>(st0, Ab, Bb, slabA, slabB, tid);
+  ozp_issue<Cfg, S>(st1, Ab + kstepA, Bb + kstepB, slabA, slabB, tid);
+  if constexpr (NST > 2) {
+    ozp_issue<Cfg, S>(st2, Ab + 2 * kstepA, Bb + 2 * kstepB, slabA, slabB, tid);
+    ozp_issue<Cfg, S>(st3, Ab + 3 * kstepA, Bb + 3 * kstepB, slabA, slabB, tid);
+  }
+  __builtin_amdgcn_s_waitcnt(oz_vmcnt((NST - 1) * Cfg::COPIES));
+  __builtin_amdgcn_s_barrier();
+  v4i fa[S][TA], fb[S][TB];
+  const int rl = lane & 31, u = lane >> 5;
+  for (;;) {
+    // first fragments of the tile (stage 0 holds its step 0: landed and
+    // visible since the last step's wait and barrier, or the prologue's)
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta)
+        fa[q][ta] = *reinterpret_cast<const v4i*>(st0 + q * Cfg::BM * OZ_BK + offA + ta * 32 * 16);
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb)
+        fb[q][tb] = *reinterpret_cast<const v4i*>(st0 + Cfg::ABYTES + q * Cfg::BN * OZ_BK + offB +
+                                                  tb * 32 * 16);
+    }
+    // source of the copy of step k (k may run past the tile into the next one)
+    auto srcA = [&](int k) { return k < nk ? Ab + k * kstepA : An + (k - nk) * nstepA; };
+    auto srcB = [&](int k) { return k < nk ? Bb + k * kstepB : Bn + (k - nk) * nstepB; };
+    int t = 0;
+    for (; t + NST < nk; t += NST) {
+      ozq_step<Cfg, S, TA, TB>(st0, st1, acc, fa, fb, srcA(t + NST), srcB(t + NST), slabA, slabB,
+                               offA, offB, tid);
+      ozq_step<Cfg, S, TA, TB>(st1, NST > 2 ? st2 : st0, acc, fa, fb, srcA(t + NST + 1),
+                               srcB(t + NST + 1), slabA, slabB, offA, offB, tid);
+      if constexpr (NST > 2) {
+        ozq_step<Cfg, S, TA, TB>(st2, st3, acc, fa, fb, srcA(t + NST + 2), srcB(t + NST + 2),
+                                 slabA, slabB, offA, offB, tid);
+        ozq_step<Cfg, S, TA, TB>(st3, st0, acc, fa, fb, srcA(t + NST + 3), srcB(t + NST + 3),
+                                 slabA, slabB, offA, offB, tid);
+      }
+    }
+    // the tile's last NST steps; the very last reads no next fragments
+    if constexpr (NST > 2) {
+      ozq_step<Cfg, S, TA, TB>(st0, st1, acc, fa, fb, srcA(t + NST), srcB(t + NST), slabA, slabB,
+                               offA, offB, tid);
+      ozq_step<Cfg, S, TA, TB>(st1, st2, acc, fa, fb, srcA(t + NST + 1), srcB(t + NST + 1), slabA,
+                               slabB, offA, offB, tid);
+      ozq_step<Cfg, S, TA, TB>(st2, st3, acc, fa, fb, srcA(t + NST + 2), srcB(t + NST + 2), slabA,
+                               slabB, offA, offB, tid);
+      ozq_step<Cfg, S, TA, TB, false>(st3, st0, acc, fa, fb, srcA(t + NST + 3), srcB(t + NST + 3),
+                                      slabA, slabB, offA, offB, tid);
+    } else {
+      ozq_step<Cfg, S, TA, TB>(st0, st1, acc, fa, fb, srcA(t + NST), srcB(t + NST), slabA, slabB,
+                               offA, offB, tid);
+      ozq_step<Cfg, S, TA, TB, false>(st1, st0, acc, fa, fb, srcA(t + NST + 1), srcB(t + NST + 1),
+                                      slabA, slabB, offA, offB, tid);
+    }
+    // write back this tile while the next tile's first copies are in flight;
+    // interior tiles (the uniform common case) skip the bounds checks
+    const int bm = ti * Cfg::BM, bn = col0 + tj * Cfg::BN;
+    const bool inside = bm + Cfg::BM <= m && bn + Cfg::BN <= col0 + ncols;
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta) {
+      const int i = bm + wm + 32 * ta + rl;
+      const int ei = ea[i];  // i < mp
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb) {
+        const int jb = bn + wn + 32 * tb + 4 * u;
+        double* Ct = C + i + (int64_t)jb * ldc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // registers 4 q .. 4 q + 3: columns jb + 8 q + 0..3
+          double cv[4];
+          int ej[4];
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int j = jb + 8 * q + rr;
+            ej[rr] = eb[j];  // j < np
+            // C streams through once per update: non-temporal accesses keep
+            // it from evicting the slices the other tiles still read from L2
+            if (inside)
+              cv[rr] = __builtin_nontemporal_load(Ct + (int64_t)(8 * q + rr) * ldc);
+            else
+              cv[rr] = (i < m && j < col0 + ncols)
+                           ? __builtin_nontemporal_load(Ct + (int64_t)(8 * q + rr) * ldc)
+                           : 0.0;
+          }
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int j = jb + 8 * q + rr;
+            double c = 0.0;
+#pragma unroll
+            for (int l = 0; l < S; ++l) c += (double)acc[l][ta][tb][4 * q + rr] * oz_level_scale(l);
+            const double v = cv[rr] - ldexp(c, ei + ej[rr]);
+            if (inside || (i < m && j < col0 + ncols))
+              __builtin_nontemporal_store(v, Ct + (int64_t)(8 * q + rr) * ldc);
+          }
+        }
+#pragma unroll
+        for (int l = 0; l < S; ++l) acc[l][ta][tb] = v16i(0);
+      }
+    }
+    if (pn >= T) break;
+    p = pn;
+    ti = nti;
+    tj = ntj;
+    Ab = An;
+    Bb = Bn;
+    pn = next_tile(p, nti, ntj);
+    An = pn < T ? As + (size_t)(nti * Cfg::BM) * OZ_BK : Ab + (nk - 1) * kstepA;
+    Bn = pn < T ? Bs + (size_t)(col0 + ntj * Cfg::BN) * OZ_BK : Bb + (nk - 1) * kstepB;
+    nstepA = pn < T ? kstepA : 0;
+    nstepB = pn < T ? kstepB : 0;
+  }
+  // drain the copies past the last tile (never read) before the workgroup ends
+  __builtin_amdgcn_s_waitcnt(oz_vmcnt(0) & OZ_LGKM0);
+}
+
 // Slice arrays: rows padded to a multiple of 128; columns to a multiple of 128
 // plus 128 more, so that a column range starting at any multiple of 64 is
 // covered by whole tiles of up to 128 columns.
@@ -446,29 +1041,144 @@ static int ozaki_launch(hipStream_t s, const OzLayout& L, int m, int n, int col0
   }
 }
 
+template <int S, int WM, int WN, int TA, int TB, int NST, int OCC>
+static int ozaki_launchp(hipStream_t s, const OzLayout& L, int m, int n, int col0, int ncols,
+                         double* C, int64_t ldc) {
+  typedef OzPCfg<S, WM, WN, TA, TB, NST, OCC> Cfg;
+  const int grid = tiles_launch(L.mp / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN);
+  const int g = tuning_get(TUNE_OZAKI_GROUP);
+  ozaki_gemmp_kernel<S, WM, WN, TA, TB, NST, OCC><<<grid, Cfg::NT, 0, s>>>(
+      m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, g > 0 ? g : GROUP_M);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+static int oz_cus() {
+  static int ncu = 0;
+  if (ncu == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu <= 0)
+      ncu = 256;
+  }
+  return ncu;
+}
+
+template <int S, int WM, int WN, int TA, int TB, int NST, int OCC>
+static int ozaki_launchq(hipStream_t s, const OzLayout& L, int m, int n, int col0, int ncols,
+                         double* C, int64_t ldc) {
+  typedef OzPCfg<S, WM, WN, TA, TB, NST, OCC> Cfg;
+  const int tm = L.mp / Cfg::BM, tn = (ncols + Cfg::BN - 1) / Cfg::BN;
+  const int tiles = tm * tn;
+  // a multiple of the XCD count, so each workgroup stays on one XCD's tiles
+  int grid = oz_cus() * OCC;
+  if (grid > tiles_launch(tm, tn)) grid = tiles_launch(tm, tn);
+  const int g = tuning_get(TUNE_OZAKI_GROUP);
+  // stagger only when every workgroup has several tiles to amortise it over
+  const int stagger = tiles >= 4 * grid ? tuning_get(TUNE_OZAKI_STAGGER) : 0;
+  ozaki_gemmq_kernel<S, WM, WN, TA, TB, NST, OCC><<<grid, Cfg::NT, 0, s>>>(
+      m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, g > 0 ? g : GROUP_M,
+      stagger);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
 template <int S>
 static int ozaki_apply_s(hipStream_t s, int m, int n, int K, int col0, int ncols, double* C,
                          int64_t ldc, void* work) {
   const OzLayout L = oz_layout(S, m, n, K, work);
-  // shape 0 (default): 128 x 64 tiles, two stages, two workgroups per CU, which
-  // overlap each other's epilogue (the FP64 read-modify-write of C) and copies;
-  // measured fastest at S = 5, 6 on 32256^2 x 512 (1.48 / 1.77 int8 POP/s).
-  // S = 7 needs too many registers for two workgroups per CU: one.
+  // shape 0 (default): the single-barrier pipelined kernel with 128 x 64
+  // block tiles (4 wavefronts of 64 x 32), two stages, two workgroups per CU,
+  // so that one workgroup's epilogue (the FP64 read-modify-write of C)
+  // overlaps the other's matrix-core work; S = 6, 7 (too many accumulators
+  // for it) use the two-barrier kernel with two workgroups per CU (S = 6) or
+  // one (S = 7). Shapes 1-5 select the two-barrier kernel, 6-9 other
+  // pipelined shapes, for tuning and tests.
   switch (tuning_get(TUNE_OZAKI_SHAPE)) {
     case 1: return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 1 per CU
     case 2: return ozaki_launch<S, 2, 4, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 128 x 128
     case 3: return ozaki_launch<S, 2, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);  // four stages
     case 4: return ozaki_launch<S, 2, 4, 4, 1>(s, L, m, n, col0, ncols, C, ldc);  // 128 x 128, four stages
-    default:
+    case 5:
       if constexpr (S <= 6) return ozaki_launch<S, 2, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 6:  // pipelined, 8 wavefronts of 64 x 32, four stages, one workgroup per CU
+      if constexpr (S <= 5) return ozaki_launchp<S, 2, 4, 2, 1, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 7:  // pipelined, 8 wavefronts of 32 x 64, four stages, one workgroup per CU
+      if constexpr (S <= 5) return ozaki_launchp<S, 4, 2, 1, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 8:  // pipelined, 4 wavefronts of 64 x 64 (S <= 4), four stages, one workgroup per CU
+      if constexpr (S <= 4) return ozaki_launchp<S, 2, 2, 2, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 9:  // pipelined, 4 wavefronts of 32 x 64, two stages, two workgroups per CU
+      if constexpr (S <= 5) return ozaki_launchp<S, 2, 2, 1, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 10:  // persistent, 8 wavefronts of 64 x 32, four stages
+      if constexpr (S <= 5) return ozaki_launchq<S, 2, 4, 2, 1, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 11:  // persistent, 4 wavefronts of 64 x 32, two stages, two workgroups per CU
+      if constexpr (S <= 5) return ozaki_launchq<S, 2, 2, 2, 1, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 12:  // persistent, 4 wavefronts of 64 x 64 (S <= 4), four stages
+      if constexpr (S <= 4) return ozaki_launchq<S, 2, 2, 2, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    default:
+      if constexpr (S <= 5) return ozaki_launchp<S, 2, 2, 2, 1, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
+      else if constexpr (S == 6) return ozaki_launch<S, 2, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
       else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 7 S accumulators
   }
+}
+
+// C[:, col0 : col0 + ncols] -= the product of the leading *Sdev slices of the
+// last ozaki_slice call with smax slices (smax <= 5; *Sdev read on the device).
+int ozaki_apply_dyn(hipStream_t s, int smax, int m, int n, int K, int col0, int ncols, double* C,
+                    int64_t ldc, void* work, const int* Sdev) {
+  if (m <= 0 || ncols <= 0 || K <= 0) return OK;
+  MPDOK_REQUIRE(col0 % 64 == 0 && col0 + ncols <= n, "ozaki_apply_dyn: bad column range");
+  MPDOK_REQUIRE(smax >= 3 && smax <= 5, "ozaki_apply_dyn: smax must be in [3, 5], got %d", smax);
+  const OzLayout L = oz_layout(smax, m, n, K, work);
+  const int g = tuning_get(TUNE_OZAKI_GROUP);
+  const int group = g > 0 ? g : GROUP_M;
+  // 128 x 64 tiles, 4 wavefronts of 64 x 32, two stages, two workgroups per CU
+  const int grid = tiles_launch(L.mp / 128, (ncols + 63) / 64);
+  switch (smax) {
+    case 3:
+      ozaki_gemmd_kernel<3, 2, 2, 2, 1, 2, 2><<<grid, 256, 0, s>>>(
+          m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, group, Sdev);
+      break;
+    case 4:
+      ozaki_gemmd_kernel<4, 2, 2, 2, 1, 2, 2><<<grid, 256, 0, s>>>(
+          m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, group, Sdev);
+      break;
+    default:
+      ozaki_gemmd_kernel<5, 2, 2, 2, 1, 2, 2><<<grid, 256, 0, s>>>(
+          m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, group, Sdev);
+  }
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+// Slice count for the product of the last ozaki_slice call (smax slices,
+// m rows, n columns) into *s_out; first != 0 records the reference exponent
+// sum in *e0 (see oz_decide_kernel).
+int ozaki_decide(hipStream_t s, int smax, int m, int n, int K, void* work, int smin, int delta,
+                 int first, int* e0, int* s_out) {
+  const OzLayout L = oz_layout(smax, m, n, K, work);
+  MPDOK_REQUIRE(smin >= 2 && smin <= smax, "ozaki_decide: smin=%d outside [2, %d]", smin, smax);
+  oz_decide_kernel<<<1, 256, 0, s>>>(L.ea, m, L.eb, n, smax, smin, delta, first, e0, s_out);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
 }
 
 // Slice A (m x K) by rows and B (K x n) by columns into the workspace.
 int ozaki_slice(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
                 const double* B, int64_t ldb, void* work) {
   if (m <= 0 || n <= 0 || K <= 0) return OK;
+  // int32 accumulators: a level sums at most S products of K terms of |d d'| <= 2^14
+  MPDOK_REQUIRE((int64_t)S * K * 16384 <= INT32_MAX, "ozaki_slice: K=%d too long for %d slices", K,
+                S);
   switch (S) {
     case 3: return ozaki_slice_s<3>(s, m, n, K, A, lda, B, ldb, work);
     case 4: return ozaki_slice_s<4>(s, m, n, K, A, lda, B, ldb, work);
@@ -512,4 +1222,34 @@ MPDOK_API int mpdok_ozaki_gemm_sub(hipStream_t s, int S, int m, int n, int K, co
                                    int64_t lda, const double* B, int64_t ldb, double* C,
                                    int64_t ldc, void* work) {
   return mpdok::ozaki_gemm(s, S, m, n, K, A, lda, B, ldb, C, ldc, work);
+}
+
+// The adaptive pieces, for parity tests: slice with smax slices, then
+// C -= the product of the leading S slices, with S given (s_fixed >= 2) or
+// picked by the decision rule against a reference exponent sum e0_in
+// (s_fixed = 0; e0_in = INT_MIN records this call as the first update).
+// *s_used receives the count taken.
+MPDOK_API int mpdok_ozaki_gemm_sub_adaptive(hipStream_t s, int smax, int m, int n, int K,
+                                            const double* A, int64_t lda, const double* B,
+                                            int64_t ldb, double* C, int64_t ldc, void* work,
+                                            int s_fixed, int smin, int delta, int e0_in,
+                                            int* dev_scratch, int* s_used, int* e_ref) {
+  MPDOK_TRY(mpdok::ozaki_slice(s, smax, m, n, K, A, lda, B, ldb, work));
+  if (s_fixed >= 2) {
+    MPDOK_REQUIRE(s_fixed <= smax, "ozaki_gemm_sub_adaptive: s_fixed=%d > smax=%d", s_fixed, smax);
+    MPDOK_HIP(hipMemcpyAsync(dev_scratch + 1, &s_fixed, sizeof(int), hipMemcpyHostToDevice, s));
+  } else {
+    const int first = e0_in == INT32_MIN;
+    if (!first)
+      MPDOK_HIP(hipMemcpyAsync(dev_scratch, &e0_in, sizeof(int), hipMemcpyHostToDevice, s));
+    MPDOK_TRY(mpdok::ozaki_decide(s, smax, m, n, K, work, smin, delta, first, dev_scratch,
+                                  dev_scratch + 1));
+  }
+  MPDOK_TRY(mpdok::ozaki_apply_dyn(s, smax, m, n, K, 0, n, C, ldc, work, dev_scratch + 1));
+  int host[2];
+  MPDOK_HIP(hipMemcpyAsync(host, dev_scratch, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  MPDOK_HIP(hipStreamSynchronize(s));
+  if (s_used) *s_used = host[1];
+  if (e_ref) *e_ref = host[0];
+  return 0;
 }

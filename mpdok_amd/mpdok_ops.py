@@ -106,13 +106,30 @@ def _is_oom(exc):
 class _LUHandle:
     """Owned factorization handle in libmpdok_hip."""
 
-    def __init__(self, lib, N, precision, pivot, nb, slices):
+    def __init__(self, lib, N, precision, pivot, nb, slices, adaptive=True):
         self._lib = lib
         h = ctypes.c_void_p()
         check(self._lib.mpdok_lu_create(N, PRECISIONS[precision], PIVOTS[pivot], nb, slices,
                                         ctypes.byref(h)), "lu_create")
         self.ptr = h.value
         self.N = N
+        if precision == "ozaki" and slices <= 5:
+            self.set_adaptive(adaptive)
+
+    def set_adaptive(self, on, smin=2, delta=0):
+        """Adaptive slice counts of the int8-sliced variant (see
+        mpdok_lu_set_adaptive): each trailing update multiplies only as many
+        digits as keep its error bound within the first update's."""
+        check(self._lib.mpdok_lu_set_adaptive(self.ptr, 1 if on else 0, int(smin), int(delta)),
+              "lu_set_adaptive")
+
+    def slice_counts(self):
+        """Slice counts of the last factorisation, one per outer step."""
+        out = (ctypes.c_int * 4096)()
+        n = ctypes.c_int(0)
+        check(self._lib.mpdok_lu_slice_counts(self.ptr, out, 4096, ctypes.byref(n)),
+              "lu_slice_counts")
+        return list(out[:min(n.value, 4096)])
 
     def factor(self, A):
         order, ld = _dev.layout(A)
@@ -157,14 +174,15 @@ class LUIRSolver:
     ~1e-14 relative residual on well-conditioned systems.
     """
 
-    def __init__(self, lib_path=None, precision="fp32", pivot="partial", nb=256, slices=6,
-                 residual="fp64"):
+    def __init__(self, lib_path=None, precision="fp32", pivot="partial", nb=256, slices=5,
+                 residual="fp64", adaptive_slices=True):
         self._lib = _resolve_lib(lib_path, "lu_ir.so")
         self._managed = None
         self.precision = precision
         self.pivot = pivot
         self.nb = nb
         self.slices = slices
+        self.adaptive_slices = adaptive_slices
         self.residual = residual
         self._handle = None
         self.last_info = None
@@ -209,7 +227,8 @@ class LUIRSolver:
             raise ValueError(f"A must be square (N×N), got {tuple(A_gpu.shape)}")
         if self._handle is None or self._handle.N != N:
             self._drop_handle()
-            self._handle = _LUHandle(self._lib, N, self.precision, self.pivot, self.nb, self.slices)
+            self._handle = _LUHandle(self._lib, N, self.precision, self.pivot, self.nb, self.slices,
+                                     self.adaptive_slices)
         self._factored_A = A_gpu
         self._factored_N = N
         self._handle.set_profiling(self.profile)
@@ -363,15 +382,15 @@ class MPDOKSolver:
 
     # -- solve ---------------------------------------------------------------
     def solve(self, A, b, tol=1e-11, maxiter_outer=5, restart=50, precond=None,
-              factor_precision="ozaki", slices=6, pivot="none", body="fp64", residual="dot2",
-              inner_tol=1e-6, nb=512, reuse_factor=False):
+              factor_precision="ozaki", slices=5, pivot="none", body="fp64", residual="dot2",
+              inner_tol=1e-6, nb=512, reuse_factor=False, adaptive_slices=True):
         """Solve A x = b by GMRES-IR; returns x (FP64 device tensor (N,))."""
         try:
             if precond is None:
                 return self._solve(A, b, tol, maxiter_outer, restart)
             return self._solve_precond(A, b, tol, maxiter_outer, restart, precond,
                                        factor_precision, slices, pivot, body, residual,
-                                       inner_tol, nb, reuse_factor)
+                                       inner_tol, nb, reuse_factor, adaptive_slices)
         except Exception as exc:
             if not _is_oom(exc):
                 raise
@@ -422,16 +441,17 @@ class MPDOKSolver:
         return x_gpu
 
     def _solve_precond(self, A, b, tol, maxiter_outer, restart, precond, factor_precision,
-                       slices, pivot, body, residual, inner_tol, nb, reuse_factor):
+                       slices, pivot, body, residual, inner_tol, nb, reuse_factor,
+                       adaptive_slices=True):
         if precond != "lu":
             raise ValueError(f"precond must be None or 'lu', got {precond!r}")
         A_gpu, b_gpu, N = self._prep(A, b)
         order, ld = _dev.layout(A_gpu)
-        key = (N, factor_precision, slices, pivot, nb)
+        key = (N, factor_precision, slices, pivot, nb, bool(adaptive_slices))
         if not (reuse_factor and self._precond is not None and self._precond[0] == key):
             if self._precond is not None:
                 self._precond[1].close()
-            h = _LUHandle(self._lib, N, factor_precision, pivot, nb, slices)
+            h = _LUHandle(self._lib, N, factor_precision, pivot, nb, slices, adaptive_slices)
             h.set_profiling(self.profile)
             info = h.factor(A_gpu)
             if info != 0:
@@ -461,7 +481,8 @@ class MPDOKSolver:
         torch.cuda.current_stream().synchronize()
         self.last_info = {"history": list(hist[:n_hist.value]), "converged": bool(conv.value),
                           "inner_iterations": inner.value,
-                          "factor_stats": h.stats() if self.profile else None}
+                          "factor_stats": h.stats() if self.profile else None,
+                          "slice_counts": h.slice_counts() if factor_precision == "ozaki" else None}
         if not conv.value:
             warnings.warn(
                 f"GMRES-IR did not converge to tol={tol:.1e} within "

@@ -15,6 +15,8 @@ function by function in tests/test_oracle.py:
                  cg/gmres/LAPACK solutions
   lu.*           P A = L U reconstruction, scipy.linalg.lu_factor on tiny N,
                  solve_triangular, LU-IR convergence on well-conditioned systems
-  ozaki.*        bit-exact integer products, error bound 2^(-7 s) scaling
+  ozaki.*        digits reconstruct the rounded input exactly (integer sums),
+                 bit-exact integer products, error bound 2^(-(8 s - 1)) scaling,
+                 digit ranges at the int8 limits
   residual.*     exact rational arithmetic (fractions.Fraction) on small inputs
 """

@@ -6,10 +6,11 @@ infrastructure only).
                  outer-loop contract of mpdok_ops.py:204-243 / :256-320.
   lu_nopiv       blocked right-looking LU without pivoting (FP64 panel), the
                  trailing update supplied as a callable.
-  ozaki_*        Ozaki-scheme splitting of FP64 matrices into signed 7-bit integer
-                 slices with power-of-two row/column scales (Ozaki, Ogita, Oishi,
-                 Rump, Numer. Algorithms 59 (2012)); products of slice pairs are
-                 exact integers; pairs with i + j <= s + 1 are kept.
+  ozaki_*        Ozaki-scheme splitting of FP64 matrices into int8 digit slices
+                 (a 7-bit signed leading digit, then balanced 8-bit digits) with
+                 power-of-two row/column scales (Ozaki, Ogita, Oishi, Rump,
+                 Numer. Algorithms 59 (2012)); products of slice pairs are exact
+                 integers; pairs with i + j <= s + 1 are kept.
   residual_*     exact and Dot2 residuals (Ogita, Rump, Oishi, SIAM J. Sci.
                  Comput. 26 (2005)).
 """
@@ -47,39 +48,55 @@ def lu_ir(A, b, tol=1e-11, maxiter_outer=3, factor_dtype=np.float32):
 # ---------------------------------------------------------------------------
 # Sliced-integer (Ozaki) products
 # ---------------------------------------------------------------------------
+def ozaki_bits(s):
+    """Bits kept per element by s slices: a signed first digit of 7 bits, then
+    s - 1 balanced 8-bit digits."""
+    return 8 * s - 1
+
+
+ZERO_EXP = -1000  # exponent of an all-zero row / column (its digits are all zero)
+
+
 def ozaki_scales(M, axis):
-    """Power-of-two scale per row (axis=1) or per column (axis=0):
-    2^e with e = floor(log2(max|.|)) + 1, so |M / 2^e| < 1; zero rows get 1."""
+    """Power-of-two scale per row (axis=1) or per column (axis=0): 2^e with
+    max|.| * 2^(7 - e) <= 127, i.e. e = floor(log2 max|.|) + 1, plus one when
+    the maximum's leading 8 bits round above 127/128; zero rows get
+    ZERO_EXP (they scale nothing and stay below every real exponent)."""
     mx = np.max(np.abs(M), axis=axis)
     # frexp: mx = f 2^e with f in [0.5, 1), i.e. e = floor(log2 mx) + 1 exactly
     # (log2 itself can round up to an integer just below a power of two)
     _, e = np.frexp(mx)
-    return np.where(mx > 0, e, 0).astype(np.int64)
+    e = np.where(np.ldexp(mx, 7 - e) > 127.0, e + 1, e)
+    return np.where(mx > 0, e, ZERO_EXP).astype(np.int64)
 
 
 def ozaki_slices(M, s, axis):
-    """Signed digits d_1..d_s in [-127, 127] with M = 2^e * sum_i d_i 2^(-7 i)
-    (truncation error < 2^(-7 s) * 2^e per element). Returns (list of int8
-    arrays, exponent array)."""
+    """Integer digits d_1..d_s with M ~= 2^(e - P) * sum_q d_q 256^(s - q),
+    P = 8 s - 1: the element scaled by 2^(P - e) is rounded to the nearest
+    integer V (ties to even), and V is written in balanced base 256 from the
+    least significant digit (d_q in [-128, 127] for q >= 2; |d_1| <= 127
+    because |V| <= 127 * 256^(s - 1)). Error <= 2^(e - P - 1) per element.
+    Returns (list of int8 arrays, exponent array)."""
     M = np.asarray(M, np.float64)
     e = ozaki_scales(M, axis)
-    sc = np.ldexp(1.0, -e)
-    r = M * (sc[:, None] if axis == 1 else sc[None, :])
-    out = []
-    for _ in range(s):
-        d = np.clip(np.rint(r * 128.0), -127.0, 127.0)
-        r = r * 128.0 - d
-        out.append(d.astype(np.int8))
+    P = ozaki_bits(s)
+    sh = (P - e)[:, None] if axis == 1 else (P - e)[None, :]
+    V = np.rint(np.ldexp(M, np.minimum(sh, 1100))).astype(np.int64)
+    out = [None] * s
+    for q in range(s - 1, 0, -1):
+        d = ((V + 128) & 0xFF) - 128
+        V = (V - d) >> 8
+        out[q] = d.astype(np.int8)
+    assert np.abs(V).max(initial=0) <= 127
+    out[0] = V.astype(np.int8)
     return out, e
 
 
-def ozaki_matmul(A, B, s):
-    """Sliced product of A (m x k) and B (k x n): for each level l = 2..s+1,
-    acc_l = sum_{i+j=l} A_i B_j (exact integers), c = sum_l acc_l 2^(-7 l)
-    accumulated in FP64 from l = 2 upwards, result = c * 2^(eA_row + eB_col)."""
-    da, ea = ozaki_slices(A, s, 1)
-    db, eb = ozaki_slices(B, s, 0)
-    m, n = A.shape[0], B.shape[1]
+def _ozaki_fold(da, ea, db, eb, s):
+    """C = sum over kept levels of the digit products: for l = 2..s+1,
+    acc_l = sum_{i+j=l} A_i B_j (exact integers), c = sum_l acc_l 2^(2 - 8 l)
+    in FP64 from l = 2 upwards, result c * 2^(eA_row + eB_col)."""
+    m, n = da[0].shape[0], db[0].shape[1]
     c = np.zeros((m, n))
     for l in range(2, s + 2):
         acc = np.zeros((m, n), np.int64)
@@ -87,8 +104,53 @@ def ozaki_matmul(A, B, s):
             j = l - i
             if j <= s:
                 acc += da[i - 1].astype(np.int64) @ db[j - 1].astype(np.int64)
-        c = c + np.ldexp(acc.astype(np.float64), -7 * l)
+        c = c + np.ldexp(acc.astype(np.float64), 2 - 8 * l)
     return np.ldexp(np.ldexp(c, ea[:, None]), eb[None, :])
+
+
+def ozaki_matmul(A, B, s):
+    """Sliced product of A (m x k) and B (k x n) with s digits per element:
+    pairs of digits with i + j <= s + 1 are kept (see _ozaki_fold)."""
+    return ozaki_matmul_lead(A, B, s, s)
+
+
+def ozaki_matmul_lead(A, B, smax, s):
+    """The product of the leading s of smax digits (s <= smax): the digits
+    come from the smax-digit split, so dropping the trailing ones truncates
+    each element to within about half a unit of digit s."""
+    da, ea = ozaki_slices(A, smax, 1)
+    db, eb = ozaki_slices(B, smax, 0)
+    return _ozaki_fold(da[:s], ea, db[:s], eb, s)
+
+
+def ozaki_decide(A, B, smax, smin, delta, e0):
+    """Slice count of one trailing update (the library's adaptive rule):
+    E = max row exponent of A + max column exponent of B; the first update
+    (e0 None) takes smax and records E; a later one takes
+    S = smax + ceil((E - e0 + delta) / 8) clamped to [smin, smax], the fewest
+    slices whose error bound 2^(E - (8 S - 1)) stays within the first
+    update's minus delta bits. Returns (S, e0)."""
+    E = int(ozaki_scales(A, 1).max(initial=ZERO_EXP)) + int(ozaki_scales(B, 0).max(initial=ZERO_EXP))
+    if e0 is None:
+        return smax, E
+    d = E - e0 + delta
+    S = smax + (-((-d) // 8))  # ceil(d / 8) for either sign
+    return min(max(S, smin), smax), e0
+
+
+class OzakiAdaptiveTrailing:
+    """Trailing-update callable for lu_nopiv with adaptive slice counts;
+    .counts lists the count of every update in order."""
+
+    def __init__(self, smax, smin=2, delta=0):
+        self.smax, self.smin, self.delta = smax, smin, delta
+        self.e0 = None
+        self.counts = []
+
+    def __call__(self, L21, U12):
+        S, self.e0 = ozaki_decide(L21, U12, self.smax, self.smin, self.delta, self.e0)
+        self.counts.append(S)
+        return ozaki_matmul_lead(L21, U12, self.smax, S)
 
 
 # ---------------------------------------------------------------------------

@@ -219,27 +219,136 @@ def test_ozaki_slices_represent_input_to_truncation_bound():
     M = np.random.default_rng(0).standard_normal((20, 30)) * np.logspace(-3, 3, 30)
     for s in (2, 4, 7):
         d, e = olu.ozaki_slices(M, s, axis=1)
-        rec = sum(np.ldexp(di.astype(np.float64), -7 * (i + 1)) for i, di in enumerate(d))
-        rec = np.ldexp(rec, e[:, None])
-        bound = np.ldexp(1.0, e[:, None] - 7 * s)
+        P = 8 * s - 1
+        # integer reconstruction: V = sum_q d_q 256^(s - q) exactly (Python ints)
+        V = np.zeros(M.shape, dtype=object)
+        for di in d:
+            V = V * 256 + di.astype(np.int64).astype(object)
+        rec = np.array([[float(np.ldexp(float(v), int(ee) - P)) for v in row]
+                        for row, ee in zip(V, e)])
+        # round-to-nearest of the scaled element: half a unit of the last digit
+        bound = np.ldexp(1.0, e[:, None] - P - 1)
         assert np.all(np.abs(rec - M) <= bound)
-        assert all(np.abs(di).max() <= 127 for di in d)
+        assert all(di.dtype == np.int8 for di in d)
+        assert np.abs(d[0].astype(int)).max() <= 127
+        # the scale is the smallest power of two that keeps the leading digit in range
+        assert np.all(np.abs(d[0].astype(int)).max(axis=1) >= 63)
+
+
+def test_ozaki_slices_digit_ranges_at_the_int8_limits():
+    # a row maximum whose leading 8 bits round to 128/128 bumps the exponent
+    # (no digit may reach +128); low digits use the full [-128, 127] range
+    M = np.array([[1.0 - 2.0 ** -9, 0.25, -0.5],
+                  [1.0 - 2.0 ** -12, -(1.0 - 2.0 ** -12), 0.0],
+                  [0.5 + 2.0 ** -9, -2.0 ** -9, 2.0 ** -10]])
+    d, e = olu.ozaki_slices(M, 3, axis=1)
+    assert e[0] == 1 and e[1] == 1  # 127.75 and 127.97 exceed 127
+    assert e[2] == 0
+    allv = np.concatenate([di.ravel().astype(int) for di in d[1:]])
+    M2 = np.random.default_rng(5).uniform(-1, 1, (200, 64))
+    d2, _ = olu.ozaki_slices(M2, 4, axis=1)
+    low = np.concatenate([di.ravel().astype(int) for di in d2[1:]])
+    assert low.min() == -128 and low.max() == 127
+    assert np.abs(d2[0].astype(int)).max() <= 127
+    # a matrix of zeros slices to zeros with exponent 0
+    dz, ez = olu.ozaki_slices(np.zeros((3, 5)), 3, axis=0)
+    assert np.all(ez == olu.ZERO_EXP) and all(np.all(di == 0) for di in dz)
+    assert allv.size > 0
 
 
 def test_ozaki_matmul_exact_on_small_integers_and_error_scaling():
     rng = np.random.default_rng(1)
     A = rng.integers(-100, 100, (17, 40)).astype(np.float64)
     B = rng.integers(-100, 100, (40, 23)).astype(np.float64)
-    # integers below 2^7 in magnitude are captured exactly by 2 slices after
-    # the power-of-two scaling: s = 3 keeps every nonzero product pair
+    # integers below 2^7 in magnitude fit the 7-bit leading digit after the
+    # power-of-two scaling, so one slice (s = 1) already gives the exact product;
+    # s = 3 keeps it (the extra digits are zero)
+    assert np.array_equal(olu.ozaki_matmul(A, B, 1), A @ B)
     assert np.array_equal(olu.ozaki_matmul(A, B, 3), A @ B)
     A = rng.standard_normal((64, 128))
     B = rng.standard_normal((128, 48))
     ref = A @ B
-    errs = [np.abs(olu.ozaki_matmul(A, B, s) - ref).max() for s in (3, 4, 5, 6)]
+    errs = [np.abs(olu.ozaki_matmul(A, B, s) - ref).max() for s in (2, 3, 4, 5)]
     for e1, e2 in zip(errs, errs[1:]):
-        assert e2 < e1 * 2.0 ** -5  # each slice buys ~7 bits
-    assert errs[-1] < 1e-10
+        assert e2 < e1 * 2.0 ** -6  # each slice buys ~8 bits
+    # against the exact rational product: error <= (few) * 2^(-(8 s - 1)) * K * max|a| max|b|
+    scale = np.abs(A).max() * np.abs(B).max() * 128
+    for s, err in zip((2, 3, 4, 5), errs):
+        assert err < 4.0 * 2.0 ** -(8 * s - 1) * scale
+    assert errs[-1] < 1e-9
+
+
+def test_ozaki_matmul_drops_exactly_the_low_pairs():
+    # one row / one column with known digits: a = 2^e (d1 2^-7 + d2 2^-15),
+    # b likewise; s = 2 keeps d1*e1 (level 2) and d1*e2 + d2*e1 (level 3),
+    # drops d2*e2 (level 4)
+    a = np.array([[ (100 * 256 + 7) * 2.0 ** -15 ]])
+    b = np.array([[ (-90 * 256 - 100) * 2.0 ** -15 ]])
+    got = olu.ozaki_matmul(a, b, 2)[0, 0]
+    want = (100 * -90 * 2.0 ** -14 + (100 * -100 + 7 * -90) * 2.0 ** -22)
+    assert got == want
+    assert olu.ozaki_matmul(a, b, 3)[0, 0] == a[0, 0] * b[0, 0]
+
+
+def test_ozaki_decide_rule_on_known_exponents():
+    # A's rows peak at 1.0 (exponent 1), B's columns at 0.75 (exponent 0):
+    # E = 1. Scaling B by 2^-k lowers E by k, and the rule drops one slice per
+    # whole 8 bits below the first update's E (ceil), within [smin, smax].
+    rng = np.random.default_rng(3)
+    A = rng.uniform(-1, 1, (40, 16))
+    A[:, 0] = 1.0
+    B = rng.uniform(-0.5, 0.5, (16, 30))
+    B[0, :] = 0.75
+    assert olu.ozaki_scales(A, 1).max() == 1 and olu.ozaki_scales(B, 0).max() == 0
+    S0, e0 = olu.ozaki_decide(A, B, 5, 2, 0, None)
+    assert (S0, e0) == (5, 1)
+    for k, want in [(0, 5), (1, 5), (7, 5), (8, 4), (9, 4), (16, 3), (17, 3), (24, 2), (40, 2)]:
+        S, _ = olu.ozaki_decide(A, np.ldexp(B, -k), 5, 2, 0, e0)
+        assert S == want, (k, S, want)
+    # growth above the first update never exceeds smax; delta tightens by bits
+    assert olu.ozaki_decide(A, B * 64, 5, 2, 0, e0)[0] == 5
+    assert olu.ozaki_decide(A, np.ldexp(B, -8), 5, 2, 4, e0)[0] == 5   # -8 + 4 -> ceil(-0.5) = 0
+    assert olu.ozaki_decide(A, np.ldexp(B, -16), 5, 3, 0, e0)[0] == 3
+    assert olu.ozaki_decide(A, np.ldexp(B, -40), 5, 3, 0, e0)[0] == 3  # clamped at smin
+    # all-zero operands carry ZERO_EXP: the smallest count
+    assert olu.ozaki_decide(np.zeros((4, 16)), B, 5, 2, 0, e0)[0] == 2
+
+
+def test_ozaki_leading_digits_truncate_like_rounding():
+    # the leading s of 5 digits err by at most about half a unit of digit s,
+    # like an s-digit split; the product error scales the same way
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((48, 96))
+    B = rng.standard_normal((96, 40))
+    ref = A @ B
+    for s in (2, 3, 4):
+        e_lead = np.abs(olu.ozaki_matmul_lead(A, B, 5, s) - ref).max()
+        e_own = np.abs(olu.ozaki_matmul(A, B, s) - ref).max()
+        assert e_lead < 4 * e_own and e_own < 4 * e_lead
+        scale = np.abs(A).max() * np.abs(B).max() * 128
+        assert e_lead < 8.0 * 2.0 ** -(8 * s - 1) * scale
+    assert np.array_equal(olu.ozaki_matmul_lead(A, B, 5, 5), olu.ozaki_matmul(A, B, 5))
+
+
+def test_ozaki_adaptive_lu_keeps_the_first_updates_error():
+    # decaying Schur complements of an RBF matrix: later updates take fewer
+    # slices, and the factors still precondition LU-IR to the FP64 floor
+    X = np.random.default_rng(42).uniform(0, 1, (512, 2))
+    A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
+    tr = olu.OzakiAdaptiveTrailing(5)
+    LU = olu.lu_nopiv(A, 64, trailing=tr)
+    assert tr.counts == [5, 5, 5, 4, 4, 3, 3]
+    Lm, Um = olu.unpack(LU)
+    LU5 = olu.lu_nopiv(A, 64, trailing=lambda L, U: olu.ozaki_matmul(L, U, 5))
+    L5, U5 = olu.unpack(LU5)
+    be = np.linalg.norm(Lm @ Um - A) / np.linalg.norm(A)
+    be5 = np.linalg.norm(L5 @ U5 - A) / np.linalg.norm(A)
+    assert be < 4 * be5 + 1e-15
+    b = np.sin(2 * np.pi * X[:, 0])
+    x = np.zeros(512)
+    for _ in range(4):
+        x += olu.lu_solve_packed(LU, b - A @ x)
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-12
 
 
 def test_ozaki_lu_converges_in_lu_ir():
