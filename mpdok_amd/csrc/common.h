@@ -31,6 +31,7 @@ enum TuneKey : int {
   TUNE_OZAKI_SHAPE = 0,  // int8-sliced GEMM block shape (waves M x N, stages, workgroups per CU)
   TUNE_OZAKI_GROUP = 1,  // int8-sliced GEMM: row tiles per group of the tile order (0: 8)
   TUNE_OZAKI_STAGGER = 2,  // persistent int8-sliced GEMM: start-up stagger unit (x 8128 cycles)
+  TUNE_DIAG_KERNEL = 3,    // 64 x 64 diagonal LU: 0 = blocked, 1 = unblocked
   TUNE_COUNT = 8,
 };
 int tuning_get(int key);

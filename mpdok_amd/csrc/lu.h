@@ -40,7 +40,7 @@ size_t ozaki_workspace_bytes(int m, int n, int K, int S);
 int ozaki_gemm(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
                const double* B, int64_t ldb, double* C, int64_t ldc, void* work);
 int ozaki_slice(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
-                const double* B, int64_t ldb, void* work);
+                const double* B, int64_t ldb, void* work, int parts = 3);
 int ozaki_apply_dyn(hipStream_t s, int smax, int m, int n, int K, int col0, int ncols, double* C,
                     int64_t ldc, void* work, const int* Sdev);
 int ozaki_decide(hipStream_t s, int smax, int m, int n, int K, void* work, int smin, int delta,

@@ -43,7 +43,8 @@ struct LUHandle {
   int* perm = nullptr;
   int* d_info = nullptr;
   void* piv_work = nullptr;
-  void* oz_work = nullptr;
+  void* oz_work = nullptr;   // int8 slices of even steps
+  void* oz_work2 = nullptr;  // ... and of odd steps (the panel slices the next step's rows)
   void* v1 = nullptr;
   void* v2 = nullptr;
   // inverse-based triangular solves (N a multiple of TRSV_BLOCK)
@@ -95,7 +96,8 @@ void lu_destroy(LUHandle* h) {
   if (h->side) (void)hipStreamDestroy(h->side);
   if (h->Wp) (void)hipFree(h->Wp);
   void* ptrs[] = {h->LU, h->Linv, h->Uinv, h->W, h->ipiv, h->perm, h->d_info,
-                  h->piv_work, h->oz_work, h->v1, h->v2, h->Lbinv, h->Ubinv, h->v3, h->acc,
+                  h->piv_work, h->oz_work, h->oz_work2, h->v1, h->v2, h->Lbinv, h->Ubinv, h->v3,
+                  h->acc,
                   h->d_S};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -143,6 +145,7 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   alloc(&h->v2, (size_t)N * 8);
   if (precision == LU_OZAKI) {
     alloc(&h->oz_work, ozaki_workspace_bytes(N, N, nb, slices));
+    alloc(&h->oz_work2, ozaki_workspace_bytes(N, N, nb, slices));
     alloc((void**)&h->d_S, sizeof(int) * (N / nb + 3));
   }
   h->adaptive = precision == LU_OZAKI && slices <= 5;
@@ -158,7 +161,10 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
     st = EHIP;
   }
   if (st == OK && pivot == PIV_NONE) {
-    alloc(&h->Wp, (size_t)N * 64 * h->es);
+    // the panel's scratch also serves the recursive doubling of its inverse
+    // block ((TRSV_BLOCK / 2)^2 elements)
+    const size_t wp = (size_t)N * 64 > (size_t)65536 ? (size_t)N * 64 : (size_t)65536;
+    alloc(&h->Wp, wp * h->es);
     // default-priority stream (no priority is ever set on any stream)
     if (st == OK && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
                      hipEventCreateWithFlags(&h->ev_next_cols, hipEventDisableTiming) != hipSuccess ||
@@ -175,23 +181,36 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   return OK;
 }
 
+// Slice workspace of the trailing update of step k (alternating, so that the
+// panel can slice the next step's rows while this step's update reads its own).
+static void* oz_ws(const LUHandle* h, int k) { return ((k / h->nb) & 1) ? h->oz_work2 : h->oz_work; }
+
+// Whether the panel [k, k + kb) is a whole TRSV_BLOCK diagonal block whose
+// inverses the handle keeps (built right after the panel: the block-row
+// solve of the next step multiplies by the L one).
+static bool panel_block_inverse(const LUHandle* h, int k, int kb) {
+  return h->pivot == PIV_NONE && h->Lbinv != nullptr && kb == TRSV_BLOCK &&
+         k % TRSV_BLOCK == 0 && k + TRSV_BLOCK <= h->N;
+}
+
 // Inverses of the TRSV_BLOCK x TRSV_BLOCK diagonal blocks of L and U by
 // recursive doubling from the 64 x 64 inverses:
 //   inv([A 0; C D]) = [inv(A) 0; -inv(D) C inv(A)  inv(D)]
 //   inv([A B; 0 D]) = [inv(A) -inv(A) B inv(D); 0  inv(D)]
 // batched over all big blocks (uniform strides along the diagonal).
+// Blocks b0 .. b0 + nbig - 1; W: scratch of (TRSV_BLOCK / 2)^2 nbig elements.
 template <typename T>
-static int build_block_inverses(LUHandle* h, hipStream_t s) {
+static int build_block_inverses(LUHandle* h, hipStream_t s, int b0, int nbig, T* W) {
   const int N = h->N, TBk = TRSV_BLOCK;
-  const int nbig = N / TBk;
-  const T* LU = static_cast<const T*>(h->LU);
-  T* Lb = static_cast<T*>(h->Lbinv);
-  T* Ub = static_cast<T*>(h->Ubinv);
-  T* W = static_cast<T*>(h->W);
+  if (nbig <= 0) return OK;
   const int64_t ld = N;
   const int64_t sLU = (int64_t)TBk * (ld + 1), sB = (int64_t)TBk * TBk;
-  MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Linv), Lb, nbig));
-  MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Uinv), Ub, nbig));
+  const T* LU = static_cast<const T*>(h->LU) + b0 * sLU;
+  T* Lb = static_cast<T*>(h->Lbinv) + b0 * sB;
+  T* Ub = static_cast<T*>(h->Ubinv) + b0 * sB;
+  const size_t i64 = (size_t)b0 * (TBk / 64) * 4096;
+  MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Linv) + i64, Lb, nbig));
+  MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Uinv) + i64, Ub, nbig));
   for (int hh = 64; hh < TBk; hh *= 2) {
     const int64_t sW = (int64_t)hh * hh;
     for (int off = 0; off < TBk; off += 2 * hh) {
@@ -215,7 +234,12 @@ static int build_block_inverses(LUHandle* h, hipStream_t s) {
 }
 
 // Factor the panel of columns [k, k + kb) over rows [k, N): 64-column
-// sub-panels (diagonal block in LDS, L21, U12 inside the panel, panel update).
+// sub-panels (diagonal block LU and the inverses of its triangles in LDS;
+// L21 = A21 inv(U11) and the U12 inside the panel = inv(L11) A12 as GEMMs
+// with those inverses; panel update by GEMM). Without pivoting, when the
+// panel is a whole inverse block of the solves, its TRSV_BLOCK-wide inverses
+// follow by recursive doubling: the block-row solve of the next step
+// multiplies by the L one. W: scratch (the side stream's, under lookahead).
 template <typename T>
 static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W) {
   const int N = h->N;
@@ -236,24 +260,33 @@ static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W) {
       MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 0));
     } else {
       MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 1));
-      if (m2 > 0) {
-        MPDOK_TRY(gemm<T>(s, m2, jb, jb, T(1), at(j + jb, j), ld, Ui, 64, T(0), W, m2, 0));
-        MPDOK_TRY((copy2d<T, T>(s, W, m2, at(j + jb, j), ld, m2, jb)));
-      }
+      // L21 = A21 inv(U11), in place (n = jb <= 64: each workgroup its own rows)
+      if (m2 > 0)
+        MPDOK_TRY(gemm<T>(s, m2, jb, jb, T(1), at(j + jb, j), ld, Ui, 64, T(0), at(j + jb, j), ld,
+                          0));
     }
     const int n2 = k + kb - j - jb;
     if (n2 > 0) {
-      MPDOK_TRY(gemm<T>(s, jb, n2, jb, T(1), Li, 64, at(j, j + jb), ld, T(0), W, jb, 0));
-      MPDOK_TRY((copy2d<T, T>(s, W, jb, at(j, j + jb), ld, jb, n2)));
+      // U12 = inv(L11) A12, in place (m = jb <= 64: each workgroup its own columns)
+      MPDOK_TRY(gemm<T>(s, jb, n2, jb, T(1), Li, 64, at(j, j + jb), ld, T(0), at(j, j + jb), ld, 0));
       if (m2 > 0)
         MPDOK_TRY(gemm<T>(s, m2, n2, jb, T(-1), at(j + jb, j), ld, at(j, j + jb), ld, T(1),
                           at(j + jb, j + jb), ld, 0));
     }
   }
+  if (panel_block_inverse(h, k, kb))
+    MPDOK_TRY(build_block_inverses<T>(h, s, k / TRSV_BLOCK, 1, W));
+  // int8-sliced variant: the rows of L21 for this step's trailing update
+  const int n3 = N - k - kb;
+  if (h->precision == LU_OZAKI && n3 > 0)
+    MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb, reinterpret_cast<const double*>(at(k + kb, k)),
+                          ld, nullptr, ld, oz_ws(h, k), 1));
   return OK;
 }
 
-// U12 = inv(L11) A12 for the block row [k, k + kb), columns [c0, N).
+// U12 = inv(L11) A12 for the block row [k, k + kb), columns [c0, N): one
+// GEMM with the TRSV_BLOCK-wide inverse when the panel built it, else blocked
+// forward substitution with the 64 x 64 inverses.
 template <typename T>
 static int block_row_solve(LUHandle* h, hipStream_t s, int k, int kb, int c0, T* W) {
   const int N = h->N, n3 = N - c0;
@@ -261,6 +294,12 @@ static int block_row_solve(LUHandle* h, hipStream_t s, int k, int kb, int c0, T*
   T* Linv = static_cast<T*>(h->Linv);
   const int64_t ld = N;
   auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
+  if (panel_block_inverse(h, k, kb)) {
+    const T* Lb = static_cast<const T*>(h->Lbinv) + (size_t)(k / TRSV_BLOCK) * TRSV_BLOCK * TRSV_BLOCK;
+    MPDOK_TRY(gemm<T>(s, kb, n3, kb, T(1), Lb, TRSV_BLOCK, at(k, c0), ld, T(0), W, kb, 0));
+    MPDOK_TRY((copy2d<T, T>(s, W, kb, at(k, c0), ld, kb, n3)));
+    return OK;
+  }
   for (int j = k; j < k + kb; j += 64) {
     const int jb = 64 < k + kb - j ? 64 : k + kb - j;
     T* Li = Linv + (size_t)(j / 64) * 4096;
@@ -278,13 +317,14 @@ static bool adaptive_on(const LUHandle* h) {
   return h->precision == LU_OZAKI && h->adaptive && h->slices <= 5 && h->d_S != nullptr;
 }
 
-// Slices of the panel pair of step k (smax digits), and with adaptive slice
-// counts the decision for its trailing update.
-static int slice_step(LUHandle* h, hipStream_t s, int k, int kb, int n3, const double* L21,
-                      const double* U12, int64_t ld) {
-  MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb, L21, ld, U12, ld, h->oz_work));
+// Column slices of U12 for step k (its rows of L21 were sliced with the
+// panel, smax digits), and with adaptive slice counts the decision for its
+// trailing update.
+static int slice_step(LUHandle* h, hipStream_t s, int k, int kb, int n3, const double* U12,
+                      int64_t ld) {
+  MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb, nullptr, ld, U12, ld, oz_ws(h, k), 2));
   if (adaptive_on(h))
-    MPDOK_TRY(ozaki_decide(s, h->slices, n3, n3, kb, h->oz_work, h->smin, h->delta, k == 0,
+    MPDOK_TRY(ozaki_decide(s, h->slices, n3, n3, kb, oz_ws(h, k), h->smin, h->delta, k == 0,
                            h->d_S, h->d_S + 1 + k / h->nb));
   return OK;
 }
@@ -313,11 +353,11 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
   }
   if (adaptive_on(h)) {
     MPDOK_TRY(ozaki_apply_dyn(s, h->slices, m3, n3, kb, c_off, ncols,
-                              reinterpret_cast<double*>(at(k + kb, k + kb)), ld, h->oz_work,
+                              reinterpret_cast<double*>(at(k + kb, k + kb)), ld, oz_ws(h, k),
                               h->d_S + 1 + k / h->nb));
   } else if (h->precision == LU_OZAKI) {
     MPDOK_TRY(ozaki_apply(s, h->slices, m3, n3, kb, c_off, ncols,
-                          reinterpret_cast<double*>(at(k + kb, k + kb)), ld, h->oz_work));
+                          reinterpret_cast<double*>(at(k + kb, k + kb)), ld, oz_ws(h, k)));
   } else {
     MPDOK_TRY(gemm<T>(s, m3, ncols, kb, T(-1), at(k + kb, k), ld, at(k, k + kb + c_off), ld, T(1),
                       at(k + kb, k + kb + c_off), ld, round));
@@ -349,8 +389,7 @@ static int factor_impl(LUHandle* h, hipStream_t s) {
       if (k > 0) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
       MPDOK_TRY(block_row_solve<T>(h, s, k, kb, k + kb, W));
       if (h->precision == LU_OZAKI)
-        MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k + kb, k)),
-                             reinterpret_cast<const double*>(at(k, k + kb)), ld));
+        MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k, k + kb)), ld));
       const int nk = nb < n3 ? nb : n3;
       MPDOK_TRY(trailing_cols<T>(h, s, k, kb, 0, nk));
       MPDOK_HIP(hipEventRecord(h->ev_next_cols, s));
@@ -360,7 +399,8 @@ static int factor_impl(LUHandle* h, hipStream_t s) {
       MPDOK_TRY(trailing_cols<T>(h, s, k, kb, nk, n3 - nk));
     }
     if (N > nb) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
-    if (h->fast_solve) MPDOK_TRY(build_block_inverses<T>(h, s));
+    if (h->fast_solve && nb != TRSV_BLOCK)
+      MPDOK_TRY(build_block_inverses<T>(h, s, 0, N / TRSV_BLOCK, W));
     return OK;
   }
   for (int k = 0; k < N; k += nb) {
@@ -370,12 +410,12 @@ static int factor_impl(LUHandle* h, hipStream_t s) {
     if (n3 <= 0) continue;
     MPDOK_TRY(block_row_solve<T>(h, s, k, kb, k + kb, W));
     if (h->precision == LU_OZAKI)
-      MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k + kb, k)),
-                           reinterpret_cast<const double*>(at(k, k + kb)), ld));
+      MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k, k + kb)), ld));
     MPDOK_TRY(trailing_cols<T>(h, s, k, kb, 0, n3));
   }
   if (h->pivot == PIV_PARTIAL) MPDOK_TRY(ipiv_to_perm(s, h->ipiv, h->perm, N));
-  if (h->fast_solve) MPDOK_TRY(build_block_inverses<T>(h, s));
+  if (h->fast_solve && !(h->pivot == PIV_NONE && nb == TRSV_BLOCK))
+    MPDOK_TRY(build_block_inverses<T>(h, s, 0, N / TRSV_BLOCK, W));
   return OK;
 }
 

@@ -66,12 +66,17 @@ constexpr int GBK = 16, GPAD = 16;
 // TB = 64 is for the skinny products of the panel and the block-row solve: it
 // makes twice the workgroups per row of tiles and needs 38 KB of LDS (FP64)
 // instead of 72 KB, so it fits next to a trailing update's workgroup on a CU.
+//
+// C may alias A when n <= TB (each workgroup then reads only the rows of A it
+// writes) or alias B when m <= TB (only its own columns): every read of the
+// operand happens in the K loop, before the workgroup's stores, and beta = 0
+// leaves C unread. The panel uses this for its in-place products with the
+// diagonal-block inverses.
 template <typename T, int ROUND, int TB>
-__global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alpha,
-                                                   const T* __restrict__ A, int64_t lda,
-                                                   const T* __restrict__ B, int64_t ldb, T beta,
-                                                   T* __restrict__ C, int64_t ldc, int64_t sA_,
-                                                   int64_t sB_, int64_t sC_) {
+__global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alpha, const T* A,
+                                                   int64_t lda, const T* B, int64_t ldb, T beta,
+                                                   T* C, int64_t ldc, int64_t sA_, int64_t sB_,
+                                                   int64_t sC_) {
   constexpr int KPH = 256 / TB;      // k phases of the A loads
   constexpr int NA = GBK / KPH;      // A elements per thread
   constexpr int NBL = TB / 16;       // B elements per thread
@@ -327,11 +332,195 @@ __global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int
   }
 }
 
+// Blocked variant (the default), same contract and the same 34 KB of LDS:
+//  1. LU in four 16-column sub-panels p: wavefront 0 eliminates the
+//     sub-panel (rows 16 p .. 63, lane r = row r, wavefront-level syncs
+//     only); the U12 rows of the sub-panel are solved against its unit-lower
+//     16 x 16 triangle (one thread per column); the rest of the block takes a
+//     rank-16 update. Three workgroup barriers per sub-panel.
+//  2. Inverses in place: the four 16 x 16 diagonal blocks of U (wavefront 0)
+//     and of the unit-lower L (wavefront 1), one lane per column, by
+//     substitution in registers; then two levels of recursive doubling,
+//       inv([A B; 0 D]) = [inv(A) -inv(A) B inv(D); 0 inv(D)]
+//       inv([A 0; C D]) = [inv(A) 0; -inv(D) C inv(A) inv(D)],
+//     16 -> 32 -> 64, each off-diagonal block formed as two products staged
+//     through registers (T = inv(A) B, then -T inv(D), written back in place).
+// The rounding differs from the unblocked kernel's (other summation orders).
+template <typename T>
+__global__ __launch_bounds__(256) void diag_factor_blk_kernel(T* __restrict__ A, int64_t lda,
+                                                              int nb, T* __restrict__ Linv,
+                                                              T* __restrict__ Uinv, int ldi,
+                                                              int* __restrict__ info, int col0,
+                                                              int factor) {
+  __shared__ T s[64][65];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  // identity outside nb x nb, so the blocked steps need no bounds
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    s[r][c] = (r < nb && c < nb) ? A[r + (int64_t)c * lda] : (r == c ? T(1) : T(0));
+  }
+  __syncthreads();
+  if (factor) {
+    for (int p = 0; p < 4; ++p) {
+      const int c0 = 16 * p;
+      if (w == 0) {
+        const int r = lane;
+        for (int j = c0; j < c0 + 16; ++j) {
+          const T piv = s[j][j];
+          if (r == 0 && j < nb && piv == T(0)) atomicCAS(info, 0, col0 + j + 1);
+          if (r > j) {
+            const T l = s[r][j] / piv;
+            s[r][j] = l;
+            for (int c = j + 1; c < c0 + 16; ++c) s[r][c] = __builtin_fma(-l, s[j][c], s[r][c]);
+          }
+          wave_sync();
+        }
+      }
+      __syncthreads();
+      if (p == 3) break;
+      // U12 of the sub-panel: rows c0 .. c0 + 15, columns c0 + 16 .. 63
+      const int ncol = 48 - c0;
+      if (t < ncol) {
+        const int c = c0 + 16 + t;
+        for (int i = c0 + 1; i < c0 + 16; ++i) {
+          T acc = s[i][c];
+          for (int k = c0; k < i; ++k) acc = __builtin_fma(-s[i][k], s[k][c], acc);
+          s[i][c] = acc;
+        }
+      }
+      __syncthreads();
+      // rank-16 update of rows and columns c0 + 16 .. 63
+      for (int e = t; e < ncol * ncol; e += 256) {
+        const int r = c0 + 16 + e % ncol, c = c0 + 16 + e / ncol;
+        T a0 = s[r][c], a1 = T(0);
+#pragma unroll
+        for (int k = 0; k < 16; k += 2) {
+          a0 = __builtin_fma(-s[r][c0 + k], s[c0 + k][c], a0);
+          a1 = __builtin_fma(-s[r][c0 + k + 1], s[c0 + k + 1][c], a1);
+        }
+        s[r][c] = a0 + a1;
+      }
+      __syncthreads();
+    }
+    for (int e = t; e < 64 * 64; e += 256) {
+      const int r = e & 63, c = e >> 6;
+      if (r < nb && c < nb) A[r + (int64_t)c * lda] = s[r][c];
+    }
+  }
+  // ---- inverses of the 16 x 16 diagonal blocks: wavefront 0 U, 1 L ----
+  if (w < 2) {
+    const int b = lane >> 4, c = lane & 15, o = 16 * b;
+    T x[16];
+    if (w == 0) {  // column c of inv(U_bb): x_c = 1/u_cc, x_i = -(sum_{k>i} u_ik x_k)/u_ii
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = k == c ? T(1) / s[o + k][o + k] : T(0);
+#pragma unroll
+      for (int i = 14; i >= 0; --i) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = i + 1; k < 16; ++k) acc = __builtin_fma(s[o + i][o + k], x[k], acc);
+        if (i < c) x[i] = -acc / s[o + i][o + i];
+      }
+      wave_sync();
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i <= c) s[o + i][o + c] = x[i];
+    } else {  // column c of inv(L_bb): x_c = 1, x_i = -sum_{k<i} l_ik x_k (i > c)
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = k == c ? T(1) : T(0);
+#pragma unroll
+      for (int i = 1; i < 16; ++i) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < i; ++k) acc = __builtin_fma(s[o + i][o + k], x[k], acc);
+        if (i > c) x[i] = -acc;
+      }
+      wave_sync();
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i > c) s[o + i][o + c] = x[i];
+    }
+  }
+  __syncthreads();
+  // ---- recursive doubling: h = 16, 32 ----
+  for (int h = 16; h < 64; h *= 2) {
+    const int npair = 32 / h;               // pairs per triangle at this level
+    const int per = h * h, total = 2 * npair * per;  // off-diagonal elements, U and L
+    constexpr int MAXE = 8;                 // total / 256 <= 8
+    T v[MAXE];
+    // T = inv(A) B (U), T = C inv(A) (L): staged in registers
+    for (int q = 0; q < MAXE; ++q) {
+      const int e = t + 256 * q;
+      if (e >= total) break;
+      const int tri = e / (npair * per), pr = (e / per) % npair, idx = e % per;
+      const int i = idx % h, j = idx / h, o = 2 * h * pr;
+      T acc = T(0);
+      if (tri == 0) {  // U: (inv(A) B)_ij = sum_{k >= i} invA_ik B_kj
+        for (int k = i; k < h; ++k) acc = __builtin_fma(s[o + i][o + k], s[o + k][o + h + j], acc);
+      } else {  // L: (C inv(A))_ij = sum_{k <= j... } C_ik invA_kj, invA unit lower
+        acc = s[o + h + i][o + j];
+        for (int k = j + 1; k < h; ++k) acc = __builtin_fma(s[o + h + i][o + k], s[o + k][o + j], acc);
+      }
+      v[q] = acc;
+    }
+    __syncthreads();
+    for (int q = 0; q < MAXE; ++q) {
+      const int e = t + 256 * q;
+      if (e >= total) break;
+      const int tri = e / (npair * per), pr = (e / per) % npair, idx = e % per;
+      const int i = idx % h, j = idx / h, o = 2 * h * pr;
+      if (tri == 0) s[o + i][o + h + j] = v[q];
+      else s[o + h + i][o + j] = v[q];
+    }
+    __syncthreads();
+    // X = -T inv(D) (U), X = -inv(D) T (L)
+    for (int q = 0; q < MAXE; ++q) {
+      const int e = t + 256 * q;
+      if (e >= total) break;
+      const int tri = e / (npair * per), pr = (e / per) % npair, idx = e % per;
+      const int i = idx % h, j = idx / h, o = 2 * h * pr;
+      T acc = T(0);
+      if (tri == 0) {  // sum_{k <= j} T_ik invD_kj
+        for (int k = 0; k <= j; ++k) acc = __builtin_fma(s[o + i][o + h + k], s[o + h + k][o + h + j], acc);
+      } else {  // sum_{k <= i} invD_ik T_kj, invD unit lower
+        acc = s[o + h + i][o + j];
+        for (int k = 0; k < i; ++k) acc = __builtin_fma(s[o + h + i][o + h + k], s[o + h + k][o + j], acc);
+      }
+      v[q] = -acc;
+    }
+    __syncthreads();
+    for (int q = 0; q < MAXE; ++q) {
+      const int e = t + 256 * q;
+      if (e >= total) break;
+      const int tri = e / (npair * per), pr = (e / per) % npair, idx = e % per;
+      const int i = idx % h, j = idx / h, o = 2 * h * pr;
+      if (tri == 0) s[o + i][o + h + j] = v[q];
+      else s[o + h + i][o + j] = v[q];
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    if (r < nb && c < nb) {
+      Uinv[r + c * ldi] = r <= c ? s[r][c] : T(0);
+      Linv[r + c * ldi] = r > c ? s[r][c] : (r == c ? T(1) : T(0));
+    }
+  }
+}
+
 template <typename T>
 int diag_factor(hipStream_t s, T* A, int64_t lda, int nb, T* Linv, T* Uinv, int ldi, int* info,
                 int col0, int factor) {
   MPDOK_REQUIRE(nb >= 1 && nb <= 64, "diag_factor: nb=%d", nb);
-  diag_factor_kernel<T><<<1, 256, 0, s>>>(A, lda, nb, Linv, Uinv, ldi, info, col0, factor);
+  if (tuning_get(TUNE_DIAG_KERNEL) == 1)
+    diag_factor_kernel<T><<<1, 256, 0, s>>>(A, lda, nb, Linv, Uinv, ldi, info, col0, factor);
+  else
+    diag_factor_blk_kernel<T><<<1, 256, 0, s>>>(A, lda, nb, Linv, Uinv, ldi, info, col0, factor);
   MPDOK_HIP(hipGetLastError());
   return OK;
 }

@@ -716,27 +716,32 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmd_kernel(
 // i.e. S = SMAX + ceil((E - E0 + delta) / 8), clamped. Rows and columns that
 // are all zero carry the exponent OZ_ZERO_EXP and never decide the maximum
 // unless everything is zero.
-__global__ __launch_bounds__(256) void oz_decide_kernel(const int* __restrict__ ea, int m,
-                                                        const int* __restrict__ eb, int n,
-                                                        int smax, int smin, int delta, int first,
-                                                        int* __restrict__ e0,
-                                                        int* __restrict__ s_out) {
-  __shared__ int red[2][256];
+__global__ __launch_bounds__(1024) void oz_decide_kernel(const int* __restrict__ ea, int m,
+                                                         const int* __restrict__ eb, int n,
+                                                         int smax, int smin, int delta, int first,
+                                                         int* __restrict__ e0,
+                                                         int* __restrict__ s_out) {
+  __shared__ int red[2][16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int a = OZ_ZERO_EXP, b = OZ_ZERO_EXP;
-  for (int i = threadIdx.x; i < m; i += 256) a = max(a, ea[i]);
-  for (int j = threadIdx.x; j < n; j += 256) b = max(b, eb[j]);
-  red[0][threadIdx.x] = a;
-  red[1][threadIdx.x] = b;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      red[0][threadIdx.x] = max(red[0][threadIdx.x], red[0][threadIdx.x + w]);
-      red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + w]);
-    }
-    __syncthreads();
+  for (int i = t; i < m; i += 1024) a = max(a, ea[i]);
+  for (int j = t; j < n; j += 1024) b = max(b, eb[j]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a = max(a, __shfl_xor(a, off, 64));
+    b = max(b, __shfl_xor(b, off, 64));
   }
-  if (threadIdx.x == 0) {
-    const int E = red[0][0] + red[1][0];
+  if (lane == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int q = 1; q < 16; ++q) {
+      a = max(a, red[0][q]);
+      b = max(b, red[1][q]);
+    }
+    const int E = a + b;
     int S = smax;
     if (first) {
       *e0 = E;
@@ -1017,10 +1022,10 @@ static OzLayout oz_layout(int S, int m, int n, int K, void* work) {
 
 template <int S>
 static int ozaki_slice_s(hipStream_t s, int m, int n, int K, const double* A, int64_t lda,
-                         const double* B, int64_t ldb, void* work) {
+                         const double* B, int64_t ldb, void* work, int parts) {
   const OzLayout L = oz_layout(S, m, n, K, work);
-  slice_rows_kernel<S><<<L.mp / 64, 256, 0, s>>>(A, lda, m, K, L.mp, L.Kp, L.As, L.ea);
-  slice_cols_kernel<S><<<L.np / 4, 256, 0, s>>>(B, ldb, n, K, L.np, L.Kp, L.Bs, L.eb);
+  if (parts & 1) slice_rows_kernel<S><<<L.mp / 64, 256, 0, s>>>(A, lda, m, K, L.mp, L.Kp, L.As, L.ea);
+  if (parts & 2) slice_cols_kernel<S><<<L.np / 4, 256, 0, s>>>(B, ldb, n, K, L.np, L.Kp, L.Bs, L.eb);
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
@@ -1167,24 +1172,25 @@ int ozaki_decide(hipStream_t s, int smax, int m, int n, int K, void* work, int s
                  int first, int* e0, int* s_out) {
   const OzLayout L = oz_layout(smax, m, n, K, work);
   MPDOK_REQUIRE(smin >= 2 && smin <= smax, "ozaki_decide: smin=%d outside [2, %d]", smin, smax);
-  oz_decide_kernel<<<1, 256, 0, s>>>(L.ea, m, L.eb, n, smax, smin, delta, first, e0, s_out);
+  oz_decide_kernel<<<1, 1024, 0, s>>>(L.ea, m, L.eb, n, smax, smin, delta, first, e0, s_out);
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
 
-// Slice A (m x K) by rows and B (K x n) by columns into the workspace.
+// Slice A (m x K) by rows (parts & 1) and B (K x n) by columns (parts & 2)
+// into the workspace laid out for (S, m, n, K).
 int ozaki_slice(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,
-                const double* B, int64_t ldb, void* work) {
+                const double* B, int64_t ldb, void* work, int parts) {
   if (m <= 0 || n <= 0 || K <= 0) return OK;
   // int32 accumulators: a level sums at most S products of K terms of |d d'| <= 2^14
   MPDOK_REQUIRE((int64_t)S * K * 16384 <= INT32_MAX, "ozaki_slice: K=%d too long for %d slices", K,
                 S);
   switch (S) {
-    case 3: return ozaki_slice_s<3>(s, m, n, K, A, lda, B, ldb, work);
-    case 4: return ozaki_slice_s<4>(s, m, n, K, A, lda, B, ldb, work);
-    case 5: return ozaki_slice_s<5>(s, m, n, K, A, lda, B, ldb, work);
-    case 6: return ozaki_slice_s<6>(s, m, n, K, A, lda, B, ldb, work);
-    case 7: return ozaki_slice_s<7>(s, m, n, K, A, lda, B, ldb, work);
+    case 3: return ozaki_slice_s<3>(s, m, n, K, A, lda, B, ldb, work, parts);
+    case 4: return ozaki_slice_s<4>(s, m, n, K, A, lda, B, ldb, work, parts);
+    case 5: return ozaki_slice_s<5>(s, m, n, K, A, lda, B, ldb, work, parts);
+    case 6: return ozaki_slice_s<6>(s, m, n, K, A, lda, B, ldb, work, parts);
+    case 7: return ozaki_slice_s<7>(s, m, n, K, A, lda, B, ldb, work, parts);
     default: MPDOK_REQUIRE(false, "ozaki_slice: slices must be in [3, 7], got %d", S);
   }
 }
