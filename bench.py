@@ -28,7 +28,7 @@ import numpy as np
 METRIC = ("Time to relres≤1e-12: mixed-precision GMRES-IR vs plain FP64 on GPU, N=32768")
 PEAK_INT8_TOPS = 5000.0     # MI355X dense int8, tera-operations/s
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                               "ozaki_gemm_traffic_n32768.md")
+                               "ozaki_gemmd_traffic_n32768.md")
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -42,12 +42,12 @@ def rhs_of(X, seed=43):
             + 1e-4 * np.random.default_rng(seed).standard_normal(X.shape[0]))
 
 
-def measured_traffic(N, slices):
+def measured_traffic(N, slices, nb):
     """HBM bytes per launch of the int8-sliced trailing update, from the
     committed FETCH_SIZE / WRITE_SIZE counter summary (its own run; the
     counters cannot be read inside a timed run). Only for the workload the
     summary was measured on."""
-    if N != 32768 or slices != 5 or not os.path.exists(TRAFFIC_PROFILE):
+    if N != 32768 or slices != 5 or nb != 512 or not os.path.exists(TRAFFIC_PROFILE):
         return None
     for line in open(TRAFFIC_PROFILE):
         if line.startswith("traffic_bytes_per_launch:"):
@@ -203,19 +203,38 @@ def main():
         log(f"[bench] extra {extra}")
 
     st = info_mix["factor_stats"] or {}
+    counts = info_mix.get("slice_counts") or []
     roofline = None
     if st.get("launches"):
         avg_ms = st["trailing_ms"] / st["launches"]
         achieved = st["trailing_ops"] / (st["trailing_ms"] * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "kernel": f"ozaki_gemm_kernel<{args.slices}>",
+        # per launch: the larger of its int8-MFMA time and its HBM time (FP64 C
+        # read + write) at peak, over the measured time: the fraction of the
+        # mixed roofline (the low slice counts are HBM-bound)
+        t_roof = 0.0
+        nb = args.nb
+        for step, k in enumerate(range(0, N, nb)):
+            n3 = N - k - nb
+            if n3 <= 0:
+                break
+            S = counts[step] if step < len(counts) else args.slices
+            for nc in ([min(nb, n3)] + ([n3 - nb] if n3 > nb else [])):
+                ops = 2.0 * n3 * nc * nb * S * (S + 1) / 2
+                byts = 16.0 * n3 * nc
+                t_roof += max(ops / (PEAK_INT8_TOPS * 1e12), byts / (PEAK_HBM_GBS * 1e9))
+        roofline = {"bound": "mfma", "kernel": f"ozaki_gemmd_kernel<{args.slices}, ...>",
                     "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                     "frac": achieved / PEAK_INT8_TOPS,
-                    "traffic": measured_traffic(N, args.slices),
-                    "note": ("int8 tera-operations/s of the trailing updates: 2 m n k "
-                             f"x {args.slices * (args.slices + 1) // 2} slice pairs per launch, "
-                             f"{st['launches']} launches, avg {avg_ms:.3f} ms (HIP events); "
-                             "traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
-                             "(profiles/ozaki_gemm_traffic_n32768.md)")}
+                    "traffic": measured_traffic(N, args.slices, args.nb),
+                    "mixed_roofline_frac": t_roof / (st["trailing_ms"] * 1e-3),
+                    "note": ("int8 tera-operations/s of the trailing updates: 2 m n k x "
+                             "S_k (S_k + 1) / 2 slice pairs per launch, S_k the adaptive slice "
+                             f"count of step k (slice_counts), {st['launches']} launches, avg "
+                             f"{avg_ms:.3f} ms (HIP events on the factorisation's stream); "
+                             "mixed_roofline_frac: sum over launches of max(int8 ops / 5.0 POP/s, "
+                             "16 B per C element / 8 TB/s) over the measured time; traffic: HBM "
+                             "bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                             "(profiles/ozaki_gemmd_traffic_n32768.md)")}
     plain_roof = None
     if plain_stats and plain_stats.get("launches"):
         a = plain_stats["trailing_ops"] / (plain_stats["trailing_ms"] * 1e-3) / 1e12
@@ -231,13 +250,14 @@ def main():
     out = {
         "metric": METRIC, "value": t_mix, "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t_mix * 1e3, "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (int8-sliced trailing updates)",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (int8-sliced trailing updates, adaptive slice counts)",
         "data": "synthetic",
         "config": {"workload": "gaussian_rbf_3d_gmres_ir", "N": N, "D": 3, "eps": 5.0,
                    "nugget": 1e-6, "tol": tol, "slices": args.slices, "nb": args.nb,
                    "seeds": {"points": 42, "noise": 43}, "parallelism": "single_gpu"},
         "relres": rel_mix, "converged": info_mix["converged"],
         "outer_history": info_mix["history"], "inner_iterations": info_mix["inner_iterations"],
+        "slice_counts": counts[:-1] if counts else None,
         "forward_error_vs_fp64_refined": fwd_mix, "forward_error_fp64_direct": fwd_direct,
         "backward_error": bwd_mix,
         "plain_fp64": {"s": t_plain, "history": plain_hist, "method": "FP64 LU + FP64 IR (Dot2)",
@@ -279,6 +299,7 @@ def other_configs(log):
     bd = _dev.as_device(b, torch.float64)
     rel = lambda x: _ops.norm(_ops.residual(A, x, bd, mode="dot2")) / _ops.norm(bd)
     s = MPDOKSolver()
+    # kappa ~ 2e10 (nugget 1e-8): 47 bits (6 slices, fixed) to precondition
     kw = dict(tol=1e-12, maxiter_outer=20, precond="lu", factor_precision="ozaki", slices=6)
     s.solve(A, bd, **kw)
     x, t = timed(lambda: s.solve(A, bd, **kw))

@@ -106,7 +106,7 @@ class RBFValidationLoss(torch.autograd.Function):
                 None, None, None, None, None)
 
 
-def rbf_validation_loss(eps, log_sigma, X, y, Xv, yv, tol=1e-12, slices=6, precision="ozaki",
+def rbf_validation_loss(eps, log_sigma, X, y, Xv, yv, tol=1e-12, slices=5, precision="ozaki",
                         maxiter_outer=20, restart=50):
     """L(eps, log_sigma) as a differentiable 0-d tensor (see module doc).
 
