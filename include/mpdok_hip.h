@@ -68,10 +68,13 @@ const char* mpdok_last_error(void);
 /* Stream used by the reference-compatible symbols (NULL = null stream). */
 int mpdok_set_stream(hipStream_t s);
 /* Kernel-shape choices picked by the tuner. Key 0: block shape of the
- * int8-sliced GEMM (0 = 128 x 64 tiles, 2 workgroups per CU (default);
- * 1 = the same, 1 per CU; 2 = 128 x 128; 3 = four pipeline stages;
- * 4 = 128 x 128 with four stages). Key 1: row tiles per group of its tile
- * order (0 = 8). No reference counterpart. */
+ * int8-sliced GEMM, 0..14 (0 = pipelined 128 x 64 tiles, 2 workgroups per CU
+ * (default); 1-5 the unpipelined kernel's shapes; 6-9, 13 and 14 other
+ * pipelined shapes; 10-12 persistent kernels). Key 1: row tiles per group of its tile
+ * order (0 = 8). Key 2: start-up stagger of the persistent shapes (0..64).
+ * Key 3: 64 x 64 diagonal LU kernel (0 = blocked, 1 = unblocked). Key 4: row
+ * slicing (0 = two passes over a 2-D grid, 1 = one workgroup per 64 rows).
+ * No reference counterpart. */
 int mpdok_set_tuning(int key, int value);
 int mpdok_get_tuning(int key);
 size_t mpdok_reduce_workspace_bytes(int n);
@@ -231,9 +234,15 @@ int mpdok_lu_stats(void* handle, double* trailing_ms, double* trailing_ops, int*
  * minus `delta` bits. Decided on the device; no host synchronisation. No
  * reference counterpart (the reference factors in FP32). */
 int mpdok_lu_set_adaptive(void* handle, int on, int smin, int delta);
-/* Slice counts of the last factorisation, one per outer step (*nsteps of
- * them; at most n written). */
+/* Slice counts of the last factorisation, one per trailing update (*nsteps
+ * of them; at most n written). */
 int mpdok_lu_slice_counts(void* handle, int* out, int n, int* nsteps);
+/* Panels per trailing update of the int8-sliced variant on the lookahead
+ * path: 2 (default) factors the panels in pairs (right-looking inside the
+ * pair, FP64) and applies one update with K = 2 nb per pair, which reads and
+ * writes the trailing matrix half as often; 1 updates after every panel.
+ * No reference counterpart. */
+int mpdok_lu_set_update_panels(void* handle, int panels);
 /* Device pointers to the packed factors (column-major, ld N) and the row
  * permutation ((P b)[i] = b[perm[i]], NULL without pivoting). */
 int mpdok_lu_factors(void* handle, void** LU, int** perm, int* elem_size);

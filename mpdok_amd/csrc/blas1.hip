@@ -20,10 +20,12 @@ int tuning_set(int key, int value) {
   MPDOK_REQUIRE(key >= 0 && key < TUNE_COUNT, "tuning: unknown key %d", key);
   MPDOK_REQUIRE(key != TUNE_OZAKI_GROUP || (value >= 0 && value <= 256),
                 "tuning: group %d out of range", value);
-  MPDOK_REQUIRE(key != TUNE_OZAKI_SHAPE || (value >= 0 && value <= 12),
+  MPDOK_REQUIRE(key != TUNE_OZAKI_SHAPE || (value >= 0 && value <= 15),
                 "tuning: key %d out of range", key);
   MPDOK_REQUIRE(key != TUNE_OZAKI_STAGGER || (value >= 0 && value <= 64),
                 "tuning: stagger %d out of range", value);
+  MPDOK_REQUIRE((key != TUNE_DIAG_KERNEL && key != TUNE_SLICE_ROWS) || (value >= 0 && value <= 1),
+                "tuning: key %d takes 0 or 1, got %d", key, value);
   g_tuning[key] = value;
   return OK;
 }

@@ -32,6 +32,7 @@ enum TuneKey : int {
   TUNE_OZAKI_GROUP = 1,  // int8-sliced GEMM: row tiles per group of the tile order (0: 8)
   TUNE_OZAKI_STAGGER = 2,  // persistent int8-sliced GEMM: start-up stagger unit (x 8128 cycles)
   TUNE_DIAG_KERNEL = 3,    // 64 x 64 diagonal LU: 0 = blocked, 1 = unblocked
+  TUNE_SLICE_ROWS = 4,     // int8 row slicing: 0 = 2-D grid (two passes), 1 = one workgroup per 64 rows
   TUNE_COUNT = 8,
 };
 int tuning_get(int key);

@@ -64,6 +64,11 @@ struct LUHandle {
   // exponent sum, d_S[1 + step] the count of step `step`
   bool adaptive = true;
   int smin = 2, delta = 0;
+  // panels per trailing update on the lookahead path of the int8-sliced
+  // variant (1 or 2): with 2 the panels are factored in pairs and each update
+  // takes K = 2 nb, so C is read and written once per two panels. cur_upd:
+  // the value the running factorisation uses (1 off the lookahead path).
+  int upd = 1, cur_upd = 1;
   int* d_S = nullptr;
   std::vector<int> steps_S;  // host copy after a factorisation
   // optional HIP-event timing of the trailing-update launches
@@ -144,11 +149,12 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   alloc(&h->v1, (size_t)N * 8);
   alloc(&h->v2, (size_t)N * 8);
   if (precision == LU_OZAKI) {
-    alloc(&h->oz_work, ozaki_workspace_bytes(N, N, nb, slices));
-    alloc(&h->oz_work2, ozaki_workspace_bytes(N, N, nb, slices));
+    alloc(&h->oz_work, ozaki_workspace_bytes(N, N, 2 * nb, slices));
+    alloc(&h->oz_work2, ozaki_workspace_bytes(N, N, 2 * nb, slices));
     alloc((void**)&h->d_S, sizeof(int) * (N / nb + 3));
   }
   h->adaptive = precision == LU_OZAKI && slices <= 5;
+  h->upd = precision == LU_OZAKI ? 2 : 1;
   h->fast_solve = (N % TRSV_BLOCK) == 0;
   if (N >= TRSV_BLOCK) {  // also for a later lu_resize to a multiple of TRSV_BLOCK
     alloc(&h->Lbinv, (size_t)N * TRSV_BLOCK * h->es);
@@ -162,8 +168,10 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   }
   if (st == OK && pivot == PIV_NONE) {
     // the panel's scratch also serves the recursive doubling of its inverse
-    // block ((TRSV_BLOCK / 2)^2 elements)
-    const size_t wp = (size_t)N * 64 > (size_t)65536 ? (size_t)N * 64 : (size_t)65536;
+    // block ((TRSV_BLOCK / 2)^2 elements) and the block-row solve inside a
+    // pair of panels (nb^2)
+    size_t wp = (size_t)N * 64 > (size_t)65536 ? (size_t)N * 64 : (size_t)65536;
+    if (wp < (size_t)nb * nb) wp = (size_t)nb * nb;
     alloc(&h->Wp, wp * h->es);
     // default-priority stream (no priority is ever set on any stream)
     if (st == OK && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
@@ -183,7 +191,9 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
 
 // Slice workspace of the trailing update of step k (alternating, so that the
 // panel can slice the next step's rows while this step's update reads its own).
-static void* oz_ws(const LUHandle* h, int k) { return ((k / h->nb) & 1) ? h->oz_work2 : h->oz_work; }
+// Index of the trailing update that starts at column k.
+static int step_of(const LUHandle* h, int k) { return k / (h->nb * h->cur_upd); }
+static void* oz_ws(const LUHandle* h, int k) { return (step_of(h, k) & 1) ? h->oz_work2 : h->oz_work; }
 
 // Whether the panel [k, k + kb) is a whole TRSV_BLOCK diagonal block whose
 // inverses the handle keeps (built right after the panel: the block-row
@@ -213,6 +223,23 @@ static int build_block_inverses(LUHandle* h, hipStream_t s, int b0, int nbig, T*
   MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Uinv) + i64, Ub, nbig));
   for (int hh = 64; hh < TBk; hh *= 2) {
     const int64_t sW = (int64_t)hh * hh;
+    if (nbig == 1) {
+      // one big block: the TBk / (2 hh) pairs of this level are batched
+      // (uniform strides along the diagonal), four launches per level
+      const int np = TBk / (2 * hh);
+      const int64_t pLU = 2 * (int64_t)hh * (ld + 1), pB = 2 * (int64_t)hh * (TBk + 1);
+      const int64_t dD = (int64_t)hh * (TBk + 1);
+      const T* C = LU + hh;
+      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), C, ld, pLU, Lb, TBk, pB, T(0), W, hh, sW, np, 0));
+      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Lb + dD, TBk, pB, W, hh, sW, T(0), Lb + hh,
+                                TBk, pB, np, 0));
+      const T* Bm = LU + (int64_t)hh * ld;
+      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), Bm, ld, pLU, Ub + dD, TBk, pB, T(0), W, hh, sW,
+                                np, 0));
+      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Ub, TBk, pB, W, hh, sW, T(0),
+                                Ub + (int64_t)hh * TBk, TBk, pB, np, 0));
+      continue;
+    }
     for (int off = 0; off < TBk; off += 2 * hh) {
       const int64_t dA = (int64_t)off * (TBk + 1);         // inv(A) inside the big block
       const int64_t dD = (int64_t)(off + hh) * (TBk + 1);  // inv(D)
@@ -241,7 +268,7 @@ static int build_block_inverses(LUHandle* h, hipStream_t s, int b0, int nbig, T*
 // follow by recursive doubling: the block-row solve of the next step
 // multiplies by the L one. W: scratch (the side stream's, under lookahead).
 template <typename T>
-static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W) {
+static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W, bool slice = true) {
   const int N = h->N;
   T* A = static_cast<T*>(h->LU);
   T* Linv = static_cast<T*>(h->Linv);
@@ -278,18 +305,20 @@ static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W) {
     MPDOK_TRY(build_block_inverses<T>(h, s, k / TRSV_BLOCK, 1, W));
   // int8-sliced variant: the rows of L21 for this step's trailing update
   const int n3 = N - k - kb;
-  if (h->precision == LU_OZAKI && n3 > 0)
+  if (slice && h->precision == LU_OZAKI && n3 > 0)
     MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb, reinterpret_cast<const double*>(at(k + kb, k)),
                           ld, nullptr, ld, oz_ws(h, k), 1));
   return OK;
 }
 
-// U12 = inv(L11) A12 for the block row [k, k + kb), columns [c0, N): one
-// GEMM with the TRSV_BLOCK-wide inverse when the panel built it, else blocked
-// forward substitution with the 64 x 64 inverses.
+// U12 = inv(L11) A12 for the block row [k, k + kb), columns [c0, c0 + ncols)
+// (ncols < 0: up to N): one GEMM with the TRSV_BLOCK-wide inverse when the
+// panel built it, else blocked forward substitution with the 64 x 64
+// inverses. W: scratch of kb ncols elements.
 template <typename T>
-static int block_row_solve(LUHandle* h, hipStream_t s, int k, int kb, int c0, T* W) {
-  const int N = h->N, n3 = N - c0;
+static int block_row_solve(LUHandle* h, hipStream_t s, int k, int kb, int c0, T* W,
+                           int ncols = -1) {
+  const int N = h->N, n3 = ncols < 0 ? N - c0 : ncols;
   T* A = static_cast<T*>(h->LU);
   T* Linv = static_cast<T*>(h->Linv);
   const int64_t ld = N;
@@ -325,7 +354,7 @@ static int slice_step(LUHandle* h, hipStream_t s, int k, int kb, int n3, const d
   MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kb, nullptr, ld, U12, ld, oz_ws(h, k), 2));
   if (adaptive_on(h))
     MPDOK_TRY(ozaki_decide(s, h->slices, n3, n3, kb, oz_ws(h, k), h->smin, h->delta, k == 0,
-                           h->d_S, h->d_S + 1 + k / h->nb));
+                           h->d_S, h->d_S + 1 + step_of(h, k)));
   return OK;
 }
 
@@ -343,7 +372,7 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
   if (h->profile) {
     // int8-sliced: S(S+1)/2 int8 products of 2 m n k operations each
     h->ev_ops.push_back(2.0 * m3 * ncols * kb);
-    h->ev_step.push_back(k / h->nb);
+    h->ev_step.push_back(step_of(h, k));
     while (h->ev.size() < 2 * h->ev_ops.size()) {
       hipEvent_t e;
       MPDOK_HIP(hipEventCreate(&e));
@@ -354,7 +383,7 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
   if (adaptive_on(h)) {
     MPDOK_TRY(ozaki_apply_dyn(s, h->slices, m3, n3, kb, c_off, ncols,
                               reinterpret_cast<double*>(at(k + kb, k + kb)), ld, oz_ws(h, k),
-                              h->d_S + 1 + k / h->nb));
+                              h->d_S + 1 + step_of(h, k)));
   } else if (h->precision == LU_OZAKI) {
     MPDOK_TRY(ozaki_apply(s, h->slices, m3, n3, kb, c_off, ncols,
                           reinterpret_cast<double*>(at(k + kb, k + kb)), ld, oz_ws(h, k)));
@@ -366,6 +395,54 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
   return OK;
 }
 
+// Factor the columns [k, k + kbs) (cur_upd panels of nb columns) over rows
+// [k, N), right-looking inside the group: each panel, then the block-row
+// solve of its rows over the group's later columns and their FP64 update.
+// The int8-sliced variant then slices the rows of the group's L21 (K = kbs)
+// for the group's trailing update.
+template <typename T>
+static int panel_group_factor(LUHandle* h, hipStream_t s, int k, int kbs, T* W) {
+  const int N = h->N, nb = h->nb;
+  T* A = static_cast<T*>(h->LU);
+  const int64_t ld = N;
+  auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
+  for (int j = k; j < k + kbs; j += nb) {
+    const int jb = nb < k + kbs - j ? nb : k + kbs - j;
+    MPDOK_TRY(panel_factor<T>(h, s, j, jb, W, false));
+    const int rem = k + kbs - j - jb;
+    if (rem > 0) {
+      MPDOK_TRY(block_row_solve<T>(h, s, j, jb, j + jb, W, rem));
+      MPDOK_TRY(gemm<T>(s, N - j - jb, rem, jb, T(-1), at(j + jb, j), ld, at(j, j + jb), ld, T(1),
+                        at(j + jb, j + jb), ld, 0));
+    }
+  }
+  const int n3 = N - k - kbs;
+  if (h->precision == LU_OZAKI && n3 > 0)
+    MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kbs,
+                          reinterpret_cast<const double*>(at(k + kbs, k)), ld, nullptr, ld,
+                          oz_ws(h, k), 1));
+  return OK;
+}
+
+// U12 of the block rows [k, k + kbs) over columns [c0, N), panel by panel:
+// each block row's solve, then the update of the group's block rows below it.
+template <typename T>
+static int block_rows_solve(LUHandle* h, hipStream_t s, int k, int kbs, int c0, T* W) {
+  const int N = h->N, nb = h->nb;
+  T* A = static_cast<T*>(h->LU);
+  const int64_t ld = N;
+  auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
+  for (int j = k; j < k + kbs; j += nb) {
+    const int jb = nb < k + kbs - j ? nb : k + kbs - j;
+    MPDOK_TRY(block_row_solve<T>(h, s, j, jb, c0, W));
+    const int rem = k + kbs - j - jb;
+    if (rem > 0)
+      MPDOK_TRY(gemm<T>(s, rem, N - c0, jb, T(-1), at(j + jb, j), ld, at(j, c0), ld, T(1),
+                        at(j + jb, c0), ld, 0));
+  }
+  return OK;
+}
+
 template <typename T>
 static int factor_impl(LUHandle* h, hipStream_t s) {
   const int N = h->N, nb = h->nb;
@@ -374,29 +451,34 @@ static int factor_impl(LUHandle* h, hipStream_t s) {
   const int64_t ld = N;
   auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
   const bool look = h->lookahead && h->pivot == PIV_NONE && h->side != nullptr;
+  h->cur_upd = look && h->precision == LU_OZAKI ? h->upd : 1;
   if (look) {
     // Step k: [main] U12 of block row k, slicing, trailing update of the next
     // panel's columns; [side] factor the next panel as soon as its columns are
     // final; [main] the rest of the trailing update meanwhile. The panel
     // touches only columns [k+kb, k+2kb) and its own scratch Wp; the rest of
     // the trailing update only columns >= k+2kb.
+    // steps of gw = cur_upd nb columns (a group of panels per trailing update)
     T* Wp = static_cast<T*>(h->Wp);
-    MPDOK_TRY(panel_factor<T>(h, s, 0, nb < N ? nb : N, W));
-    for (int k = 0; k < N; k += nb) {
-      const int kb = nb < N - k ? nb : N - k;
+    const int gw = h->cur_upd * nb;
+    MPDOK_TRY(panel_group_factor<T>(h, s, 0, gw < N ? gw : N, W));
+    for (int k = 0; k < N; k += gw) {
+      const int kb = gw < N - k ? gw : N - k;
       const int n3 = N - k - kb;
       if (n3 <= 0) break;
       if (k > 0) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
-      MPDOK_TRY(block_row_solve<T>(h, s, k, kb, k + kb, W));
+      MPDOK_TRY(block_rows_solve<T>(h, s, k, kb, k + kb, W));
       if (h->precision == LU_OZAKI)
         MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k, k + kb)), ld));
-      const int nk = nb < n3 ? nb : n3;
+      const int nk = gw < n3 ? gw : n3;
       MPDOK_TRY(trailing_cols<T>(h, s, k, kb, 0, nk));
       MPDOK_HIP(hipEventRecord(h->ev_next_cols, s));
-      MPDOK_HIP(hipStreamWaitEvent(h->side, h->ev_next_cols, 0));
-      MPDOK_TRY(panel_factor<T>(h, h->side, k + kb, nk, Wp));
-      MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
+      // the rest of the update is enqueued before the panel's launches, so
+      // that it never waits for the host to issue them
       MPDOK_TRY(trailing_cols<T>(h, s, k, kb, nk, n3 - nk));
+      MPDOK_HIP(hipStreamWaitEvent(h->side, h->ev_next_cols, 0));
+      MPDOK_TRY(panel_group_factor<T>(h, h->side, k + kb, nk, Wp));
+      MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
     }
     if (N > nb) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
     if (h->fast_solve && nb != TRSV_BLOCK)
@@ -436,7 +518,7 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
     MPDOK_TRY(factor_impl<float>(h, s));
   int hinfo = 0;
   MPDOK_HIP(hipMemcpyAsync(&hinfo, h->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-  const int nsteps = (N + h->nb - 1) / h->nb;
+  const int nsteps = (N + h->nb * h->cur_upd - 1) / (h->nb * h->cur_upd);
   h->steps_S.assign(nsteps, h->precision == LU_OZAKI ? h->slices : 0);
   if (adaptive_on(h) && N > h->nb)
     MPDOK_HIP(hipMemcpyAsync(h->steps_S.data(), h->d_S + 1, sizeof(int) * (nsteps - 1),
@@ -592,7 +674,20 @@ MPDOK_API int mpdok_lu_set_adaptive(void* hv, int on, int smin, int delta) {
   }
   return 0;
 }
-// Slice counts the last factorisation used, one per step (n of them at most).
+// Panels per trailing update on the lookahead path of the int8-sliced
+// variant: 1, or 2 (the default: panels factored in pairs, one update with
+// K = 2 nb per pair).
+MPDOK_API int mpdok_lu_set_update_panels(void* hv, int panels) {
+  auto* h = static_cast<mpdok::LUHandle*>(hv);
+  MPDOK_REQUIRE(h != nullptr, "lu_set_update_panels: null handle");
+  MPDOK_REQUIRE(panels == 1 || panels == 2, "lu_set_update_panels: panels must be 1 or 2, got %d",
+                panels);
+  MPDOK_REQUIRE(panels == 1 || h->precision == mpdok::LU_OZAKI,
+                "lu_set_update_panels: panel pairs need the int8-sliced variant");
+  h->upd = panels;
+  return 0;
+}
+// Slice counts the last factorisation used, one per trailing update (n of them at most).
 MPDOK_API int mpdok_lu_slice_counts(void* hv, int* out, int n, int* nsteps) {
   auto* h = static_cast<mpdok::LUHandle*>(hv);
   MPDOK_REQUIRE(h != nullptr, "lu_slice_counts: null handle");

@@ -127,6 +127,72 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
   }
 }
 
+// Row slicing over a 2-D grid, for panels too short to fill the chip with
+// one workgroup per 64 rows. Pass 1: partial maxima of |A| over each 64 x 64
+// tile, pm[kblock * mp + row]. Pass 2: each 64 x 64 tile takes its rows'
+// exponents from the partial maxima (the same maximum, so the same digits as
+// slice_rows_kernel) and writes its records; tiles of k block 0 write ea.
+__global__ __launch_bounds__(256) void rowmax_part_kernel(const double* __restrict__ A,
+                                                          int64_t lda, int m, int K, int mp,
+                                                          double* __restrict__ pm) {
+  __shared__ double red[4][64];
+  const int t = threadIdx.x, r = t & 63, p = t >> 6;
+  const int i = blockIdx.x * 64 + r, k0 = blockIdx.y * 64;
+  const int k1 = K < k0 + 64 ? K : k0 + 64;
+  double mx = 0.0;
+  if (i < m)
+#pragma unroll 4
+    for (int k = k0 + p; k < k1; k += 4) mx = fmax(mx, fabs(A[i + (int64_t)k * lda]));
+  red[p][r] = mx;
+  __syncthreads();
+  if (t < 64)
+    pm[(size_t)blockIdx.y * mp + blockIdx.x * 64 + t] =
+        fmax(fmax(red[0][t], red[1][t]), fmax(red[2][t], red[3][t]));
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void slice_rows_tile_kernel(const double* __restrict__ A,
+                                                              int64_t lda, int m, int K, int mp,
+                                                              int Kp, const double* __restrict__ pm,
+                                                              signed char* __restrict__ As,
+                                                              int* __restrict__ ea) {
+  __shared__ double tile[64][65];
+  __shared__ int se[64];
+  const int t = threadIdx.x;
+  const int row0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  if (t < 64) {
+    double mx = 0.0;
+    for (int b = 0; b < Kp / 64; ++b) mx = fmax(mx, pm[(size_t)b * mp + row0 + t]);
+    const int e = oz_exponent(mx);
+    se[t] = e;
+    if (blockIdx.y == 0) ea[row0 + t] = e;
+  }
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int rr = e & 63, kk = e >> 6;
+    const int ii = row0 + rr, k = k0 + kk;
+    tile[rr][kk] = (ii < m && k < K) ? A[ii + (int64_t)k * lda] : 0.0;
+  }
+  __syncthreads();
+  // thread -> (row rr, 16 consecutive k); rows < mp and k < Kp by the grid
+  const int rr = t >> 2, kq = (t & 3) * 16;
+  const int ii = row0 + rr;
+  const size_t slab = (size_t)mp * Kp;
+  signed char dig[S][16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    signed char d[S];
+    make_digits<S>(tile[rr][kq + c], se[rr], d);
+#pragma unroll
+    for (int q = 0; q < S; ++q) dig[q][c] = d[q];
+  }
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    v4i w;
+    __builtin_memcpy(&w, dig[q], 16);
+    *reinterpret_cast<v4i*>(As + q * slab + oz_rec(mp, ii, k0 + kq)) = w;
+  }
+}
+
 // Columns of B (K x n, column-major, ldb) -> Bs[q] (K-blocked); eb[col].
 template <int S>
 __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restrict__ B, int64_t ldb,
@@ -1000,13 +1066,15 @@ static int oz_kp(int K) { return (K + 4 * OZ_BK - 1) / (4 * OZ_BK) * (4 * OZ_BK)
 
 size_t ozaki_workspace_bytes(int m, int n, int K, int S) {
   const size_t mp = oz_mp(m), np = oz_np(n), Kp = oz_kp(K);
-  return (size_t)S * Kp * (mp + np) + sizeof(int) * (mp + np) + 256;
+  return (size_t)S * Kp * (mp + np) + sizeof(int) * (mp + np) + sizeof(double) * (Kp / 64) * mp +
+         256;
 }
 
 struct OzLayout {
   int mp, np, Kp;
   signed char *As, *Bs;
   int *ea, *eb;
+  double* pm;  // partial row maxima of the 2-D row slicing, (Kp / 64) x mp
 };
 static OzLayout oz_layout(int S, int m, int n, int K, void* work) {
   OzLayout L;
@@ -1017,6 +1085,7 @@ static OzLayout oz_layout(int S, int m, int n, int K, void* work) {
   L.Bs = L.As + (size_t)S * L.mp * L.Kp;
   L.ea = reinterpret_cast<int*>(L.Bs + (size_t)S * L.np * L.Kp);
   L.eb = L.ea + L.mp;
+  L.pm = reinterpret_cast<double*>(L.eb + L.np);  // (mp + np) ints: a multiple of 8 bytes
   return L;
 }
 
@@ -1024,7 +1093,15 @@ template <int S>
 static int ozaki_slice_s(hipStream_t s, int m, int n, int K, const double* A, int64_t lda,
                          const double* B, int64_t ldb, void* work, int parts) {
   const OzLayout L = oz_layout(S, m, n, K, work);
-  if (parts & 1) slice_rows_kernel<S><<<L.mp / 64, 256, 0, s>>>(A, lda, m, K, L.mp, L.Kp, L.As, L.ea);
+  if (parts & 1) {
+    if (tuning_get(TUNE_SLICE_ROWS) == 1) {
+      slice_rows_kernel<S><<<L.mp / 64, 256, 0, s>>>(A, lda, m, K, L.mp, L.Kp, L.As, L.ea);
+    } else {
+      const dim3 g(L.mp / 64, L.Kp / 64);
+      rowmax_part_kernel<<<g, 256, 0, s>>>(A, lda, m, K, L.mp, L.pm);
+      slice_rows_tile_kernel<S><<<g, 256, 0, s>>>(A, lda, m, K, L.mp, L.Kp, L.pm, L.As, L.ea);
+    }
+  }
   if (parts & 2) slice_cols_kernel<S><<<L.np / 4, 256, 0, s>>>(B, ldb, n, K, L.np, L.Kp, L.Bs, L.eb);
   MPDOK_HIP(hipGetLastError());
   return OK;
@@ -1125,6 +1202,15 @@ static int ozaki_apply_s(hipStream_t s, int m, int n, int K, int col0, int ncols
       else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
     case 11:  // persistent, 4 wavefronts of 64 x 32, two stages, two workgroups per CU
       if constexpr (S <= 5) return ozaki_launchq<S, 2, 2, 2, 1, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 13:  // pipelined, four stages, two workgroups per CU (S <= 3)
+      if constexpr (S <= 3) return ozaki_launchp<S, 2, 2, 2, 1, 4, 2>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 14:  // pipelined, two stages, three workgroups per CU (S <= 3)
+      if constexpr (S <= 3) return ozaki_launchp<S, 2, 2, 2, 1, 2, 3>(s, L, m, n, col0, ncols, C, ldc);
+      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
+    case 15:  // pipelined, 4 wavefronts of 64 x 64, two stages, two workgroups per CU (S <= 3)
+      if constexpr (S <= 3) return ozaki_launchp<S, 2, 2, 2, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
       else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
     case 12:  // persistent, 4 wavefronts of 64 x 64 (S <= 4), four stages
       if constexpr (S <= 4) return ozaki_launchq<S, 2, 2, 2, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);

@@ -123,8 +123,14 @@ class _LUHandle:
         check(self._lib.mpdok_lu_set_adaptive(self.ptr, 1 if on else 0, int(smin), int(delta)),
               "lu_set_adaptive")
 
+    def set_update_panels(self, panels):
+        """Panels per trailing update of the int8-sliced variant (see
+        mpdok_lu_set_update_panels): 2 factors them in pairs and updates once
+        per pair with K = 2 nb; 1 updates after every panel."""
+        check(self._lib.mpdok_lu_set_update_panels(self.ptr, int(panels)), "lu_set_update_panels")
+
     def slice_counts(self):
-        """Slice counts of the last factorisation, one per outer step."""
+        """Slice counts of the last factorisation, one per trailing update."""
         out = (ctypes.c_int * 4096)()
         n = ctypes.c_int(0)
         check(self._lib.mpdok_lu_slice_counts(self.ptr, out, 4096, ctypes.byref(n)),

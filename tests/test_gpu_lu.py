@@ -91,7 +91,7 @@ def test_ozaki_gemm_bit_exact_vs_oracle(L, S, m, n, K):
     assert np.array_equal(Cd.cpu().numpy(), ref)
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14])
 @pytest.mark.parametrize("S,m,n,K", [(5, 257, 129, 300), (6, 640, 384, 512), (3, 130, 260, 96),
                                      (4, 300, 520, 640)])
 def test_ozaki_gemm_block_shapes_bit_exact(L, shape, S, m, n, K):
@@ -110,6 +110,31 @@ def test_ozaki_gemm_block_shapes_bit_exact(L, shape, S, m, n, K):
         torch.cuda.synchronize()
     finally:
         L.mpdok_set_tuning(0, 0)
+    assert st == 0, L.mpdok_last_error()
+    assert np.array_equal(Cd.cpu().numpy(), C - olu.ozaki_matmul(A, B, S))
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("S,m,n,K", [(5, 700, 90, 512), (3, 130, 64, 200), (6, 64, 64, 1)])
+def test_ozaki_row_slicing_variants_bit_exact(L, variant, S, m, n, K):
+    """Both row-slicing kernels (tuning key 4: two passes over a 2-D grid, or
+    one workgroup per 64 rows) give the oracle's bits, including all-zero
+    rows and rows whose maximum sits in the last 64-column block."""
+    rng = np.random.default_rng(11 * S + m + variant)
+    A = rng.standard_normal((m, K)) * np.exp(rng.uniform(-6, 6, (m, 1)))
+    A[::7] = 0.0
+    A[3::11, -1] = 1e6
+    B = rng.standard_normal((K, n))
+    C = rng.standard_normal((m, n))
+    ws = torch.empty(L.mpdok_ozaki_workspace_bytes(m, n, K, S), dtype=torch.uint8, device="cuda")
+    Ad, Bd, Cd = dev(A), dev(B), dev(C)
+    assert L.mpdok_set_tuning(4, variant) == 0
+    try:
+        st = L.mpdok_ozaki_gemm_sub(stream(), S, m, n, K, Ad.data_ptr(), m, Bd.data_ptr(), K,
+                                    Cd.data_ptr(), m, ws.data_ptr())
+        torch.cuda.synchronize()
+    finally:
+        L.mpdok_set_tuning(4, 0)
     assert st == 0, L.mpdok_last_error()
     assert np.array_equal(Cd.cpu().numpy(), C - olu.ozaki_matmul(A, B, S))
 
@@ -138,7 +163,8 @@ def _ozaki_adaptive(L, A, B, C, smax, s_fixed=0, smin=2, delta=0, e0=None):
 
 
 @pytest.mark.parametrize("smax", [3, 4, 5])
-@pytest.mark.parametrize("m,n,K", [(100, 70, 64), (257, 129, 300), (640, 512, 512)])
+@pytest.mark.parametrize("m,n,K", [(100, 70, 64), (257, 129, 300), (640, 512, 512),
+                                   (2047, 1100, 64)])
 def test_ozaki_leading_slices_bit_exact_vs_oracle(L, smax, m, n, K):
     """The runtime-count kernel multiplies the leading S of smax digits: bit for
     bit the oracle's ozaki_matmul_lead for every S in [2, smax]."""
@@ -172,22 +198,28 @@ def test_ozaki_decision_matches_oracle(L):
         assert np.array_equal(out, C - olu.ozaki_matmul_lead(A, Bk, 5, want))
 
 
-def test_ozaki_adaptive_lu_counts_follow_its_own_panels(L):
-    """Adaptive factorisation of a decaying RBF matrix: each step's count is
-    the oracle's decision on the very panels the library sliced (read back
-    from its factors), the counts drop below 5, and the backward error stays
-    within a small factor of the fixed 5-slice factorisation's."""
-    N, nb = 2048, 256
+@pytest.mark.parametrize("panels,nb", [(1, 256), (2, 128)])
+def test_ozaki_adaptive_lu_counts_follow_its_own_panels(L, panels, nb):
+    """Adaptive factorisation of a decaying RBF matrix, one or two panels per
+    trailing update: each update's count is the oracle's decision on the very
+    L21 and U12 the library sliced (read back from its factors; K = panels nb),
+    the counts drop below 5, and the backward error stays within a small
+    factor of the fixed 5-slice factorisation's."""
+    N = 2048
     X = np.random.default_rng(42).uniform(0, 1, (N, 2))
     A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
-    h, info = _factor(L, A, "ozaki", "none", nb, 5, adaptive=True)
+    from mpdok_amd.mpdok_ops import _LUHandle
+    h = _LUHandle(L, N, "ozaki", "none", nb, 5, True)
+    h.set_update_panels(panels)
+    info = h.factor(dev(A))
     assert info == 0
     counts = h.slice_counts()
     LU, _ = _factors(L, h, N)
     e0 = None
     want = []
-    for k in range(0, N - nb, nb):
-        S, e0 = olu.ozaki_decide(LU[k + nb:, k:k + nb], LU[k:k + nb, k + nb:], 5, 2, 0, e0)
+    g = panels * nb
+    for k in range(0, N - g, g):
+        S, e0 = olu.ozaki_decide(LU[k + g:, k:k + g], LU[k:k + g, k + g:], 5, 2, 0, e0)
         want.append(S)
     assert counts[:len(want)] == want
     assert want[0] == 5 and min(want) < 5
@@ -266,17 +298,22 @@ def test_lu_singular_reports_info(L):
 
 
 @pytest.mark.parametrize("S", [4, 5, 6])
-def test_ozaki_lu_matches_oracle_on_well_conditioned_spd(L, S):
+@pytest.mark.parametrize("panels", [1, 2])
+def test_ozaki_lu_matches_oracle_on_well_conditioned_spd(L, S, panels):
     # Factors of a well-conditioned matrix are well-conditioned functions of
     # the rounding, so the GPU factors must match the oracle's restatement.
+    # With two panels per update the pair is factored in FP64 and the update
+    # takes K = 512: the oracle's blocked LU with 512-column panels.
     N = 1024
     rng = np.random.default_rng(S)
     X = rng.standard_normal((N, N))
     A = X.T @ X / N + np.eye(N)
-    h, info = _factor(L, A, "ozaki", "none", 256, S)
-    assert info == 0
+    from mpdok_amd.mpdok_ops import _LUHandle
+    h = _LUHandle(L, N, "ozaki", "none", 256, S, False)
+    h.set_update_panels(panels)
+    assert h.factor(dev(A)) == 0
     LU, _ = _factors(L, h, N)
-    ref = olu.lu_nopiv(A, nb=256, trailing=lambda a, b: olu.ozaki_matmul(a, b, S))
+    ref = olu.lu_nopiv(A, nb=256 * panels, trailing=lambda a, b: olu.ozaki_matmul(a, b, S))
     # panel roundings differ (FP64 order), and a last digit may flip where an
     # input sits on a rounding boundary: allow a few units of 2^(-(8 S - 1))
     assert np.abs(LU - ref).max() < 2.0 ** (-(8 * S - 1) + 6) * np.abs(ref).max()
