@@ -332,19 +332,34 @@ __global__ __launch_bounds__(256) void diag_factor_kernel(T* __restrict__ A, int
   }
 }
 
+// Broadcast of lane `src`'s value to the whole wavefront (through SGPRs).
+__device__ __forceinline__ double lane_bcast(double x, int src) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_bcast(float x, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+
 // Blocked variant (the default), same contract and the same 34 KB of LDS:
 //  1. LU in four 16-column sub-panels p: wavefront 0 eliminates the
-//     sub-panel (rows 16 p .. 63, lane r = row r, wavefront-level syncs
-//     only); the U12 rows of the sub-panel are solved against its unit-lower
-//     16 x 16 triangle (one thread per column); the rest of the block takes a
-//     rank-16 update. Three workgroup barriers per sub-panel.
+//     sub-panel with lane r holding row r's 16 entries in registers and the
+//     pivot row broadcast from its lane (no LDS round trips inside the
+//     elimination; the operations and their order are those of the
+//     unblocked kernel's elimination); the U12 rows of the sub-panel are
+//     solved against its unit-lower 16 x 16 triangle (one thread per
+//     column); the rest of the block takes a rank-16 update. Three
+//     workgroup barriers per sub-panel.
 //  2. Inverses in place: the four 16 x 16 diagonal blocks of U (wavefront 0)
 //     and of the unit-lower L (wavefront 1), one lane per column, by
 //     substitution in registers; then two levels of recursive doubling,
 //       inv([A B; 0 D]) = [inv(A) -inv(A) B inv(D); 0 inv(D)]
 //       inv([A 0; C D]) = [inv(A) 0; -inv(D) C inv(A) inv(D)],
-//     16 -> 32 -> 64, each off-diagonal block formed as two products staged
-//     through registers (T = inv(A) B, then -T inv(D), written back in place).
+//     16 -> 32 -> 64, each off-diagonal block formed as two products on the
+//     matrix cores (16 x 16 output tiles, triangular operands masked on
+//     their way out of LDS), staged in registers and written back in place.
 // The rounding differs from the unblocked kernel's (other summation orders).
 template <typename T>
 __global__ __launch_bounds__(256) void diag_factor_blk_kernel(T* __restrict__ A, int64_t lda,
@@ -359,27 +374,46 @@ __global__ __launch_bounds__(256) void diag_factor_blk_kernel(T* __restrict__ A,
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-  // identity outside nb x nb, so the blocked steps need no bounds
-  for (int e = t; e < 64 * 64; e += 256) {
-    const int r = e & 63, c = e >> 6;
-    s[r][c] = (r < nb && c < nb) ? A[r + (int64_t)c * lda] : (r == c ? T(1) : T(0));
+  // identity outside nb x nb, so the blocked steps need no bounds; the 16
+  // loads of a thread go out together
+  {
+    T v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = t + 256 * q, r = e & 63, c = e >> 6;
+      v[q] = (r < nb && c < nb) ? A[r + (int64_t)c * lda] : (r == c ? T(1) : T(0));
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = t + 256 * q;
+      s[e & 63][e >> 6] = v[q];
+    }
   }
   __syncthreads();
   if (factor) {
+#pragma unroll
     for (int p = 0; p < 4; ++p) {
       const int c0 = 16 * p;
       if (w == 0) {
         const int r = lane;
-        for (int j = c0; j < c0 + 16; ++j) {
-          const T piv = s[j][j];
-          if (r == 0 && j < nb && piv == T(0)) atomicCAS(info, 0, col0 + j + 1);
-          if (r > j) {
-            const T l = s[r][j] / piv;
-            s[r][j] = l;
-            for (int c = j + 1; c < c0 + 16; ++c) s[r][c] = __builtin_fma(-l, s[j][c], s[r][c]);
+        T a[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) a[c] = s[r][c0 + c];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          T u[16];  // pivot row c0 + j, columns j .. 15 of the sub-panel
+#pragma unroll
+          for (int c = j; c < 16; ++c) u[c] = lane_bcast(a[c], c0 + j);
+          if (r == 0 && c0 + j < nb && u[j] == T(0)) atomicCAS(info, 0, col0 + c0 + j + 1);
+          if (r > c0 + j) {
+            const T l = a[j] / u[j];
+            a[j] = l;
+#pragma unroll
+            for (int c = j + 1; c < 16; ++c) a[c] = __builtin_fma(-l, u[c], a[c]);
           }
-          wave_sync();
         }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s[r][c0 + c] = a[c];
       }
       __syncthreads();
       if (p == 3) break;
@@ -447,62 +481,74 @@ __global__ __launch_bounds__(256) void diag_factor_blk_kernel(T* __restrict__ A,
     }
   }
   __syncthreads();
-  // ---- recursive doubling: h = 16, 32 ----
+  // ---- recursive doubling on the matrix cores: h = 16, 32 ----
+  // Work units: (triangle, pair, 16 x 16 output tile), wavefront w takes
+  // units w and w + 4. Lane layout of the 16 x 16 x 4 MFMA: A[l % 16][l / 16],
+  // B[l / 16][l % 16], D[Mfma<T>::row(l, r)][l % 16].
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
   for (int h = 16; h < 64; h *= 2) {
-    const int npair = 32 / h;               // pairs per triangle at this level
-    const int per = h * h, total = 2 * npair * per;  // off-diagonal elements, U and L
-    constexpr int MAXE = 8;                 // total / 256 <= 8
-    T v[MAXE];
-    // T = inv(A) B (U), T = C inv(A) (L): staged in registers
-    for (int q = 0; q < MAXE; ++q) {
-      const int e = t + 256 * q;
-      if (e >= total) break;
-      const int tri = e / (npair * per), pr = (e / per) % npair, idx = e % per;
-      const int i = idx % h, j = idx / h, o = 2 * h * pr;
-      T acc = T(0);
-      if (tri == 0) {  // U: (inv(A) B)_ij = sum_{k >= i} invA_ik B_kj
-        for (int k = i; k < h; ++k) acc = __builtin_fma(s[o + i][o + k], s[o + k][o + h + j], acc);
-      } else {  // L: (C inv(A))_ij = sum_{k <= j... } C_ik invA_kj, invA unit lower
-        acc = s[o + h + i][o + j];
-        for (int k = j + 1; k < h; ++k) acc = __builtin_fma(s[o + h + i][o + k], s[o + k][o + j], acc);
+    const int nt = h / 16, npair = 32 / h, per = npair * nt * nt, units = 2 * per;
+#pragma unroll
+    for (int phase = 0; phase < 2; ++phase) {
+      // phase 0: U: P = inv(A) B             L: P = C + C inv_s(A)
+      // phase 1: U: X = -(P inv(D))          L: X = -(P + inv_s(D) P)
+      // (inv_s: the strictly lower part of a unit-lower inverse)
+      acc_t res[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int u = w + 4 * q;
+        res[q] = acc_t(0);
+        if (u >= units) continue;
+        const int tri = u / per, pr = (u / (nt * nt)) % npair, tile = u % (nt * nt);
+        const int o = 2 * h * pr, ti = 16 * (tile % nt), tj = 16 * (tile / nt);
+        acc_t acc = acc_t(0);
+        if (tri == 1) {  // L: the added term (C, or P)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[r] = s[o + h + ti + Mfma<T>::row(lane, r)][o + tj + li];
+        }
+        const int ia = ti + li, jb = tj + li;  // A row, B column of this lane
+        for (int k0 = 0; k0 < h; k0 += 4) {
+          const int k = k0 + lk;
+          T a, b;
+          if (tri == 0) {
+            if (phase == 0) {
+              a = k >= ia ? s[o + ia][o + k] : T(0);
+              b = s[o + k][o + h + jb];
+            } else {
+              a = s[o + ia][o + h + k];
+              b = k <= jb ? s[o + h + k][o + h + jb] : T(0);
+            }
+          } else {
+            if (phase == 0) {
+              a = s[o + h + ia][o + k];
+              b = k > jb ? s[o + k][o + jb] : T(0);
+            } else {
+              a = k < ia ? s[o + h + ia][o + h + k] : T(0);
+              b = s[o + h + k][o + jb];
+            }
+          }
+          acc = Mfma<T>::mma(a, b, acc);
+        }
+        res[q] = phase == 0 ? acc : -acc;
       }
-      v[q] = acc;
-    }
-    __syncthreads();
-    for (int q = 0; q < MAXE; ++q) {
-      const int e = t + 256 * q;
-      if (e >= total) break;
-      const int tri = e / (npair * per), pr = (e / per) % npair, idx = e % per;
-      const int i = idx % h, j = idx / h, o = 2 * h * pr;
-      if (tri == 0) s[o + i][o + h + j] = v[q];
-      else s[o + h + i][o + j] = v[q];
-    }
-    __syncthreads();
-    // X = -T inv(D) (U), X = -inv(D) T (L)
-    for (int q = 0; q < MAXE; ++q) {
-      const int e = t + 256 * q;
-      if (e >= total) break;
-      const int tri = e / (npair * per), pr = (e / per) % npair, idx = e % per;
-      const int i = idx % h, j = idx / h, o = 2 * h * pr;
-      T acc = T(0);
-      if (tri == 0) {  // sum_{k <= j} T_ik invD_kj
-        for (int k = 0; k <= j; ++k) acc = __builtin_fma(s[o + i][o + h + k], s[o + h + k][o + h + j], acc);
-      } else {  // sum_{k <= i} invD_ik T_kj, invD unit lower
-        acc = s[o + h + i][o + j];
-        for (int k = 0; k < i; ++k) acc = __builtin_fma(s[o + h + i][o + h + k], s[o + h + k][o + j], acc);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int u = w + 4 * q;
+        if (u >= units) continue;
+        const int tri = u / per, pr = (u / (nt * nt)) % npair, tile = u % (nt * nt);
+        const int o = 2 * h * pr, ti = 16 * (tile % nt), tj = 16 * (tile / nt);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = ti + Mfma<T>::row(lane, r), j = tj + li;
+          if (tri == 0) s[o + i][o + h + j] = res[q][r];
+          else s[o + h + i][o + j] = res[q][r];
+        }
       }
-      v[q] = -acc;
+      __syncthreads();
     }
-    __syncthreads();
-    for (int q = 0; q < MAXE; ++q) {
-      const int e = t + 256 * q;
-      if (e >= total) break;
-      const int tri = e / (npair * per), pr = (e / per) % npair, idx = e % per;
-      const int i = idx % h, j = idx / h, o = 2 * h * pr;
-      if (tri == 0) s[o + i][o + h + j] = v[q];
-      else s[o + h + i][o + j] = v[q];
-    }
-    __syncthreads();
   }
   for (int e = t; e < 64 * 64; e += 256) {
     const int r = e & 63, c = e >> 6;

@@ -139,6 +139,30 @@ def test_ozaki_row_slicing_variants_bit_exact(L, variant, S, m, n, K):
     assert np.array_equal(Cd.cpu().numpy(), C - olu.ozaki_matmul(A, B, S))
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("N", [17, 64, 130, 512])
+def test_diag_block_kernels_factor_and_inverses(L, variant, N):
+    """Both 64 x 64 diagonal-block kernels (tuning key 3: blocked with matrix-core
+    doubling, or unblocked): with nb = 64 every panel step is one diagonal
+    block, so the packed factors must match the oracle's unpivoted LU, and the
+    solves, which apply the stored 64 x 64 inverses (and, at N = 512, the
+    512-wide ones doubled from them), must reproduce numpy's solution."""
+    rng = np.random.default_rng(N + 10 * variant)
+    A = rng.standard_normal((N, N)) + N * np.eye(N)
+    assert L.mpdok_set_tuning(3, variant) == 0
+    try:
+        h, info = _factor(L, A, "fp64", "none", 64)
+        assert info == 0
+        LU, _ = _factors(L, h, N)
+        b = rng.standard_normal(N)
+        x = h.solve(dev(b)).cpu().numpy()
+    finally:
+        L.mpdok_set_tuning(3, 0)
+    ref = olu.lu_nopiv(A, nb=64)
+    assert np.abs(LU - ref).max() < 1e-13 * np.abs(ref).max()
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-12, atol=1e-13)
+
+
 def _factor(L, A, precision, pivot, nb=128, slices=6, adaptive=False):
     from mpdok_amd.mpdok_ops import _LUHandle
     h = _LUHandle(L, A.shape[0], precision, pivot, nb, slices, adaptive)
