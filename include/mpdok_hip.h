@@ -238,11 +238,12 @@ int mpdok_lu_set_adaptive(void* handle, int on, int smin, int delta);
  * of them; at most n written). */
 int mpdok_lu_slice_counts(void* handle, int* out, int n, int* nsteps);
 /* Panels per trailing update of the int8-sliced variant on the lookahead
- * path: 2 (default) factors the panels in pairs (right-looking inside the
- * pair, FP64) and applies one update with K = 2 nb per pair, which reads and
- * writes the trailing matrix half as often; 1 updates after every panel.
- * No reference counterpart. */
-int mpdok_lu_set_update_panels(void* handle, int panels);
+ * path: groups of `panels` (1 or 2), and of `panels_early` (panels .. 4)
+ * while the trailing matrix has at least early_rows rows. A group is
+ * factored right-looking inside (FP64) and its trailing update takes
+ * K = (panels in the group) nb, so the trailing matrix is read and written
+ * once per group. Defaults: 2, 2, 0. No reference counterpart. */
+int mpdok_lu_set_update_panels(void* handle, int panels, int panels_early, int early_rows);
 /* Device pointers to the packed factors (column-major, ld N) and the row
  * permutation ((P b)[i] = b[perm[i]], NULL without pivoting). */
 int mpdok_lu_factors(void* handle, void** LU, int** perm, int* elem_size);

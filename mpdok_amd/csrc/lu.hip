@@ -65,10 +65,15 @@ struct LUHandle {
   bool adaptive = true;
   int smin = 2, delta = 0;
   // panels per trailing update on the lookahead path of the int8-sliced
-  // variant (1 or 2): with 2 the panels are factored in pairs and each update
-  // takes K = 2 nb, so C is read and written once per two panels. cur_upd:
-  // the value the running factorisation uses (1 off the lookahead path).
-  int upd = 1, cur_upd = 1;
+  // variant: groups of `upd` panels (1 or 2), and of `upd_early` (up to
+  // MAX_GROUP) while the trailing matrix has at least early_rows rows. A
+  // group is factored right-looking inside (FP64) and its trailing update
+  // takes K = panels nb, so C is read and written once per group.
+  static constexpr int MAX_GROUP = 4;
+  int upd = 1, upd_early = 1, early_rows = 0;
+  // the running factorisation's groups: first columns (plus N at the end)
+  // and the step of the group starting at panel k / nb
+  std::vector<int> group_start, step_idx;
   int* d_S = nullptr;
   std::vector<int> steps_S;  // host copy after a factorisation
   // optional HIP-event timing of the trailing-update launches
@@ -149,12 +154,20 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   alloc(&h->v1, (size_t)N * 8);
   alloc(&h->v2, (size_t)N * 8);
   if (precision == LU_OZAKI) {
-    alloc(&h->oz_work, ozaki_workspace_bytes(N, N, 2 * nb, slices));
-    alloc(&h->oz_work2, ozaki_workspace_bytes(N, N, 2 * nb, slices));
+    alloc(&h->oz_work, ozaki_workspace_bytes(N, N, LUHandle::MAX_GROUP * nb, slices));
+    alloc(&h->oz_work2, ozaki_workspace_bytes(N, N, LUHandle::MAX_GROUP * nb, slices));
     alloc((void**)&h->d_S, sizeof(int) * (N / nb + 3));
   }
   h->adaptive = precision == LU_OZAKI && slices <= 5;
-  h->upd = precision == LU_OZAKI ? 2 : 1;
+  if (precision == LU_OZAKI) {
+    // pairs throughout: groups of four while the trailing matrix is large
+    // shorten the trailing updates but cost more FP64 block-row work and
+    // lower the later slice counts (the first group's exponents are the
+    // reference); measured slower end to end at N = 32768
+    h->upd = 2;
+    h->upd_early = 2;
+    h->early_rows = 0;
+  }
   h->fast_solve = (N % TRSV_BLOCK) == 0;
   if (N >= TRSV_BLOCK) {  // also for a later lu_resize to a multiple of TRSV_BLOCK
     alloc(&h->Lbinv, (size_t)N * TRSV_BLOCK * h->es);
@@ -168,10 +181,10 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   }
   if (st == OK && pivot == PIV_NONE) {
     // the panel's scratch also serves the recursive doubling of its inverse
-    // block ((TRSV_BLOCK / 2)^2 elements) and the block-row solve inside a
-    // pair of panels (nb^2)
+    // block ((TRSV_BLOCK / 2)^2 elements) and the block-row solves inside a
+    // group of panels (nb (MAX_GROUP - 1) nb)
     size_t wp = (size_t)N * 64 > (size_t)65536 ? (size_t)N * 64 : (size_t)65536;
-    if (wp < (size_t)nb * nb) wp = (size_t)nb * nb;
+    if (wp < (size_t)nb * nb * (LUHandle::MAX_GROUP - 1)) wp = (size_t)nb * nb * (LUHandle::MAX_GROUP - 1);
     alloc(&h->Wp, wp * h->es);
     // default-priority stream (no priority is ever set on any stream)
     if (st == OK && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
@@ -192,7 +205,7 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
 // Slice workspace of the trailing update of step k (alternating, so that the
 // panel can slice the next step's rows while this step's update reads its own).
 // Index of the trailing update that starts at column k.
-static int step_of(const LUHandle* h, int k) { return k / (h->nb * h->cur_upd); }
+static int step_of(const LUHandle* h, int k) { return h->step_idx[k / h->nb]; }
 static void* oz_ws(const LUHandle* h, int k) { return (step_of(h, k) & 1) ? h->oz_work2 : h->oz_work; }
 
 // Whether the panel [k, k + kb) is a whole TRSV_BLOCK diagonal block whose
@@ -395,7 +408,7 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
   return OK;
 }
 
-// Factor the columns [k, k + kbs) (cur_upd panels of nb columns) over rows
+// Factor the columns [k, k + kbs) (a group of panels of nb columns) over rows
 // [k, N), right-looking inside the group: each panel, then the block-row
 // solve of its rows over the group's later columns and their FP64 update.
 // The int8-sliced variant then slices the rows of the group's L21 (K = kbs)
@@ -451,26 +464,38 @@ static int factor_impl(LUHandle* h, hipStream_t s) {
   const int64_t ld = N;
   auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
   const bool look = h->lookahead && h->pivot == PIV_NONE && h->side != nullptr;
-  h->cur_upd = look && h->precision == LU_OZAKI ? h->upd : 1;
+  // steps: groups of panels on the lookahead path of the int8-sliced variant,
+  // single panels otherwise
+  h->group_start.clear();
+  for (int k = 0; k < N;) {
+    h->group_start.push_back(k);
+    int g = 1;
+    if (look && h->precision == LU_OZAKI)
+      g = h->upd_early > h->upd && N - k >= h->early_rows ? h->upd_early : h->upd;
+    k += g * nb;
+  }
+  h->group_start.push_back(N);
+  h->step_idx.assign(N / nb + 2, 0);
+  for (size_t i = 0; i + 1 < h->group_start.size(); ++i) h->step_idx[h->group_start[i] / nb] = (int)i;
   if (look) {
     // Step k: [main] U12 of block row k, slicing, trailing update of the next
     // panel's columns; [side] factor the next panel as soon as its columns are
     // final; [main] the rest of the trailing update meanwhile. The panel
     // touches only columns [k+kb, k+2kb) and its own scratch Wp; the rest of
     // the trailing update only columns >= k+2kb.
-    // steps of gw = cur_upd nb columns (a group of panels per trailing update)
+    // one step per group of panels (one trailing update each)
     T* Wp = static_cast<T*>(h->Wp);
-    const int gw = h->cur_upd * nb;
-    MPDOK_TRY(panel_group_factor<T>(h, s, 0, gw < N ? gw : N, W));
-    for (int k = 0; k < N; k += gw) {
-      const int kb = gw < N - k ? gw : N - k;
+    const std::vector<int>& gs = h->group_start;
+    MPDOK_TRY(panel_group_factor<T>(h, s, 0, gs[1], W));
+    for (size_t g = 0; g + 1 < gs.size(); ++g) {
+      const int k = gs[g], kb = gs[g + 1] - k;
       const int n3 = N - k - kb;
       if (n3 <= 0) break;
       if (k > 0) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
       MPDOK_TRY(block_rows_solve<T>(h, s, k, kb, k + kb, W));
       if (h->precision == LU_OZAKI)
         MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k, k + kb)), ld));
-      const int nk = gw < n3 ? gw : n3;
+      const int nk = gs[g + 2] - gs[g + 1];  // the next group
       MPDOK_TRY(trailing_cols<T>(h, s, k, kb, 0, nk));
       MPDOK_HIP(hipEventRecord(h->ev_next_cols, s));
       // the rest of the update is enqueued before the panel's launches, so
@@ -518,7 +543,7 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
     MPDOK_TRY(factor_impl<float>(h, s));
   int hinfo = 0;
   MPDOK_HIP(hipMemcpyAsync(&hinfo, h->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-  const int nsteps = (N + h->nb * h->cur_upd - 1) / (h->nb * h->cur_upd);
+  const int nsteps = (int)h->group_start.size() - 1;
   h->steps_S.assign(nsteps, h->precision == LU_OZAKI ? h->slices : 0);
   if (adaptive_on(h) && N > h->nb)
     MPDOK_HIP(hipMemcpyAsync(h->steps_S.data(), h->d_S + 1, sizeof(int) * (nsteps - 1),
@@ -675,16 +700,23 @@ MPDOK_API int mpdok_lu_set_adaptive(void* hv, int on, int smin, int delta) {
   return 0;
 }
 // Panels per trailing update on the lookahead path of the int8-sliced
-// variant: 1, or 2 (the default: panels factored in pairs, one update with
-// K = 2 nb per pair).
-MPDOK_API int mpdok_lu_set_update_panels(void* hv, int panels) {
+// variant: groups of `panels` (1 or 2), and of `panels_early` (panels ..
+// MAX_GROUP) while the trailing matrix has at least early_rows rows
+// (defaults 2, 2 and 0).
+MPDOK_API int mpdok_lu_set_update_panels(void* hv, int panels, int panels_early, int early_rows) {
   auto* h = static_cast<mpdok::LUHandle*>(hv);
   MPDOK_REQUIRE(h != nullptr, "lu_set_update_panels: null handle");
   MPDOK_REQUIRE(panels == 1 || panels == 2, "lu_set_update_panels: panels must be 1 or 2, got %d",
                 panels);
-  MPDOK_REQUIRE(panels == 1 || h->precision == mpdok::LU_OZAKI,
-                "lu_set_update_panels: panel pairs need the int8-sliced variant");
+  MPDOK_REQUIRE(panels_early >= panels && panels_early <= mpdok::LUHandle::MAX_GROUP,
+                "lu_set_update_panels: panels_early=%d outside [%d, %d]", panels_early, panels,
+                mpdok::LUHandle::MAX_GROUP);
+  MPDOK_REQUIRE(early_rows >= 0, "lu_set_update_panels: early_rows < 0");
+  MPDOK_REQUIRE(panels_early == 1 || h->precision == mpdok::LU_OZAKI,
+                "lu_set_update_panels: panel groups need the int8-sliced variant");
   h->upd = panels;
+  h->upd_early = panels_early;
+  h->early_rows = early_rows;
   return 0;
 }
 // Slice counts the last factorisation used, one per trailing update (n of them at most).

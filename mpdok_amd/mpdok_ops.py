@@ -123,11 +123,15 @@ class _LUHandle:
         check(self._lib.mpdok_lu_set_adaptive(self.ptr, 1 if on else 0, int(smin), int(delta)),
               "lu_set_adaptive")
 
-    def set_update_panels(self, panels):
+    def set_update_panels(self, panels, panels_early=None, early_rows=20480):
         """Panels per trailing update of the int8-sliced variant (see
-        mpdok_lu_set_update_panels): 2 factors them in pairs and updates once
-        per pair with K = 2 nb; 1 updates after every panel."""
-        check(self._lib.mpdok_lu_set_update_panels(self.ptr, int(panels)), "lu_set_update_panels")
+        mpdok_lu_set_update_panels): groups of `panels` (1 or 2), and of
+        `panels_early` (default: the same) while the trailing matrix has at
+        least early_rows rows; each group is updated once, with K = its
+        panels times nb."""
+        pe = panels if panels_early is None else panels_early
+        check(self._lib.mpdok_lu_set_update_panels(self.ptr, int(panels), int(pe), int(early_rows)),
+              "lu_set_update_panels")
 
     def slice_counts(self):
         """Slice counts of the last factorisation, one per trailing update."""

@@ -156,15 +156,17 @@ class OzakiAdaptiveTrailing:
 # ---------------------------------------------------------------------------
 # Blocked LU without pivoting
 # ---------------------------------------------------------------------------
-def lu_nopiv(A, nb=256, trailing=None):
+def lu_nopiv(A, nb=256, trailing=None, starts=None):
     """Right-looking blocked LU (no pivoting) in FP64; L unit-lower and U packed
     in one array. trailing(L21, U12) returns the product used for
-    A22 -= L21 @ U12 (default: the FP64 product)."""
+    A22 -= L21 @ U12 (default: the FP64 product). starts: first columns of
+    the blocks when they are not all nb wide (ascending, starting at 0)."""
     A = np.array(A, np.float64, order="F", copy=True)
     n = A.shape[0]
     mm = trailing if trailing is not None else (lambda L, U: L @ U)
-    for k in range(0, n, nb):
-        e = min(k + nb, n)
+    ks = list(starts) if starts is not None else list(range(0, n, nb))
+    for b, k in enumerate(ks):
+        e = min(ks[b + 1], n) if b + 1 < len(ks) else n
         for j in range(k, e):
             if A[j, j] == 0.0:
                 raise ZeroDivisionError(f"zero pivot at {j}")

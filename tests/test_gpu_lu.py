@@ -222,29 +222,41 @@ def test_ozaki_decision_matches_oracle(L):
         assert np.array_equal(out, C - olu.ozaki_matmul_lead(A, Bk, 5, want))
 
 
-@pytest.mark.parametrize("panels,nb", [(1, 256), (2, 128)])
-def test_ozaki_adaptive_lu_counts_follow_its_own_panels(L, panels, nb):
-    """Adaptive factorisation of a decaying RBF matrix, one or two panels per
-    trailing update: each update's count is the oracle's decision on the very
-    L21 and U12 the library sliced (read back from its factors; K = panels nb),
-    the counts drop below 5, and the backward error stays within a small
-    factor of the fixed 5-slice factorisation's."""
+def _groups(N, nb, panels, early, early_rows):
+    """First columns of the library's panel groups (one trailing update each)."""
+    out, k = [], 0
+    while k < N:
+        out.append(k)
+        k += (early if early > panels and N - k >= early_rows else panels) * nb
+    return out
+
+
+@pytest.mark.parametrize("panels,early,early_rows,nb",
+                         [(1, 1, 0, 256), (2, 2, 0, 128), (2, 4, 1024, 128)])
+def test_ozaki_adaptive_lu_counts_follow_its_own_panels(L, panels, early, early_rows, nb):
+    """Adaptive factorisation of a decaying RBF matrix with one, two, or four
+    then two panels per trailing update: each update's count is the oracle's
+    decision on the very L21 and U12 the library sliced (read back from its
+    factors; K = the group's width), the counts drop below 5, and the
+    backward error stays within a small factor of the fixed 5-slice
+    factorisation's."""
     N = 2048
     X = np.random.default_rng(42).uniform(0, 1, (N, 2))
     A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
     from mpdok_amd.mpdok_ops import _LUHandle
     h = _LUHandle(L, N, "ozaki", "none", nb, 5, True)
-    h.set_update_panels(panels)
+    h.set_update_panels(panels, early, early_rows)
     info = h.factor(dev(A))
     assert info == 0
     counts = h.slice_counts()
     LU, _ = _factors(L, h, N)
     e0 = None
     want = []
-    g = panels * nb
-    for k in range(0, N - g, g):
-        S, e0 = olu.ozaki_decide(LU[k + g:, k:k + g], LU[k:k + g, k + g:], 5, 2, 0, e0)
+    gs = _groups(N, nb, panels, early, early_rows) + [N]
+    for k, e in zip(gs[:-2], gs[1:-1]):
+        S, e0 = olu.ozaki_decide(LU[e:, k:e], LU[k:e, e:], 5, 2, 0, e0)
         want.append(S)
+    assert len(counts) == len(gs) - 1
     assert counts[:len(want)] == want
     assert want[0] == 5 and min(want) < 5
     Lm, Um = olu.unpack(LU)
@@ -322,22 +334,23 @@ def test_lu_singular_reports_info(L):
 
 
 @pytest.mark.parametrize("S", [4, 5, 6])
-@pytest.mark.parametrize("panels", [1, 2])
-def test_ozaki_lu_matches_oracle_on_well_conditioned_spd(L, S, panels):
+@pytest.mark.parametrize("panels,early", [(1, 1), (2, 2), (2, 4)])
+def test_ozaki_lu_matches_oracle_on_well_conditioned_spd(L, S, panels, early):
     # Factors of a well-conditioned matrix are well-conditioned functions of
     # the rounding, so the GPU factors must match the oracle's restatement.
-    # With two panels per update the pair is factored in FP64 and the update
-    # takes K = 512: the oracle's blocked LU with 512-column panels.
-    N = 1024
+    # A group of panels is factored in FP64 and its update takes K = the
+    # group's width: the oracle's blocked LU with the same block starts.
+    N, nb = 1024, 128
     rng = np.random.default_rng(S)
     X = rng.standard_normal((N, N))
     A = X.T @ X / N + np.eye(N)
     from mpdok_amd.mpdok_ops import _LUHandle
-    h = _LUHandle(L, N, "ozaki", "none", 256, S, False)
-    h.set_update_panels(panels)
+    h = _LUHandle(L, N, "ozaki", "none", nb, S, False)
+    h.set_update_panels(panels, early, 768)
     assert h.factor(dev(A)) == 0
     LU, _ = _factors(L, h, N)
-    ref = olu.lu_nopiv(A, nb=256 * panels, trailing=lambda a, b: olu.ozaki_matmul(a, b, S))
+    ref = olu.lu_nopiv(A, trailing=lambda a, b: olu.ozaki_matmul(a, b, S),
+                       starts=_groups(N, nb, panels, early, 768))
     # panel roundings differ (FP64 order), and a last digit may flip where an
     # input sits on a rounding boundary: allow a few units of 2^(-(8 S - 1))
     assert np.abs(LU - ref).max() < 2.0 ** (-(8 * S - 1) + 6) * np.abs(ref).max()

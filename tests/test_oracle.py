@@ -205,6 +205,22 @@ def test_lu_nopiv_reconstructs_and_matches_lapack_on_spd():
     assert np.allclose(x, sla.solve(A, b), rtol=1e-12)
 
 
+def test_lu_nopiv_block_starts_match_lapack_and_uniform_blocking():
+    """Blocks of several widths (starts) give LAPACK's unpivoted factors like
+    uniform blocks do; starts equal to the uniform ones give the same bits;
+    and the trailing-update callable sees each block's K = its width."""
+    A, _ = _spd(150, 3)
+    lu, piv = sla.lu_factor(A)
+    assert np.all(piv == np.arange(150))
+    starts = [0, 64, 128, 136]
+    LU = olu.lu_nopiv(A, starts=starts)
+    assert np.allclose(LU, lu, rtol=1e-10, atol=1e-10)
+    assert np.array_equal(olu.lu_nopiv(A, nb=32, starts=range(0, 150, 32)), olu.lu_nopiv(A, nb=32))
+    widths = []
+    olu.lu_nopiv(A, starts=starts, trailing=lambda L, U: widths.append(L.shape[1]) or L @ U)
+    assert widths == [64, 64, 8]
+
+
 def test_lu_solve_packed_with_pivots_matches_lapack():
     rng = np.random.default_rng(11)
     A = rng.standard_normal((80, 80))
