@@ -211,15 +211,16 @@ def main():
         # per launch: the larger of its int8-MFMA time and its HBM time (FP64 C
         # read + write) at peak, over the measured time: the fraction of the
         # mixed roofline (the low slice counts are HBM-bound)
+        # steps: groups of panels, one trailing update (two launches: the next
+        # group's columns, then the rest) with K = the group's width each
         t_roof = 0.0
-        nb = args.nb
-        for step, k in enumerate(range(0, N, nb)):
-            n3 = N - k - nb
-            if n3 <= 0:
-                break
+        starts = list(info_mix.get("step_starts") or range(0, N, args.nb)) + [N]
+        for step in range(len(starts) - 2):
+            k, kb, nxt = starts[step], starts[step + 1] - starts[step], starts[step + 2] - starts[step + 1]
+            n3 = N - k - kb
             S = counts[step] if step < len(counts) else args.slices
-            for nc in ([min(nb, n3)] + ([n3 - nb] if n3 > nb else [])):
-                ops = 2.0 * n3 * nc * nb * S * (S + 1) / 2
+            for nc in ([nxt] + ([n3 - nxt] if n3 > nxt else [])):
+                ops = 2.0 * n3 * nc * kb * S * (S + 1) / 2
                 byts = 16.0 * n3 * nc
                 t_roof += max(ops / (PEAK_INT8_TOPS * 1e12), byts / (PEAK_HBM_GBS * 1e9))
         roofline = {"bound": "mfma", "kernel": f"ozaki_gemmd_kernel<{args.slices}, ...>",

@@ -237,6 +237,10 @@ int mpdok_lu_set_adaptive(void* handle, int on, int smin, int delta);
 /* Slice counts of the last factorisation, one per trailing update (*nsteps
  * of them; at most n written). */
 int mpdok_lu_slice_counts(void* handle, int* out, int n, int* nsteps);
+/* First columns of the last factorisation's steps, one trailing update per
+ * step (a panel, or a group of panels); *nsteps of them, at most n written.
+ * No reference counterpart. */
+int mpdok_lu_step_starts(void* handle, int* out, int n, int* nsteps);
 /* Panels per trailing update of the int8-sliced variant on the lookahead
  * path: groups of `panels` (1 or 2), and of `panels_early` (panels .. 4)
  * while the trailing matrix has at least early_rows rows. A group is

@@ -719,6 +719,16 @@ MPDOK_API int mpdok_lu_set_update_panels(void* hv, int panels, int panels_early,
   h->early_rows = early_rows;
   return 0;
 }
+// First columns of the last factorisation's steps (one trailing update per
+// step: a panel, or a group of panels), n of them at most.
+MPDOK_API int mpdok_lu_step_starts(void* hv, int* out, int n, int* nsteps) {
+  auto* h = static_cast<mpdok::LUHandle*>(hv);
+  MPDOK_REQUIRE(h != nullptr, "lu_step_starts: null handle");
+  const int ns = h->group_start.empty() ? 0 : (int)h->group_start.size() - 1;
+  for (int i = 0; i < ns && i < n; ++i) out[i] = h->group_start[i];
+  if (nsteps) *nsteps = ns;
+  return 0;
+}
 // Slice counts the last factorisation used, one per trailing update (n of them at most).
 MPDOK_API int mpdok_lu_slice_counts(void* hv, int* out, int n, int* nsteps) {
   auto* h = static_cast<mpdok::LUHandle*>(hv);

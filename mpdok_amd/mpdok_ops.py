@@ -133,6 +133,15 @@ class _LUHandle:
         check(self._lib.mpdok_lu_set_update_panels(self.ptr, int(panels), int(pe), int(early_rows)),
               "lu_set_update_panels")
 
+    def step_starts(self):
+        """First columns of the last factorisation's steps (one trailing
+        update each: a panel, or a group of panels)."""
+        out = (ctypes.c_int * 4096)()
+        n = ctypes.c_int(0)
+        check(self._lib.mpdok_lu_step_starts(self.ptr, out, 4096, ctypes.byref(n)),
+              "lu_step_starts")
+        return list(out[:min(n.value, 4096)])
+
     def slice_counts(self):
         """Slice counts of the last factorisation, one per trailing update."""
         out = (ctypes.c_int * 4096)()
@@ -492,7 +501,8 @@ class MPDOKSolver:
         self.last_info = {"history": list(hist[:n_hist.value]), "converged": bool(conv.value),
                           "inner_iterations": inner.value,
                           "factor_stats": h.stats() if self.profile else None,
-                          "slice_counts": h.slice_counts() if factor_precision == "ozaki" else None}
+                          "slice_counts": h.slice_counts() if factor_precision == "ozaki" else None,
+                          "step_starts": h.step_starts()}
         if not conv.value:
             warnings.warn(
                 f"GMRES-IR did not converge to tol={tol:.1e} within "
