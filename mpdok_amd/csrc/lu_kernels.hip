@@ -829,6 +829,12 @@ __global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ B
     if (t < 64) out[t] = TV(0);
     return;
   }
+  // the inverse's entries do not depend on v': their loads go out first
+  const int r = t & 63, p = t >> 6;
+  const T* col = Binv + rg * 64 + r + (size_t)(cg * TB_CW) * TB;
+  T bv[TB_CW / 4];
+#pragma unroll
+  for (int q = 0; q < TB_CW / 4; ++q) bv[q] = col[(size_t)(p + 4 * q) * TB];
   if (t < TB_CW) {
     const int g = r0 + cg * TB_CW + t;
     TV s = v[g];
@@ -837,11 +843,9 @@ __global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ B
     vs[t] = s;
   }
   __syncthreads();
-  const int r = t & 63, p = t >> 6;
-  const T* col = Binv + rg * 64 + r + (size_t)(cg * TB_CW) * TB;
   TV a = TV(0);
-#pragma unroll 4
-  for (int j = p; j < TB_CW; j += 4) a = __builtin_fma((TV)col[(size_t)j * TB], vs[j], a);
+#pragma unroll
+  for (int q = 0; q < TB_CW / 4; ++q) a = __builtin_fma((TV)bv[q], vs[p + 4 * q], a);
   red[p][r] = a;
   __syncthreads();
   if (t < 64) out[t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
@@ -859,6 +863,20 @@ __global__ __launch_bounds__(256) void tri_acc_kernel(const T* __restrict__ M, i
   __shared__ TV xs[TB_CW];
   const int c = blockIdx.y;
   const int cb = c0 + c * TB_CW;
+  const int r = rbeg + blockIdx.x * 256 + threadIdx.x;
+  const bool active = r < rend;
+  // the first 32 matrix entries of the row and its accumulator do not depend
+  // on x: their loads go out before the partial sums of x and the barrier
+  // (bytes in flight per CU, not issue, bound this kernel); the sum keeps the
+  // order j = 0 .. 63
+  const T* mr = M + r + (int64_t)cb * ldm;
+  T m[32];
+  TV a0 = TV(0);
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < 32; ++j) m[j] = mr[(int64_t)j * ldm];
+    a0 = acc[(size_t)c * N + r];
+  }
   if (threadIdx.x < TB_CW) {
     TV s = TV(0);
 #pragma unroll
@@ -867,12 +885,15 @@ __global__ __launch_bounds__(256) void tri_acc_kernel(const T* __restrict__ M, i
     if (blockIdx.x == 0) x[cb + threadIdx.x] = s;
   }
   __syncthreads();
-  const int r = rbeg + blockIdx.x * 256 + threadIdx.x;
-  if (r >= rend) return;
+  if (!active) return;
   TV a = TV(0);
-#pragma unroll 8
-  for (int j = 0; j < TB_CW; ++j) a = __builtin_fma((TV)M[r + (int64_t)(cb + j) * ldm], xs[j], a);
-  acc[(size_t)c * N + r] += a;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) a = __builtin_fma((TV)m[j], xs[j], a);
+#pragma unroll
+  for (int j = 0; j < 32; ++j) m[j] = mr[(int64_t)(32 + j) * ldm];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) a = __builtin_fma((TV)m[j], xs[32 + j], a);
+  acc[(size_t)c * N + r] = a0 + a;
 }
 
 template <typename T, typename TV>
