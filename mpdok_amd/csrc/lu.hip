@@ -456,27 +456,50 @@ static int block_rows_solve(LUHandle* h, hipStream_t s, int k, int kbs, int c0, 
   return OK;
 }
 
-template <typename T>
-static int factor_impl(LUHandle* h, hipStream_t s) {
+static bool lookahead_on(const LUHandle* h) {
+  return h->lookahead && h->pivot == PIV_NONE && h->side != nullptr;
+}
+
+// Steps of the next factorisation: groups of panels on the lookahead path of
+// the int8-sliced variant, single panels otherwise.
+static void build_schedule(LUHandle* h) {
   const int N = h->N, nb = h->nb;
-  T* A = static_cast<T*>(h->LU);
-  T* W = static_cast<T*>(h->W);
-  const int64_t ld = N;
-  auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
-  const bool look = h->lookahead && h->pivot == PIV_NONE && h->side != nullptr;
-  // steps: groups of panels on the lookahead path of the int8-sliced variant,
-  // single panels otherwise
+  const bool grouped = lookahead_on(h) && h->precision == LU_OZAKI;
   h->group_start.clear();
   for (int k = 0; k < N;) {
     h->group_start.push_back(k);
     int g = 1;
-    if (look && h->precision == LU_OZAKI)
-      g = h->upd_early > h->upd && N - k >= h->early_rows ? h->upd_early : h->upd;
+    if (grouped) g = h->upd_early > h->upd && N - k >= h->early_rows ? h->upd_early : h->upd;
     k += g * nb;
   }
   h->group_start.push_back(N);
   h->step_idx.assign(N / nb + 2, 0);
   for (size_t i = 0; i + 1 < h->group_start.size(); ++i) h->step_idx[h->group_start[i] / nb] = (int)i;
+}
+
+// The input matrix's columns [c0, c1) into the handle's storage.
+static int copy_columns(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int order, int c0,
+                        int c1) {
+  if (c1 <= c0) return OK;
+  const double* src = order == COL_MAJOR ? A + (int64_t)c0 * lda : A + c0;
+  char* dst = static_cast<char*>(h->LU) + (size_t)c0 * h->N * h->es;
+  return cast_matrix(s, src, F64, lda, order, dst, fp64_storage(h->precision) ? F64 : F32, h->N,
+                     COL_MAJOR, h->N, c1 - c0);
+}
+
+// Factor the handle's storage. On the lookahead path the caller has copied
+// only the first step's columns; the rest of the input (rest_A, rest_lda,
+// rest_order) is copied here on the main stream while the lookahead stream
+// factors the first step.
+template <typename T>
+static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t rest_lda,
+                       int rest_order) {
+  const int N = h->N, nb = h->nb;
+  T* A = static_cast<T*>(h->LU);
+  T* W = static_cast<T*>(h->W);
+  const int64_t ld = N;
+  auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
+  const bool look = lookahead_on(h);
   if (look) {
     // Step k: [main] U12 of block row k, slicing, trailing update of the next
     // panel's columns; [side] factor the next panel as soon as its columns are
@@ -486,12 +509,18 @@ static int factor_impl(LUHandle* h, hipStream_t s) {
     // one step per group of panels (one trailing update each)
     T* Wp = static_cast<T*>(h->Wp);
     const std::vector<int>& gs = h->group_start;
-    MPDOK_TRY(panel_group_factor<T>(h, s, 0, gs[1], W));
+    // the first step's columns are in place: factor them on the lookahead
+    // stream while the main stream copies the rest of the input
+    MPDOK_HIP(hipEventRecord(h->ev_next_cols, s));
+    MPDOK_HIP(hipStreamWaitEvent(h->side, h->ev_next_cols, 0));
+    MPDOK_TRY(panel_group_factor<T>(h, h->side, 0, gs[1], Wp));
+    MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
+    if (rest_A) MPDOK_TRY(copy_columns(h, s, rest_A, rest_lda, rest_order, gs[1], N));
     for (size_t g = 0; g + 1 < gs.size(); ++g) {
       const int k = gs[g], kb = gs[g + 1] - k;
       const int n3 = N - k - kb;
+      MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
       if (n3 <= 0) break;
-      if (k > 0) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
       MPDOK_TRY(block_rows_solve<T>(h, s, k, kb, k + kb, W));
       if (h->precision == LU_OZAKI)
         MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k, k + kb)), ld));
@@ -534,13 +563,16 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
   h->ev_ops.clear();
   h->ev_step.clear();
   MPDOK_HIP(hipMemsetAsync(h->d_info, 0, sizeof(int), s));
-  // copy (and convert / transpose) A into the handle's column-major storage
-  MPDOK_TRY(cast_matrix(s, A, F64, lda, order, h->LU, fp64_storage(h->precision) ? F64 : F32, N,
-                        COL_MAJOR, N, N));
+  // copy (and convert / transpose) A into the handle's column-major storage:
+  // on the lookahead path only the first step's columns before the
+  // factorisation starts (it copies the rest beside the first panel)
+  build_schedule(h);
+  const bool split = lookahead_on(h);
+  MPDOK_TRY(copy_columns(h, s, A, lda, order, 0, split ? h->group_start[1] : N));
   if (fp64_storage(h->precision))
-    MPDOK_TRY(factor_impl<double>(h, s));
+    MPDOK_TRY(factor_impl<double>(h, s, split ? A : nullptr, lda, order));
   else
-    MPDOK_TRY(factor_impl<float>(h, s));
+    MPDOK_TRY(factor_impl<float>(h, s, split ? A : nullptr, lda, order));
   int hinfo = 0;
   MPDOK_HIP(hipMemcpyAsync(&hinfo, h->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
   const int nsteps = (int)h->group_start.size() - 1;

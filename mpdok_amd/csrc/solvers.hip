@@ -200,9 +200,16 @@ static int gmres_ir_impl(hipStream_t s, const double* A64, int64_t lda64, int or
   }
   double rel = 0.0;
   for (int outer = 0; outer < o.maxiter_outer; ++outer) {
-    MPDOK_TRY(residual_f64(s, A64, lda64, order64, N, x, b, r, o.resid_mode, W.resid.p, W.resid.n));
     double rn = 0.0;
-    MPDOK_TRY(host_norm(s, r, N, W.dots, &rn));
+    if (outer == 0) {
+      // x = 0: the residual is b exactly, without a pass over A
+      MPDOK_HIP(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+      rn = bnorm;
+    } else {
+      MPDOK_TRY(residual_f64(s, A64, lda64, order64, N, x, b, r, o.resid_mode, W.resid.p,
+                             W.resid.n));
+      MPDOK_TRY(host_norm(s, r, N, W.dots, &rn));
+    }
     rel = rn / bnorm;
     res.hist.push_back(rel);
     res.outer = outer;
@@ -251,9 +258,15 @@ static int lu_ir_impl(hipStream_t s, const double* A64, int64_t lda, int order, 
     return OK;
   }
   for (int it = 0; it < maxiter_outer; ++it) {
-    MPDOK_TRY(residual_f64(s, A64, lda, order, N, x, b, r, resid_mode, W.resid.p, W.resid.n));
     double rn = 0.0;
-    MPDOK_TRY(host_norm(s, r, N, W.dots, &rn));
+    if (it == 0) {
+      // x = 0: the residual is b exactly, without a pass over A
+      MPDOK_HIP(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+      rn = bnorm;
+    } else {
+      MPDOK_TRY(residual_f64(s, A64, lda, order, N, x, b, r, resid_mode, W.resid.p, W.resid.n));
+      MPDOK_TRY(host_norm(s, r, N, W.dots, &rn));
+    }
     res.hist.push_back(rn / bnorm);
     res.outer = it;
     if (rn / bnorm < tol) {
