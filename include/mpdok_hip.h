@@ -241,6 +241,13 @@ int mpdok_lu_slice_counts(void* handle, int* out, int n, int* nsteps);
  * step (a panel, or a group of panels); *nsteps of them, at most n written.
  * No reference counterpart. */
 int mpdok_lu_step_starts(void* handle, int* out, int n, int* nsteps);
+/* Row lookahead on the no-pivot lookahead path (the int8-sliced variant
+ * needs adaptive slice counts): while the trailing matrix has at least
+ * min_rows rows, a step's block-row solve, U12 slicing and slice-count
+ * decision run on the lookahead stream right after its panels, and the main
+ * stream updates the next step's rows before the rest of its trailing update.
+ * 0 turns it off; default 20480. No reference counterpart. */
+int mpdok_lu_set_row_lookahead(void* handle, int min_rows);
 /* Panels per trailing update of the int8-sliced variant on the lookahead
  * path: groups of `panels` (1 or 2), and of `panels_early` (panels .. 4)
  * while the trailing matrix has at least early_rows rows. A group is

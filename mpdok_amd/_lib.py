@@ -87,6 +87,7 @@ SIGNATURES = {
     "mpdok_lu_slice_counts": (I, [P, PI, I, PI]),
     "mpdok_lu_set_update_panels": (I, [P, I, I, I]),
     "mpdok_lu_step_starts": (I, [P, PI, I, PI]),
+    "mpdok_lu_set_row_lookahead": (I, [P, I]),
     "mpdok_gemm": (I, [P, I, I, I, I, D, P, I64, P, I64, D, P, I64, I]),
     "mpdok_ozaki_workspace_bytes": (SZ, [I, I, I, I]),
     "mpdok_ozaki_gemm_sub": (I, [P, I, I, I, I, P, I64, P, I64, P, I64, P]),

@@ -74,6 +74,11 @@ struct LUHandle {
   // the running factorisation's groups: first columns (plus N at the end)
   // and the step of the group starting at panel k / nb
   std::vector<int> group_start, step_idx;
+  // row lookahead: while the trailing matrix has at least rows_side_min rows
+  // (0: never), a group's block-row solve, U12 slicing and slice-count
+  // decision run on the lookahead stream right after its panels (the main
+  // stream updates the group's rows before the rest of its trailing update)
+  int rows_side_min = 0;
   int* d_S = nullptr;
   std::vector<int> steps_S;  // host copy after a factorisation
   // optional HIP-event timing of the trailing-update launches
@@ -159,6 +164,11 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
     alloc((void**)&h->d_S, sizeof(int) * (N / nb + 3));
   }
   h->adaptive = precision == LU_OZAKI && slices <= 5;
+  // row lookahead while the trailing matrix has at least 20480 rows:
+  // measured 175 -> 172 ms for the N = 32768 factorisation (the early
+  // trailing updates saturate the GPU, so the moved work still competes for
+  // it; the gain is the latency it no longer adds to the main stream)
+  h->rows_side_min = 20480;
   if (precision == LU_OZAKI) {
     // pairs throughout: groups of four while the trailing matrix is large
     // shorten the trailing updates but cost more FP64 block-row work and
@@ -185,6 +195,7 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
     // group of panels (nb (MAX_GROUP - 1) nb)
     size_t wp = (size_t)N * 64 > (size_t)65536 ? (size_t)N * 64 : (size_t)65536;
     if (wp < (size_t)nb * nb * (LUHandle::MAX_GROUP - 1)) wp = (size_t)nb * nb * (LUHandle::MAX_GROUP - 1);
+    if (wp < (size_t)nb * N) wp = (size_t)nb * N;  // block-row solves of the row lookahead
     alloc(&h->Wp, wp * h->es);
     // default-priority stream (no priority is ever set on any stream)
     if (st == OK && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
@@ -371,12 +382,15 @@ static int slice_step(LUHandle* h, hipStream_t s, int k, int kb, int n3, const d
   return OK;
 }
 
-// Trailing update of step k restricted to columns [k+kb+c_off, k+kb+c_off+ncols).
-// For the int8-sliced variant the operands must have been sliced already.
+// Trailing update of step k restricted to rows [k+kb+r_off, k+kb+r_off+nrows)
+// and columns [k+kb+c_off, k+kb+c_off+ncols). For the int8-sliced variant the
+// operands must have been sliced already (row ranges need the adaptive
+// kernel; r_off a multiple of 128).
 template <typename T>
-static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, int ncols) {
+static int trailing_block(LUHandle* h, hipStream_t s, int k, int kb, int r_off, int nrows,
+                          int c_off, int ncols) {
   const int N = h->N, m3 = N - k - kb, n3 = N - k - kb;
-  if (ncols <= 0 || m3 <= 0) return OK;
+  if (ncols <= 0 || nrows <= 0) return OK;
   T* A = static_cast<T*>(h->LU);
   const int64_t ld = N;
   auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
@@ -384,7 +398,7 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
   const size_t nev = h->ev_ops.size();
   if (h->profile) {
     // int8-sliced: S(S+1)/2 int8 products of 2 m n k operations each
-    h->ev_ops.push_back(2.0 * m3 * ncols * kb);
+    h->ev_ops.push_back(2.0 * nrows * ncols * kb);
     h->ev_step.push_back(step_of(h, k));
     while (h->ev.size() < 2 * h->ev_ops.size()) {
       hipEvent_t e;
@@ -394,18 +408,23 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
     MPDOK_HIP(hipEventRecord(h->ev[2 * nev], s));
   }
   if (adaptive_on(h)) {
-    MPDOK_TRY(ozaki_apply_dyn(s, h->slices, m3, n3, kb, c_off, ncols,
+    MPDOK_TRY(ozaki_apply_dyn(s, h->slices, m3, n3, kb, r_off, nrows, c_off, ncols,
                               reinterpret_cast<double*>(at(k + kb, k + kb)), ld, oz_ws(h, k),
                               h->d_S + 1 + step_of(h, k)));
   } else if (h->precision == LU_OZAKI) {
+    MPDOK_REQUIRE(r_off == 0 && nrows == m3, "trailing update: row ranges need adaptive slicing");
     MPDOK_TRY(ozaki_apply(s, h->slices, m3, n3, kb, c_off, ncols,
                           reinterpret_cast<double*>(at(k + kb, k + kb)), ld, oz_ws(h, k)));
   } else {
-    MPDOK_TRY(gemm<T>(s, m3, ncols, kb, T(-1), at(k + kb, k), ld, at(k, k + kb + c_off), ld, T(1),
-                      at(k + kb, k + kb + c_off), ld, round));
+    MPDOK_TRY(gemm<T>(s, nrows, ncols, kb, T(-1), at(k + kb + r_off, k), ld, at(k, k + kb + c_off),
+                      ld, T(1), at(k + kb + r_off, k + kb + c_off), ld, round));
   }
   if (h->profile) MPDOK_HIP(hipEventRecord(h->ev[2 * nev + 1], s));
   return OK;
+}
+template <typename T>
+static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, int ncols) {
+  return trailing_block<T>(h, s, k, kb, 0, h->N - k - kb, c_off, ncols);
 }
 
 // Factor the columns [k, k + kbs) (a group of panels of nb columns) over rows
@@ -516,22 +535,40 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
     MPDOK_TRY(panel_group_factor<T>(h, h->side, 0, gs[1], Wp));
     MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
     if (rest_A) MPDOK_TRY(copy_columns(h, s, rest_A, rest_lda, rest_order, gs[1], N));
+    // row lookahead for group g (see LUHandle::rows_side_min); the first
+    // group's rows stay on the main stream (its copy runs there)
+    auto rows_on_side = [&](size_t g) {
+      return g >= 1 && g + 1 < gs.size() && gs[g + 1] < N && h->rows_side_min > 0 &&
+             N - gs[g] >= h->rows_side_min && (adaptive_on(h) || h->precision != LU_OZAKI);
+    };
     for (size_t g = 0; g + 1 < gs.size(); ++g) {
       const int k = gs[g], kb = gs[g + 1] - k;
       const int n3 = N - k - kb;
       MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
       if (n3 <= 0) break;
-      MPDOK_TRY(block_rows_solve<T>(h, s, k, kb, k + kb, W));
-      if (h->precision == LU_OZAKI)
-        MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k, k + kb)), ld));
+      if (!rows_on_side(g)) {
+        MPDOK_TRY(block_rows_solve<T>(h, s, k, kb, k + kb, W));
+        if (h->precision == LU_OZAKI)
+          MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k, k + kb)), ld));
+      }
       const int nk = gs[g + 2] - gs[g + 1];  // the next group
-      MPDOK_TRY(trailing_cols<T>(h, s, k, kb, 0, nk));
+      const bool nxt = rows_on_side(g + 1);
+      // the next group's columns, then (row lookahead) its rows
+      MPDOK_TRY(trailing_block<T>(h, s, k, kb, 0, n3, 0, nk));
+      if (nxt) MPDOK_TRY(trailing_block<T>(h, s, k, kb, 0, nk, nk, n3 - nk));
       MPDOK_HIP(hipEventRecord(h->ev_next_cols, s));
       // the rest of the update is enqueued before the panel's launches, so
       // that it never waits for the host to issue them
-      MPDOK_TRY(trailing_cols<T>(h, s, k, kb, nk, n3 - nk));
+      MPDOK_TRY(trailing_block<T>(h, s, k, kb, nxt ? nk : 0, nxt ? n3 - nk : n3, nk, n3 - nk));
       MPDOK_HIP(hipStreamWaitEvent(h->side, h->ev_next_cols, 0));
       MPDOK_TRY(panel_group_factor<T>(h, h->side, k + kb, nk, Wp));
+      if (nxt) {
+        const int k2 = k + kb, n32 = N - k2 - nk;
+        MPDOK_TRY(block_rows_solve<T>(h, h->side, k2, nk, k2 + nk, Wp));
+        if (h->precision == LU_OZAKI)
+          MPDOK_TRY(slice_step(h, h->side, k2, nk, n32,
+                               reinterpret_cast<const double*>(at(k2, k2 + nk)), ld));
+      }
       MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
     }
     if (N > nb) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
@@ -759,6 +796,17 @@ MPDOK_API int mpdok_lu_step_starts(void* hv, int* out, int n, int* nsteps) {
   const int ns = h->group_start.empty() ? 0 : (int)h->group_start.size() - 1;
   for (int i = 0; i < ns && i < n; ++i) out[i] = h->group_start[i];
   if (nsteps) *nsteps = ns;
+  return 0;
+}
+// Row lookahead (no-pivot lookahead path; the int8-sliced variant needs
+// adaptive slice counts): while the trailing matrix has at least min_rows
+// rows, a step's block-row solve, U12 slicing and slice-count decision run on
+// the lookahead stream right after its panels. 0 turns it off.
+MPDOK_API int mpdok_lu_set_row_lookahead(void* hv, int min_rows) {
+  auto* h = static_cast<mpdok::LUHandle*>(hv);
+  MPDOK_REQUIRE(h != nullptr, "lu_set_row_lookahead: null handle");
+  MPDOK_REQUIRE(min_rows >= 0, "lu_set_row_lookahead: min_rows < 0");
+  h->rows_side_min = min_rows;
   return 0;
 }
 // Slice counts the last factorisation used, one per trailing update (n of them at most).

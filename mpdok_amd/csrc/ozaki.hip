@@ -616,7 +616,7 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave % WM) * 32 * TA, wn = (wave / WM) * 32 * TB;
   int ti, tj;
-  if (!tile_of(blockIdx.x, mp / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN, ti, tj, group)) return;
+  if (!tile_of(blockIdx.x, (m + Cfg::BM - 1) / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN, ti, tj, group)) return;
   const int bm = ti * Cfg::BM, bn = col0 + tj * Cfg::BN;
   const size_t slabA = (size_t)mp * Kp, slabB = (size_t)np * Kp;
   const size_t kstepA = (size_t)mp * OZ_BK, kstepB = (size_t)np * OZ_BK;
@@ -1224,16 +1224,23 @@ static int ozaki_apply_s(hipStream_t s, int m, int n, int K, int col0, int ncols
 
 // C[:, col0 : col0 + ncols] -= the product of the leading *Sdev slices of the
 // last ozaki_slice call with smax slices (smax <= 5; *Sdev read on the device).
-int ozaki_apply_dyn(hipStream_t s, int smax, int m, int n, int K, int col0, int ncols, double* C,
-                    int64_t ldc, void* work, const int* Sdev) {
-  if (m <= 0 || ncols <= 0 || K <= 0) return OK;
+int ozaki_apply_dyn(hipStream_t s, int smax, int m, int n, int K, int row0, int nrows, int col0,
+                    int ncols, double* C, int64_t ldc, void* work, const int* Sdev) {
+  if (nrows <= 0 || ncols <= 0 || K <= 0) return OK;
   MPDOK_REQUIRE(col0 % 64 == 0 && col0 + ncols <= n, "ozaki_apply_dyn: bad column range");
+  MPDOK_REQUIRE(row0 % 128 == 0 && row0 >= 0 && row0 + nrows <= m, "ozaki_apply_dyn: bad row range");
   MPDOK_REQUIRE(smax >= 3 && smax <= 5, "ozaki_apply_dyn: smax must be in [3, 5], got %d", smax);
-  const OzLayout L = oz_layout(smax, m, n, K, work);
+  OzLayout L = oz_layout(smax, m, n, K, work);
+  // rows [row0, row0 + nrows): the K-blocked slices keep a K step's rows
+  // contiguous (stride mp), so the block starts row0 records in
+  L.As += (size_t)row0 * OZ_BK;
+  L.ea += row0;
+  C += row0;
+  m = nrows;
   const int g = tuning_get(TUNE_OZAKI_GROUP);
   const int group = g > 0 ? g : GROUP_M;
   // 128 x 64 tiles, 4 wavefronts of 64 x 32, two stages, two workgroups per CU
-  const int grid = tiles_launch(L.mp / 128, (ncols + 63) / 64);
+  const int grid = tiles_launch((m + 127) / 128, (ncols + 63) / 64);
   switch (smax) {
     case 3:
       ozaki_gemmd_kernel<3, 2, 2, 2, 1, 2, 2><<<grid, 256, 0, s>>>(
@@ -1337,7 +1344,7 @@ MPDOK_API int mpdok_ozaki_gemm_sub_adaptive(hipStream_t s, int smax, int m, int 
     MPDOK_TRY(mpdok::ozaki_decide(s, smax, m, n, K, work, smin, delta, first, dev_scratch,
                                   dev_scratch + 1));
   }
-  MPDOK_TRY(mpdok::ozaki_apply_dyn(s, smax, m, n, K, 0, n, C, ldc, work, dev_scratch + 1));
+  MPDOK_TRY(mpdok::ozaki_apply_dyn(s, smax, m, n, K, 0, m, 0, n, C, ldc, work, dev_scratch + 1));
   int host[2];
   MPDOK_HIP(hipMemcpyAsync(host, dev_scratch, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   MPDOK_HIP(hipStreamSynchronize(s));

@@ -133,6 +133,12 @@ class _LUHandle:
         check(self._lib.mpdok_lu_set_update_panels(self.ptr, int(panels), int(pe), int(early_rows)),
               "lu_set_update_panels")
 
+    def set_row_lookahead(self, min_rows):
+        """Row lookahead (see mpdok_lu_set_row_lookahead): while the trailing
+        matrix has at least min_rows rows, block-row solves, U12 slicing and
+        slice-count decisions run on the lookahead stream; 0 turns it off."""
+        check(self._lib.mpdok_lu_set_row_lookahead(self.ptr, int(min_rows)), "lu_set_row_lookahead")
+
     def step_starts(self):
         """First columns of the last factorisation's steps (one trailing
         update each: a panel, or a group of panels)."""
