@@ -471,12 +471,13 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemm_kernel(
 // Single-barrier pipelined variant (the default). WM x WN wavefronts, each
 // owning TA x TB MFMA tiles of 32 x 32 (wavefront tile 32 TA x 32 TB,
 // 16 S TA TB accumulator registers per lane, held in the accumulation
-// registers across the loop); block tile (32 TA WM) x (32 TB WN); one
-// workgroup per CU; four LDS stages. Per K step (32 bytes of every row and
-// column of all S slices) the wavefronts synchronise once:
+// registers across the loop); block tile (32 TA WM) x (32 TB WN); NST LDS
+// stages (2 or 4) and OCC workgroups per CU (the default shape: 2 x 2
+// wavefronts of 64 x 32, two stages, two workgroups). Per K step (32 bytes of
+// every row and column of all S slices) the wavefronts synchronise once:
 //   multiply the first slice pair of step k (fragments in registers); wait
 //   for the copies of step k + 1 and for this wavefront's reads of step k's
-//   fragments; s_barrier; refill the stage of step k with step k + 4;
+//   fragments; s_barrier; refill the stage of step k with step k + NST;
 //   multiply the remaining pairs while reading step k + 1's fragments from
 //   LDS into the registers each pair frees.
 // The pair order (level i ascending, j descending) frees B fragment j after
