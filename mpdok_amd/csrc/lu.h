@@ -29,9 +29,15 @@ int gemm_batched(hipStream_t s, int m, int n, int k, T alpha, const T* A, int64_
                  int64_t sA, const T* B, int64_t ldb, int64_t sB, T beta, T* C, int64_t ldc,
                  int64_t sC, int batch, int round);
 constexpr int TRSV_BLOCK = 512;  // diagonal block of the inverse-based triangular solves
+// tbw: width of the diagonal blocks whose inverses Lbinv / Ubinv hold
+// (TRSV_BLOCK or 2 TRSV_BLOCK); acc: 2 (tbw / 64) N elements.
 template <typename T, typename TV>
 int trsv_block_inverse(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
-                       const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc);
+                       const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc, int tbw);
+// The off-diagonal-free parts of 2 TRSV_BLOCK-wide diagonal blocks of an
+// inverse from its TRSV_BLOCK-wide ones (see lu_kernels.hip).
+template <typename T>
+int widen_block_inverse(hipStream_t s, const T* in, T* out, int nbig, int lower);
 template <typename T>
 int seed_block_inverse(hipStream_t s, const T* inv64, T* binv, int nbig);
 

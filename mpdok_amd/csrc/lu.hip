@@ -39,6 +39,7 @@ struct LUHandle {
   void* Linv = nullptr;
   void* Uinv = nullptr;
   void* W = nullptr;
+  size_t w_elems = 0;  // elements W holds
   int* ipiv = nullptr;
   int* perm = nullptr;
   int* d_info = nullptr;
@@ -51,6 +52,11 @@ struct LUHandle {
   bool fast_solve = false;
   void* Lbinv = nullptr;
   void* Ubinv = nullptr;
+  // 2 TRSV_BLOCK-wide diagonal-block inverses (N a multiple of 2 TRSV_BLOCK):
+  // the solves then take half as many dependent steps (wide_solve)
+  void* Lbinv2 = nullptr;
+  void* Ubinv2 = nullptr;
+  bool wide_solve = false;
   void* v3 = nullptr;
   void* acc = nullptr;
   // lookahead (no pivoting): panel k+1 is factored on `side` while the main
@@ -97,6 +103,7 @@ int lu_resize(LUHandle* h, int n) {
   MPDOK_REQUIRE(n >= 1 && n <= h->cap, "lu_resize: n=%d outside [1, %d]", n, h->cap);
   h->N = n;
   h->fast_solve = (n % TRSV_BLOCK) == 0 && h->Lbinv != nullptr;
+  h->wide_solve = false;
   h->factored = false;
   return OK;
 }
@@ -112,6 +119,7 @@ void lu_destroy(LUHandle* h) {
   if (h->Wp) (void)hipFree(h->Wp);
   void* ptrs[] = {h->LU, h->Linv, h->Uinv, h->W, h->ipiv, h->perm, h->d_info,
                   h->piv_work, h->oz_work, h->oz_work2, h->v1, h->v2, h->Lbinv, h->Ubinv, h->v3,
+                  h->Lbinv2, h->Ubinv2,
                   h->acc,
                   h->d_S};
   for (void* p : ptrs)
@@ -139,7 +147,11 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   h->slices = slices;
   h->es = fp64_storage(precision) ? 8 : 4;
   const size_t nblk = (N + 63) / 64;
-  const size_t wel = (size_t)N * (nb > 128 ? nb : 128);
+  size_t wel = (size_t)N * (nb > 128 ? nb : 128);
+  // the wide inverses' products: TRSV_BLOCK^2 per 2 TRSV_BLOCK-wide block
+  if (N % (2 * TRSV_BLOCK) == 0 && wel < (size_t)N * (TRSV_BLOCK / 2))
+    wel = (size_t)N * (TRSV_BLOCK / 2);
+  h->w_elems = wel;
   int st = OK;
   auto alloc = [&](void** p, size_t bytes) {
     if (st != OK) return;
@@ -183,7 +195,12 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
     alloc(&h->Lbinv, (size_t)N * TRSV_BLOCK * h->es);
     alloc(&h->Ubinv, (size_t)N * TRSV_BLOCK * h->es);
     alloc(&h->v3, (size_t)N * 8);
-    alloc(&h->acc, (size_t)N * 8 * 16);  // 8 accumulators + 8 column-group partials
+    // accumulators + column-group partials: 2 (block width / 64) per row
+    alloc(&h->acc, (size_t)N * 8 * 32);
+    if (N % (2 * TRSV_BLOCK) == 0) {
+      alloc(&h->Lbinv2, (size_t)N * 2 * TRSV_BLOCK * h->es);
+      alloc(&h->Ubinv2, (size_t)N * 2 * TRSV_BLOCK * h->es);
+    }
   }
   if (st == OK && hipMemset(h->piv_work, 0, panel_gepp_workspace_bytes(N)) != hipSuccess) {
     set_error("lu_create: hipMemset failed");
@@ -281,6 +298,38 @@ static int build_block_inverses(LUHandle* h, hipStream_t s, int b0, int nbig, T*
                                 Ub + off + (int64_t)(off + hh) * TBk, TBk, sB, nbig, 0));
     }
   }
+  return OK;
+}
+
+// The 2 TRSV_BLOCK-wide diagonal-block inverses from the TRSV_BLOCK-wide
+// ones, one more level of the doubling batched over all N / (2 TRSV_BLOCK)
+// blocks:
+//   inv([A 0; C D]) = [inv(A) 0; -inv(D) C inv(A) inv(D)]
+//   inv([A B; 0 D]) = [inv(A) -inv(A) B inv(D); 0 inv(D)]
+// W: scratch of TRSV_BLOCK^2 N / (2 TRSV_BLOCK) elements.
+template <typename T>
+static int build_wide_inverses(LUHandle* h, hipStream_t s, T* W) {
+  const int N = h->N, B1 = TRSV_BLOCK, B2 = 2 * TRSV_BLOCK, nbig = N / B2;
+  const int64_t ld = N;
+  const int64_t sLU = (int64_t)B2 * (ld + 1), sB = (int64_t)B2 * B2, sW = (int64_t)B1 * B1;
+  const T* LU = static_cast<const T*>(h->LU);
+  T* L2 = static_cast<T*>(h->Lbinv2);
+  T* U2 = static_cast<T*>(h->Ubinv2);
+  MPDOK_TRY(widen_block_inverse<T>(s, static_cast<const T*>(h->Lbinv), L2, nbig, 1));
+  MPDOK_TRY(widen_block_inverse<T>(s, static_cast<const T*>(h->Ubinv), U2, nbig, 0));
+  MPDOK_REQUIRE((size_t)nbig * sW <= h->w_elems, "lu: scratch of %zu elements below %zu",
+                h->w_elems, (size_t)nbig * sW);
+  const int64_t dD = (int64_t)B1 * (B2 + 1);  // inv(D) inside a wide block
+  // L: T = C inv(A); X21 = -inv(D) T
+  MPDOK_TRY(gemm_batched<T>(s, B1, B1, B1, T(1), LU + B1, ld, sLU, L2, B2, sB, T(0), W, B1, sW,
+                            nbig, 0));
+  MPDOK_TRY(gemm_batched<T>(s, B1, B1, B1, T(-1), L2 + dD, B2, sB, W, B1, sW, T(0), L2 + B1, B2,
+                            sB, nbig, 0));
+  // U: T = B inv(D); X12 = -inv(A) T
+  MPDOK_TRY(gemm_batched<T>(s, B1, B1, B1, T(1), LU + (int64_t)B1 * ld, ld, sLU, U2 + dD, B2, sB,
+                            T(0), W, B1, sW, nbig, 0));
+  MPDOK_TRY(gemm_batched<T>(s, B1, B1, B1, T(-1), U2, B2, sB, W, B1, sW, T(0),
+                            U2 + (int64_t)B1 * B2, B2, sB, nbig, 0));
   return OK;
 }
 
@@ -574,6 +623,8 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
     if (N > nb) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
     if (h->fast_solve && nb != TRSV_BLOCK)
       MPDOK_TRY(build_block_inverses<T>(h, s, 0, N / TRSV_BLOCK, W));
+    h->wide_solve = h->fast_solve && h->Lbinv2 != nullptr && N % (2 * TRSV_BLOCK) == 0;
+    if (h->wide_solve) MPDOK_TRY(build_wide_inverses<T>(h, s, W));
     return OK;
   }
   for (int k = 0; k < N; k += nb) {
@@ -589,6 +640,8 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
   if (h->pivot == PIV_PARTIAL) MPDOK_TRY(ipiv_to_perm(s, h->ipiv, h->perm, N));
   if (h->fast_solve && !(h->pivot == PIV_NONE && nb == TRSV_BLOCK))
     MPDOK_TRY(build_block_inverses<T>(h, s, 0, N / TRSV_BLOCK, W));
+  h->wide_solve = h->fast_solve && h->Lbinv2 != nullptr && N % (2 * TRSV_BLOCK) == 0;
+  if (h->wide_solve) MPDOK_TRY(build_wide_inverses<T>(h, s, W));
   return OK;
 }
 
@@ -653,10 +706,12 @@ static int solve_impl(LUHandle* h, hipStream_t s, const double* b, double* x) {
   }
   if (h->fast_solve) {
     // v: permuted right-hand side; result into v2 (v3 holds the forward result)
-    MPDOK_TRY((trsv_block_inverse<T, TV>(s, static_cast<const T*>(h->LU), N, N,
-                                         static_cast<const T*>(h->Lbinv),
-                                         static_cast<const T*>(h->Ubinv), v, v2,
-                                         static_cast<TV*>(h->v3), static_cast<TV*>(h->acc))));
+    MPDOK_TRY((trsv_block_inverse<T, TV>(
+        s, static_cast<const T*>(h->LU), N, N,
+        static_cast<const T*>(h->wide_solve ? h->Lbinv2 : h->Lbinv),
+        static_cast<const T*>(h->wide_solve ? h->Ubinv2 : h->Ubinv), v, v2,
+        static_cast<TV*>(h->v3), static_cast<TV*>(h->acc),
+        h->wide_solve ? 2 * TRSV_BLOCK : TRSV_BLOCK)));
     v = v2;
   } else {
     MPDOK_TRY((trsv_blocked<T, TV>(s, static_cast<const T*>(h->LU), N, N,

@@ -803,24 +803,26 @@ template int trsv_blocked<float, double>(hipStream_t, const float*, int64_t, int
                                          const float*, double*);
 
 // ---------------------------------------------------------------------------
-// Triangular solves with explicit inverses of TB x TB diagonal blocks
-// (TB = 512). One step per diagonal block, two kernels per step:
+// Triangular solves with explicit inverses of TBW x TBW diagonal blocks
+// (TBW = 512 or 1024). One step per diagonal block, two kernels per step:
 //   diag_apply : x_i = inv(D_i) (v_i - sum_c acc[c, rows of i])
 //   tri_acc    : acc[c, r] += sum_{j in chunk c of block i} M[r, j] x_j  for the
 //                rows r still to be solved (below i forward, above i backward)
-// Every (c, r) accumulator is written by one thread per step in stream order,
-// so the result does not depend on scheduling.
+// Column chunks are 64 wide (TBW / 64 of them). Every (c, r) accumulator is
+// written by one thread per step in stream order, so the result does not
+// depend on scheduling. Wider blocks halve the number of dependent steps.
 // ---------------------------------------------------------------------------
-constexpr int TB = 512, TB_CH = 8, TB_CW = TB / TB_CH;
+constexpr int TB_CW = 64;
 
 // xpart[cg, r0 + rg*64 + r] = sum_{j < 64} inv(D)[rg*64 + r, cg*64 + j] v'[cg*64 + j]
 // with v' = v - sum_c acc[c]; one workgroup per (row group, column group) tile;
 // tiles of the zero triangle are skipped.
-template <typename T, typename TV>
+template <int TBW, typename T, typename TV>
 __global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ Binv, int r0,
                                                          const TV* __restrict__ v,
                                                          const TV* __restrict__ acc, int N,
                                                          TV* __restrict__ xpart, int lower) {
+  constexpr int TB_CH = TBW / TB_CW;
   __shared__ TV vs[TB_CW];
   __shared__ TV red[4][64];
   const int rg = blockIdx.x, cg = blockIdx.y, t = threadIdx.x;
@@ -831,10 +833,10 @@ __global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ B
   }
   // the inverse's entries do not depend on v': their loads go out first
   const int r = t & 63, p = t >> 6;
-  const T* col = Binv + rg * 64 + r + (size_t)(cg * TB_CW) * TB;
+  const T* col = Binv + rg * 64 + r + (size_t)(cg * TB_CW) * TBW;
   T bv[TB_CW / 4];
 #pragma unroll
-  for (int q = 0; q < TB_CW / 4; ++q) bv[q] = col[(size_t)(p + 4 * q) * TB];
+  for (int q = 0; q < TB_CW / 4; ++q) bv[q] = col[(size_t)(p + 4 * q) * TBW];
   if (t < TB_CW) {
     const int g = r0 + cg * TB_CW + t;
     TV s = v[g];
@@ -851,15 +853,16 @@ __global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ B
   if (t < 64) out[t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
 }
 
-// Completes x for columns [c0, c0 + TB) from the column-group partials, and
+// Completes x for columns [c0, c0 + TBW) from the column-group partials, and
 // adds the products of those columns into the accumulators of rows
 // [rbeg, rend). Workgroups with blockIdx.x == 0 store the finished x values.
-template <typename T, typename TV>
+template <int TBW, typename T, typename TV>
 __global__ __launch_bounds__(256) void tri_acc_kernel(const T* __restrict__ M, int64_t ldm,
                                                       int rbeg, int rend, int c0,
                                                       const TV* __restrict__ xpart,
                                                       TV* __restrict__ x, TV* __restrict__ acc,
                                                       int N) {
+  constexpr int TB_CH = TBW / TB_CW;
   __shared__ TV xs[TB_CW];
   const int c = blockIdx.y;
   const int cb = c0 + c * TB_CW;
@@ -896,45 +899,92 @@ __global__ __launch_bounds__(256) void tri_acc_kernel(const T* __restrict__ M, i
   acc[(size_t)c * N + r] = a0 + a;
 }
 
-template <typename T, typename TV>
-int trsv_block_inverse(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
-                       const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc) {
-  MPDOK_REQUIRE(N % TB == 0, "trsv_block_inverse: N must be a multiple of %d", TB);
-  const int nb = N / TB;
+template <int TBW, typename T, typename TV>
+static int trsv_block_inverse_w(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
+                                const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc) {
+  constexpr int TB_CH = TBW / TB_CW;
+  MPDOK_REQUIRE(N % TBW == 0, "trsv_block_inverse: N must be a multiple of %d", TBW);
+  const int nb = N / TBW;
   TV* xpart = acc + (size_t)TB_CH * N;  // second half of the accumulator buffer
-  const dim3 dgrid(TB / 64, TB_CH);
+  const dim3 dgrid(TBW / 64, TB_CH);
   // forward: L y = b, y -> v
   MPDOK_HIP(hipMemsetAsync(acc, 0, sizeof(TV) * (size_t)TB_CH * N, s));
   for (int i = 0; i < nb; ++i) {
-    const int r0 = i * TB;
-    diag_apply_kernel<T, TV><<<dgrid, 256, 0, s>>>(Lbinv + (size_t)i * TB * TB, r0, b, acc, N,
-                                                   xpart, 1);
-    tri_acc_kernel<T, TV><<<dim3(r0 + TB < N ? (N - r0 - TB + 255) / 256 : 1, TB_CH), 256, 0, s>>>(
-        LU, ld, r0 + TB, N, r0, xpart, v, acc, N);
+    const int r0 = i * TBW;
+    diag_apply_kernel<TBW, T, TV><<<dgrid, 256, 0, s>>>(Lbinv + (size_t)i * TBW * TBW, r0, b, acc,
+                                                        N, xpart, 1);
+    tri_acc_kernel<TBW, T, TV>
+        <<<dim3(r0 + TBW < N ? (N - r0 - TBW + 255) / 256 : 1, TB_CH), 256, 0, s>>>(
+            LU, ld, r0 + TBW, N, r0, xpart, v, acc, N);
   }
   // backward: U x = y
   MPDOK_HIP(hipMemsetAsync(acc, 0, sizeof(TV) * (size_t)TB_CH * N, s));
   for (int i = nb - 1; i >= 0; --i) {
-    const int r0 = i * TB;
-    diag_apply_kernel<T, TV><<<dgrid, 256, 0, s>>>(Ubinv + (size_t)i * TB * TB, r0, v, acc, N,
-                                                   xpart, 0);
-    tri_acc_kernel<T, TV><<<dim3(r0 > 0 ? (r0 + 255) / 256 : 1, TB_CH), 256, 0, s>>>(
+    const int r0 = i * TBW;
+    diag_apply_kernel<TBW, T, TV><<<dgrid, 256, 0, s>>>(Ubinv + (size_t)i * TBW * TBW, r0, v, acc,
+                                                        N, xpart, 0);
+    tri_acc_kernel<TBW, T, TV><<<dim3(r0 > 0 ? (r0 + 255) / 256 : 1, TB_CH), 256, 0, s>>>(
         LU, ld, 0, r0, r0, xpart, x, acc, N);
   }
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
+
+template <typename T, typename TV>
+int trsv_block_inverse(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
+                       const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc, int tbw) {
+  if (tbw == 2 * TRSV_BLOCK)
+    return trsv_block_inverse_w<2 * TRSV_BLOCK, T, TV>(s, LU, ld, N, Lbinv, Ubinv, b, x, v, acc);
+  MPDOK_REQUIRE(tbw == TRSV_BLOCK, "trsv_block_inverse: block width %d", tbw);
+  return trsv_block_inverse_w<TRSV_BLOCK, T, TV>(s, LU, ld, N, Lbinv, Ubinv, b, x, v, acc);
+}
 template int trsv_block_inverse<double, double>(hipStream_t, const double*, int64_t, int,
                                                 const double*, const double*, const double*,
-                                                double*, double*, double*);
+                                                double*, double*, double*, int);
 template int trsv_block_inverse<float, float>(hipStream_t, const float*, int64_t, int, const float*,
-                                              const float*, const float*, float*, float*, float*);
+                                              const float*, const float*, float*, float*, float*,
+                                              int);
 
-// Place the 64 x 64 diagonal inverses into the TB x TB blocks (zero elsewhere).
+// 1024-wide diagonal blocks of the inverses from the 512-wide ones: block b
+// takes inv(A) = in[2b] top-left and inv(D) = in[2b + 1] bottom-right; the
+// off-diagonal block of the zero triangle (top-right for a lower inverse,
+// bottom-left for an upper one) is zero, and the other off-diagonal block is
+// left for the doubling products. One thread per element, nbig blocks.
+template <typename T>
+__global__ __launch_bounds__(256) void widen_block_inverse_kernel(const T* __restrict__ in,
+                                                                  T* __restrict__ out, int nbig,
+                                                                  int lower) {
+  constexpr int B1 = TRSV_BLOCK, B2 = 2 * TRSV_BLOCK;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)nbig * B2 * B2) return;
+  const int b = (int)(e / ((size_t)B2 * B2));
+  const int rc = (int)(e % ((size_t)B2 * B2));
+  const int r = rc % B2, c = rc / B2;
+  const int hr = r / B1, hc = c / B1;
+  if (hr == hc) {
+    out[e] = in[(size_t)(2 * b + hr) * B1 * B1 + (r % B1) + (size_t)(c % B1) * B1];
+  } else if (lower ? hc > hr : hc < hr) {
+    out[e] = T(0);
+  }
+}
+
+template <typename T>
+int widen_block_inverse(hipStream_t s, const T* in, T* out, int nbig, int lower) {
+  if (nbig <= 0) return OK;
+  const size_t n = (size_t)nbig * 4 * TRSV_BLOCK * TRSV_BLOCK;
+  widen_block_inverse_kernel<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(in, out, nbig, lower);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int widen_block_inverse<double>(hipStream_t, const double*, double*, int, int);
+template int widen_block_inverse<float>(hipStream_t, const float*, float*, int, int);
+
+// Place the 64 x 64 diagonal inverses into the TRSV_BLOCK-wide blocks (zero elsewhere).
 template <typename T>
 __global__ __launch_bounds__(256) void seed_block_inverse_kernel(const T* __restrict__ inv64,
                                                                  T* __restrict__ binv, int nbig) {
-  // blockIdx.x: TB x TB block index; blockIdx.y: column of that block
+  // blockIdx.x: TRSV_BLOCK-wide block index; blockIdx.y: column of that block
+  constexpr int TB = TRSV_BLOCK;
   const int b = blockIdx.x, col = blockIdx.y;
   T* dst = binv + (size_t)b * TB * TB + (size_t)col * TB;
   const int g = b * (TB / 64) + col / 64;  // global 64-block index
@@ -948,7 +998,7 @@ __global__ __launch_bounds__(256) void seed_block_inverse_kernel(const T* __rest
 template <typename T>
 int seed_block_inverse(hipStream_t s, const T* inv64, T* binv, int nbig) {
   if (nbig <= 0) return OK;
-  seed_block_inverse_kernel<T><<<dim3(nbig, TB), 256, 0, s>>>(inv64, binv, nbig);
+  seed_block_inverse_kernel<T><<<dim3(nbig, TRSV_BLOCK), 256, 0, s>>>(inv64, binv, nbig);
   MPDOK_HIP(hipGetLastError());
   return OK;
 }

@@ -140,13 +140,14 @@ def test_ozaki_row_slicing_variants_bit_exact(L, variant, S, m, n, K):
 
 
 @pytest.mark.parametrize("variant", [0, 1])
-@pytest.mark.parametrize("N", [17, 64, 130, 512])
+@pytest.mark.parametrize("N", [17, 64, 130, 512, 2048])
 def test_diag_block_kernels_factor_and_inverses(L, variant, N):
     """Both 64 x 64 diagonal-block kernels (tuning key 3: blocked with matrix-core
     doubling, or unblocked): with nb = 64 every panel step is one diagonal
     block, so the packed factors must match the oracle's unpivoted LU, and the
-    solves, which apply the stored 64 x 64 inverses (and, at N = 512, the
-    512-wide ones doubled from them), must reproduce numpy's solution."""
+    solves, which apply the stored 64 x 64 inverses (at N = 512 the 512-wide
+    ones doubled from them, at N = 2048 the 1024-wide ones doubled once more),
+    must reproduce numpy's solution."""
     rng = np.random.default_rng(N + 10 * variant)
     A = rng.standard_normal((N, N)) + N * np.eye(N)
     assert L.mpdok_set_tuning(3, variant) == 0
