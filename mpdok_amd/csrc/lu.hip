@@ -149,7 +149,7 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   const size_t nblk = (N + 63) / 64;
   size_t wel = (size_t)N * (nb > 128 ? nb : 128);
   // the wide inverses' products: TRSV_BLOCK^2 per 2 TRSV_BLOCK-wide block
-  if (N % (2 * TRSV_BLOCK) == 0 && wel < (size_t)N * (TRSV_BLOCK / 2))
+  if (N >= 2 * TRSV_BLOCK && wel < (size_t)N * (TRSV_BLOCK / 2))
     wel = (size_t)N * (TRSV_BLOCK / 2);
   h->w_elems = wel;
   int st = OK;
@@ -197,7 +197,9 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
     alloc(&h->v3, (size_t)N * 8);
     // accumulators + column-group partials: 2 (block width / 64) per row
     alloc(&h->acc, (size_t)N * 8 * 32);
-    if (N % (2 * TRSV_BLOCK) == 0) {
+    // whenever a size up to N can use them, so that the solve a system gets
+    // depends on its own size only, not on the capacity of the handle
+    if (N >= 2 * TRSV_BLOCK) {
       alloc(&h->Lbinv2, (size_t)N * 2 * TRSV_BLOCK * h->es);
       alloc(&h->Ubinv2, (size_t)N * 2 * TRSV_BLOCK * h->es);
     }
