@@ -252,8 +252,11 @@ static bool panel_block_inverse(const LUHandle* h, int k, int kb) {
 //   inv([A B; 0 D]) = [inv(A) -inv(A) B inv(D); 0  inv(D)]
 // batched over all big blocks (uniform strides along the diagonal).
 // Blocks b0 .. b0 + nbig - 1; W: scratch of (TRSV_BLOCK / 2)^2 nbig elements.
+// parts: INV_L, INV_U or both.
+constexpr int INV_L = 1, INV_U = 2, INV_LU = 3;
 template <typename T>
-static int build_block_inverses(LUHandle* h, hipStream_t s, int b0, int nbig, T* W) {
+static int build_block_inverses(LUHandle* h, hipStream_t s, int b0, int nbig, T* W,
+                                int parts = INV_LU) {
   const int N = h->N, TBk = TRSV_BLOCK;
   if (nbig <= 0) return OK;
   const int64_t ld = N;
@@ -262,8 +265,9 @@ static int build_block_inverses(LUHandle* h, hipStream_t s, int b0, int nbig, T*
   T* Lb = static_cast<T*>(h->Lbinv) + b0 * sB;
   T* Ub = static_cast<T*>(h->Ubinv) + b0 * sB;
   const size_t i64 = (size_t)b0 * (TBk / 64) * 4096;
-  MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Linv) + i64, Lb, nbig));
-  MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Uinv) + i64, Ub, nbig));
+  const bool doL = parts & INV_L, doU = parts & INV_U;
+  if (doL) MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Linv) + i64, Lb, nbig));
+  if (doU) MPDOK_TRY(seed_block_inverse<T>(s, static_cast<const T*>(h->Uinv) + i64, Ub, nbig));
   for (int hh = 64; hh < TBk; hh *= 2) {
     const int64_t sW = (int64_t)hh * hh;
     if (nbig == 1) {
@@ -273,14 +277,19 @@ static int build_block_inverses(LUHandle* h, hipStream_t s, int b0, int nbig, T*
       const int64_t pLU = 2 * (int64_t)hh * (ld + 1), pB = 2 * (int64_t)hh * (TBk + 1);
       const int64_t dD = (int64_t)hh * (TBk + 1);
       const T* C = LU + hh;
-      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), C, ld, pLU, Lb, TBk, pB, T(0), W, hh, sW, np, 0));
-      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Lb + dD, TBk, pB, W, hh, sW, T(0), Lb + hh,
-                                TBk, pB, np, 0));
+      if (doL) {
+        MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), C, ld, pLU, Lb, TBk, pB, T(0), W, hh, sW, np,
+                                  0));
+        MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Lb + dD, TBk, pB, W, hh, sW, T(0), Lb + hh,
+                                  TBk, pB, np, 0));
+      }
       const T* Bm = LU + (int64_t)hh * ld;
-      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), Bm, ld, pLU, Ub + dD, TBk, pB, T(0), W, hh, sW,
-                                np, 0));
-      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Ub, TBk, pB, W, hh, sW, T(0),
-                                Ub + (int64_t)hh * TBk, TBk, pB, np, 0));
+      if (doU) {
+        MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), Bm, ld, pLU, Ub + dD, TBk, pB, T(0), W, hh,
+                                  sW, np, 0));
+        MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Ub, TBk, pB, W, hh, sW, T(0),
+                                  Ub + (int64_t)hh * TBk, TBk, pB, np, 0));
+      }
       continue;
     }
     for (int off = 0; off < TBk; off += 2 * hh) {
@@ -288,16 +297,20 @@ static int build_block_inverses(LUHandle* h, hipStream_t s, int b0, int nbig, T*
       const int64_t dD = (int64_t)(off + hh) * (TBk + 1);  // inv(D)
       // L: T = C inv(A); X21 = -inv(D) T
       const T* C = LU + (off + hh) + (int64_t)off * ld;
-      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), C, ld, sLU, Lb + dA, TBk, sB, T(0), W, hh, sW,
-                                nbig, 0));
-      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Lb + dD, TBk, sB, W, hh, sW, T(0),
-                                Lb + (off + hh) + (int64_t)off * TBk, TBk, sB, nbig, 0));
+      if (doL) {
+        MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), C, ld, sLU, Lb + dA, TBk, sB, T(0), W, hh,
+                                  sW, nbig, 0));
+        MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Lb + dD, TBk, sB, W, hh, sW, T(0),
+                                  Lb + (off + hh) + (int64_t)off * TBk, TBk, sB, nbig, 0));
+      }
       // U: T = B inv(D); X12 = -inv(A) T
       const T* Bm = LU + off + (int64_t)(off + hh) * ld;
-      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), Bm, ld, sLU, Ub + dD, TBk, sB, T(0), W, hh, sW,
-                                nbig, 0));
-      MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Ub + dA, TBk, sB, W, hh, sW, T(0),
-                                Ub + off + (int64_t)(off + hh) * TBk, TBk, sB, nbig, 0));
+      if (doU) {
+        MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(1), Bm, ld, sLU, Ub + dD, TBk, sB, T(0), W, hh,
+                                  sW, nbig, 0));
+        MPDOK_TRY(gemm_batched<T>(s, hh, hh, hh, T(-1), Ub + dA, TBk, sB, W, hh, sW, T(0),
+                                  Ub + off + (int64_t)(off + hh) * TBk, TBk, sB, nbig, 0));
+      }
     }
   }
   return OK;
@@ -343,7 +356,8 @@ static int build_wide_inverses(LUHandle* h, hipStream_t s, T* W) {
 // follow by recursive doubling: the block-row solve of the next step
 // multiplies by the L one. W: scratch (the side stream's, under lookahead).
 template <typename T>
-static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W, bool slice = true) {
+static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W, bool slice = true,
+                        int inv_parts = INV_LU) {
   const int N = h->N;
   T* A = static_cast<T*>(h->LU);
   T* Linv = static_cast<T*>(h->Linv);
@@ -377,7 +391,7 @@ static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W, bool sl
     }
   }
   if (panel_block_inverse(h, k, kb))
-    MPDOK_TRY(build_block_inverses<T>(h, s, k / TRSV_BLOCK, 1, W));
+    MPDOK_TRY(build_block_inverses<T>(h, s, k / TRSV_BLOCK, 1, W, inv_parts));
   // int8-sliced variant: the rows of L21 for this step's trailing update
   const int n3 = N - k - kb;
   if (slice && h->precision == LU_OZAKI && n3 > 0)
@@ -482,7 +496,9 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
 // [k, N), right-looking inside the group: each panel, then the block-row
 // solve of its rows over the group's later columns and their FP64 update.
 // The int8-sliced variant then slices the rows of the group's L21 (K = kbs)
-// for the group's trailing update.
+// for the group's trailing update. Of the panels' TRSV_BLOCK-wide inverses
+// only the L ones are built here (the block-row solves need them); the U ones
+// serve only the solves after the factorisation (group_u_inverses).
 template <typename T>
 static int panel_group_factor(LUHandle* h, hipStream_t s, int k, int kbs, T* W) {
   const int N = h->N, nb = h->nb;
@@ -491,7 +507,7 @@ static int panel_group_factor(LUHandle* h, hipStream_t s, int k, int kbs, T* W) 
   auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
   for (int j = k; j < k + kbs; j += nb) {
     const int jb = nb < k + kbs - j ? nb : k + kbs - j;
-    MPDOK_TRY(panel_factor<T>(h, s, j, jb, W, false));
+    MPDOK_TRY(panel_factor<T>(h, s, j, jb, W, false, INV_L));
     const int rem = k + kbs - j - jb;
     if (rem > 0) {
       MPDOK_TRY(block_row_solve<T>(h, s, j, jb, j + jb, W, rem));
@@ -504,6 +520,17 @@ static int panel_group_factor(LUHandle* h, hipStream_t s, int k, int kbs, T* W) 
     MPDOK_TRY(ozaki_slice(s, h->slices, n3, n3, kbs,
                           reinterpret_cast<const double*>(at(k + kbs, k)), ld, nullptr, ld,
                           oz_ws(h, k), 1));
+  return OK;
+}
+
+// The TRSV_BLOCK-wide U inverses of the panels of the group [k, k + kbs).
+template <typename T>
+static int group_u_inverses(LUHandle* h, hipStream_t s, int k, int kbs, T* W) {
+  for (int j = k; j < k + kbs; j += h->nb) {
+    const int jb = h->nb < k + kbs - j ? h->nb : k + kbs - j;
+    if (panel_block_inverse(h, j, jb))
+      MPDOK_TRY(build_block_inverses<T>(h, s, j / TRSV_BLOCK, 1, W, INV_U));
+  }
   return OK;
 }
 
@@ -585,18 +612,27 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
     MPDOK_HIP(hipStreamWaitEvent(h->side, h->ev_next_cols, 0));
     MPDOK_TRY(panel_group_factor<T>(h, h->side, 0, gs[1], Wp));
     MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
-    if (rest_A) MPDOK_TRY(copy_columns(h, s, rest_A, rest_lda, rest_order, gs[1], N));
     // row lookahead for group g (see LUHandle::rows_side_min); the first
     // group's rows stay on the main stream (its copy runs there)
     auto rows_on_side = [&](size_t g) {
       return g >= 1 && g + 1 < gs.size() && gs[g + 1] < N && h->rows_side_min > 0 &&
              N - gs[g] >= h->rows_side_min && (adaptive_on(h) || h->precision != LU_OZAKI);
     };
+    // The U inverses of group g go to the stream that waits for the other:
+    // the lookahead stream while the trailing updates are long (the first
+    // group and the row-lookahead steps), the main stream, after its part of
+    // the step, once it waits for each group's panels.
+    auto u_on_main = [&](size_t g) { return g >= 1 && !rows_on_side(g); };
+    MPDOK_TRY(group_u_inverses<T>(h, h->side, 0, gs[1], Wp));
+    if (rest_A) MPDOK_TRY(copy_columns(h, s, rest_A, rest_lda, rest_order, gs[1], N));
     for (size_t g = 0; g + 1 < gs.size(); ++g) {
       const int k = gs[g], kb = gs[g + 1] - k;
       const int n3 = N - k - kb;
       MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
-      if (n3 <= 0) break;
+      if (n3 <= 0) {
+        if (u_on_main(g)) MPDOK_TRY(group_u_inverses<T>(h, s, k, kb, W));
+        break;
+      }
       if (!rows_on_side(g)) {
         MPDOK_TRY(block_rows_solve<T>(h, s, k, kb, k + kb, W));
         if (h->precision == LU_OZAKI)
@@ -621,8 +657,12 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
                                reinterpret_cast<const double*>(at(k2, k2 + nk)), ld));
       }
       MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
+      if (!u_on_main(g + 1)) MPDOK_TRY(group_u_inverses<T>(h, h->side, k + kb, nk, Wp));
+      if (u_on_main(g)) MPDOK_TRY(group_u_inverses<T>(h, s, k, kb, W));
     }
-    if (N > nb) MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
+    // everything the lookahead stream holds, U inverses included
+    MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
+    MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
     if (h->fast_solve && nb != TRSV_BLOCK)
       MPDOK_TRY(build_block_inverses<T>(h, s, 0, N / TRSV_BLOCK, W));
     h->wide_solve = h->fast_solve && h->Lbinv2 != nullptr && N % (2 * TRSV_BLOCK) == 0;
