@@ -7,6 +7,23 @@ namespace mpdok {
 template <typename T>
 int gemm(hipStream_t s, int m, int n, int k, T alpha, const T* A, int64_t lda, const T* B,
          int64_t ldb, T beta, T* C, int64_t ldc, int round);
+// One product C = alpha A B + beta C (column-major) of a grouped launch.
+template <typename T>
+struct GemmArgs {
+  int m, n, k;
+  T alpha;
+  const T* A;
+  int64_t lda;
+  const T* B;
+  int64_t ldb;
+  T beta;
+  T* C;
+  int64_t ldc;
+};
+// Two independent products in one launch (64 x 64 tiles, so the in-place
+// aliasing rules of gemm hold for a side of at most 64).
+template <typename T>
+int gemm_pair(hipStream_t s, const GemmArgs<T>& p, const GemmArgs<T>& q);
 template <typename T>
 int diag_factor(hipStream_t s, T* A, int64_t lda, int nb, T* Linv, T* Uinv, int ldi, int* info,
                 int col0, int factor);

@@ -368,23 +368,26 @@ static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W, bool sl
     const int jb = 64 < k + kb - j ? 64 : k + kb - j;
     T* Li = Linv + (size_t)(j / 64) * 4096;
     T* Ui = Uinv + (size_t)(j / 64) * 4096;
-    const int m2 = N - j - jb;
+    const int m2 = N - j - jb, n2 = k + kb - j - jb;
     if (h->pivot == PIV_PARTIAL) {
       MPDOK_TRY(panel_gepp<T>(s, A, ld, N, j, jb, h->ipiv, h->d_info, h->piv_work));
       MPDOK_TRY(laswp<T>(s, A, ld, 0, j, h->ipiv, j, j + jb));
       MPDOK_TRY(laswp<T>(s, A, ld, j + jb, N, h->ipiv, j, j + jb));
       MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 0));
+      // U12 = inv(L11) A12, in place (m = jb <= 64: each workgroup its own columns)
+      if (n2 > 0)
+        MPDOK_TRY(gemm<T>(s, jb, n2, jb, T(1), Li, 64, at(j, j + jb), ld, T(0), at(j, j + jb), ld,
+                          0));
     } else {
       MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 1));
-      // L21 = A21 inv(U11), in place (n = jb <= 64: each workgroup its own rows)
-      if (m2 > 0)
-        MPDOK_TRY(gemm<T>(s, m2, jb, jb, T(1), at(j + jb, j), ld, Ui, 64, T(0), at(j + jb, j), ld,
-                          0));
+      // in place, in one launch: L21 = A21 inv(U11) (n = jb <= 64: each
+      // workgroup its own rows) and U12 = inv(L11) A12 (m = jb <= 64: its own
+      // columns); neither reads what the other writes
+      const GemmArgs<T> l21{m2, jb, jb, T(1), at(j + jb, j), ld, Ui, 64, T(0), at(j + jb, j), ld};
+      const GemmArgs<T> u12{jb, n2, jb, T(1), Li, 64, at(j, j + jb), ld, T(0), at(j, j + jb), ld};
+      MPDOK_TRY(gemm_pair<T>(s, l21, u12));
     }
-    const int n2 = k + kb - j - jb;
     if (n2 > 0) {
-      // U12 = inv(L11) A12, in place (m = jb <= 64: each workgroup its own columns)
-      MPDOK_TRY(gemm<T>(s, jb, n2, jb, T(1), Li, 64, at(j, j + jb), ld, T(0), at(j, j + jb), ld, 0));
       if (m2 > 0)
         MPDOK_TRY(gemm<T>(s, m2, n2, jb, T(-1), at(j + jb, j), ld, at(j, j + jb), ld, T(1),
                           at(j + jb, j + jb), ld, 0));

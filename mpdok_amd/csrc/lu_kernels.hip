@@ -72,11 +72,13 @@ constexpr int GBK = 16, GPAD = 16;
 // operand happens in the K loop, before the workgroup's stores, and beta = 0
 // leaves C unread. The panel uses this for its in-place products with the
 // diagonal-block inverses.
+//
+// gemm_tile: the tile of C that workgroup b of a launch computes (the body of
+// both kernels below).
 template <typename T, int ROUND, int TB>
-__global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alpha, const T* A,
-                                                   int64_t lda, const T* B, int64_t ldb, T beta,
-                                                   T* C, int64_t ldc, int64_t sA_, int64_t sB_,
-                                                   int64_t sC_) {
+__device__ __forceinline__ void gemm_tile(int b, int m, int n, int k, T alpha, const T* A,
+                                          int64_t lda, const T* B, int64_t ldb, T beta, T* C,
+                                          int64_t ldc) {
   constexpr int KPH = 256 / TB;      // k phases of the A loads
   constexpr int NA = GBK / KPH;      // A elements per thread
   constexpr int NBL = TB / 16;       // B elements per thread
@@ -87,12 +89,8 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alp
   // 16 consecutive columns, both fall into distinct banks.
   __shared__ T sB[2][TB][GBK + 1];
   typedef typename Mfma<T>::acc_t acc_t;
-  // batched use: blockIdx.y selects the problem; blockIdx.x the tile
-  A += blockIdx.y * sA_;
-  B += blockIdx.y * sB_;
-  C += blockIdx.y * sC_;
   int ti, tj;
-  if (!tile_of(blockIdx.x, (m + TB - 1) / TB, (n + TB - 1) / TB, ti, tj)) return;
+  if (!tile_of(b, (m + TB - 1) / TB, (n + TB - 1) / TB, ti, tj)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * (TB / 2), wn = (wave >> 1) * (TB / 2);
   const int bm = ti * TB, bn = tj * TB;
@@ -176,6 +174,27 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alp
     }
 }
 
+// Batched use: blockIdx.y selects the problem (operands at strides sA_, sB_,
+// sC_); blockIdx.x the tile.
+template <typename T, int ROUND, int TB>
+__global__ __launch_bounds__(256, 2) void gemm_kernel(int m, int n, int k, T alpha, const T* A,
+                                                   int64_t lda, const T* B, int64_t ldb, T beta,
+                                                   T* C, int64_t ldc, int64_t sA_, int64_t sB_,
+                                                   int64_t sC_) {
+  gemm_tile<T, ROUND, TB>(blockIdx.x, m, n, k, alpha, A + blockIdx.y * sA_, lda,
+                          B + blockIdx.y * sB_, ldb, beta, C + blockIdx.y * sC_, ldc);
+}
+
+// Two unrelated products in one launch (blockIdx.y picks p or q), 64 x 64
+// tiles: the panel's L21 = A21 inv(U11) and U12 = inv(L11) A12 of a
+// sub-panel, which both wait for its diagonal block and for nothing else.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void gemm_pair_kernel(GemmArgs<T> p, GemmArgs<T> q) {
+  const GemmArgs<T> g = blockIdx.y == 0 ? p : q;
+  gemm_tile<T, 0, 64>(blockIdx.x, g.m, g.n, g.k, g.alpha, g.A, g.lda, g.B, g.ldb, g.beta, g.C,
+                      g.ldc);
+}
+
 template <typename T, int TB>
 static void gemm_launch(hipStream_t s, dim3 grid, int round, int m, int n, int k, T alpha,
                         const T* A, int64_t lda, int64_t sA, const T* B, int64_t ldb, int64_t sB,
@@ -218,6 +237,23 @@ int gemm(hipStream_t s, int m, int n, int k, T alpha, const T* A, int64_t lda, c
 }
 template int gemm<double>(hipStream_t, int, int, int, double, const double*, int64_t,
                           const double*, int64_t, double, double*, int64_t, int);
+
+template <typename T>
+int gemm_pair(hipStream_t s, const GemmArgs<T>& p, const GemmArgs<T>& q) {
+  const bool ep = p.m <= 0 || p.n <= 0, eq = q.m <= 0 || q.n <= 0;
+  if (ep || eq) {
+    const GemmArgs<T>& g = ep ? q : p;
+    if (ep && eq) return OK;
+    return gemm<T>(s, g.m, g.n, g.k, g.alpha, g.A, g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, 0);
+  }
+  const int tp = tiles_launch((p.m + 63) / 64, (p.n + 63) / 64);
+  const int tq = tiles_launch((q.m + 63) / 64, (q.n + 63) / 64);
+  gemm_pair_kernel<T><<<dim3(tp > tq ? tp : tq, 2), 256, 0, s>>>(p, q);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+template int gemm_pair<double>(hipStream_t, const GemmArgs<double>&, const GemmArgs<double>&);
+template int gemm_pair<float>(hipStream_t, const GemmArgs<float>&, const GemmArgs<float>&);
 template int gemm<float>(hipStream_t, int, int, int, float, const float*, int64_t, const float*,
                          int64_t, float, float*, int64_t, int);
 template int gemm_batched<double>(hipStream_t, int, int, int, double, const double*, int64_t,
