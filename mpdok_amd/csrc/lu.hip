@@ -52,8 +52,9 @@ struct LUHandle {
   bool fast_solve = false;
   void* Lbinv = nullptr;
   void* Ubinv = nullptr;
-  // 2 TRSV_BLOCK-wide diagonal-block inverses (N a multiple of 2 TRSV_BLOCK):
-  // the solves then take half as many dependent steps (wide_solve)
+  // 2 TRSV_BLOCK-wide diagonal-block inverses (allocated for N >= 2
+  // TRSV_BLOCK, used when N is a multiple of it): the solves then take half
+  // as many dependent steps (wide_solve)
   void* Lbinv2 = nullptr;
   void* Ubinv2 = nullptr;
   bool wide_solve = false;

@@ -190,6 +190,31 @@ def test_row_lookahead_gives_the_same_factors(L, precision, N, nb):
     assert np.array_equal(out[0][2], out[1][2])
 
 
+@pytest.mark.parametrize("rows", [0, 1024])
+def test_wide_block_solve_matches_own_factors(L, rows):
+    """Panels of 512 columns in pairs, N a multiple of 1024: the 512-wide U
+    inverses are built on the main stream (rows = 0) or on the lookahead
+    stream (rows = 1024, row lookahead throughout), the 1024-wide ones after
+    the factorisation, and the solve that applies them must match triangular
+    solves with the handle's own factors."""
+    N = 4096
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((N, N))
+    A = X.T @ X / N + np.eye(N)
+    from mpdok_amd.mpdok_ops import _LUHandle
+    h = _LUHandle(L, N, "ozaki", "none", 512, 5, True)
+    h.set_row_lookahead(rows)
+    assert h.factor(dev(A)) == 0
+    assert h.step_starts() == list(range(0, N, 1024))
+    LU, _ = _factors(L, h, N)
+    b = rng.standard_normal(N)
+    x = h.solve(dev(b)).cpu().numpy()
+    h.close()
+    Lm, Um = olu.unpack(LU)
+    xr = sla.solve_triangular(Um, sla.solve_triangular(Lm, b, lower=True, unit_diagonal=True))
+    assert np.linalg.norm(x - xr) <= 1e-12 * np.linalg.norm(xr)
+
+
 def _factor(L, A, precision, pivot, nb=128, slices=6, adaptive=False):
     from mpdok_amd.mpdok_ops import _LUHandle
     h = _LUHandle(L, A.shape[0], precision, pivot, nb, slices, adaptive)
