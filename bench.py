@@ -91,8 +91,8 @@ def main():
                          "out-of-core stream, batched systems, adjoint) into 'configs'")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU layout is "
-              "described in DESIGN.md and not launched.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): this build runs the "
+              "solver on a single GPU and launches no multi-GPU layout.", file=sys.stderr)
         return 2
 
     import torch
