@@ -21,6 +21,7 @@
 // Reference: the sgetrf/sgetrs of lu_ir.so (mpdok_ops.py:69-320,
 // benchmark_mpdok.ipynb:983-996); pivoting semantics follow LAPACK getrf
 // (ipiv = sequential interchanges).
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -63,7 +64,8 @@ struct LUHandle {
   // lookahead (no pivoting): panel k+1 is factored on `side` while the main
   // stream finishes the trailing update of step k; Wp is the side's scratch
   hipStream_t side = nullptr;
-  hipEvent_t ev_next_cols = nullptr, ev_panel = nullptr;
+  // ev_first: the first panel of the group being factored is done
+  hipEvent_t ev_next_cols = nullptr, ev_panel = nullptr, ev_first = nullptr;
   void* Wp = nullptr;
   bool lookahead = true;
   // int8-sliced variant: per-update slice counts picked on the device from the
@@ -116,6 +118,7 @@ void lu_destroy(LUHandle* h) {
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   if (h->ev_next_cols) (void)hipEventDestroy(h->ev_next_cols);
   if (h->ev_panel) (void)hipEventDestroy(h->ev_panel);
+  if (h->ev_first) (void)hipEventDestroy(h->ev_first);
   if (h->side) (void)hipStreamDestroy(h->side);
   if (h->Wp) (void)hipFree(h->Wp);
   void* ptrs[] = {h->LU, h->Linv, h->Uinv, h->W, h->ipiv, h->perm, h->d_info,
@@ -220,7 +223,8 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
     // default-priority stream (no priority is ever set on any stream)
     if (st == OK && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
                      hipEventCreateWithFlags(&h->ev_next_cols, hipEventDisableTiming) != hipSuccess ||
-                     hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming) != hipSuccess)) {
+                     hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming) != hipSuccess ||
+                     hipEventCreateWithFlags(&h->ev_first, hipEventDisableTiming) != hipSuccess)) {
       set_error("lu_create: stream/event creation failed");
       st = EHIP;
     }
@@ -503,8 +507,10 @@ static int trailing_cols(LUHandle* h, hipStream_t s, int k, int kb, int c_off, i
 // for the group's trailing update. Of the panels' TRSV_BLOCK-wide inverses
 // only the L ones are built here (the block-row solves need them); the U ones
 // serve only the solves after the factorisation (group_u_inverses).
+// ev_first (if not null) is recorded once the first panel is factored.
 template <typename T>
-static int panel_group_factor(LUHandle* h, hipStream_t s, int k, int kbs, T* W) {
+static int panel_group_factor(LUHandle* h, hipStream_t s, int k, int kbs, T* W,
+                              hipEvent_t ev_first = nullptr) {
   const int N = h->N, nb = h->nb;
   T* A = static_cast<T*>(h->LU);
   const int64_t ld = N;
@@ -512,6 +518,7 @@ static int panel_group_factor(LUHandle* h, hipStream_t s, int k, int kbs, T* W) 
   for (int j = k; j < k + kbs; j += nb) {
     const int jb = nb < k + kbs - j ? nb : k + kbs - j;
     MPDOK_TRY(panel_factor<T>(h, s, j, jb, W, false, INV_L));
+    if (ev_first && j == k) MPDOK_HIP(hipEventRecord(ev_first, s));
     const int rem = k + kbs - j - jb;
     if (rem > 0) {
       MPDOK_TRY(block_row_solve<T>(h, s, j, jb, j + jb, W, rem));
@@ -540,13 +547,17 @@ static int group_u_inverses(LUHandle* h, hipStream_t s, int k, int kbs, T* W) {
 
 // U12 of the block rows [k, k + kbs) over columns [c0, N), panel by panel:
 // each block row's solve, then the update of the group's block rows below it.
+// Only the group's panels p0 .. p1 - 1 (p1 < 0: to the end); the first
+// panel's part needs only that panel to be factored.
 template <typename T>
-static int block_rows_solve(LUHandle* h, hipStream_t s, int k, int kbs, int c0, T* W) {
+static int block_rows_solve(LUHandle* h, hipStream_t s, int k, int kbs, int c0, T* W,
+                            int p0 = 0, int p1 = -1) {
   const int N = h->N, nb = h->nb;
   T* A = static_cast<T*>(h->LU);
   const int64_t ld = N;
   auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
-  for (int j = k; j < k + kbs; j += nb) {
+  const int jend = p1 < 0 ? k + kbs : std::min(k + kbs, k + p1 * nb);
+  for (int j = k + p0 * nb; j < jend; j += nb) {
     const int jb = nb < k + kbs - j ? nb : k + kbs - j;
     MPDOK_TRY(block_row_solve<T>(h, s, j, jb, c0, W));
     const int rem = k + kbs - j - jb;
@@ -614,7 +625,7 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
     // stream while the main stream copies the rest of the input
     MPDOK_HIP(hipEventRecord(h->ev_next_cols, s));
     MPDOK_HIP(hipStreamWaitEvent(h->side, h->ev_next_cols, 0));
-    MPDOK_TRY(panel_group_factor<T>(h, h->side, 0, gs[1], Wp));
+    MPDOK_TRY(panel_group_factor<T>(h, h->side, 0, gs[1], Wp, h->ev_first));
     MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
     // row lookahead for group g (see LUHandle::rows_side_min); the first
     // group's rows stay on the main stream (its copy runs there)
@@ -632,13 +643,19 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
     for (size_t g = 0; g + 1 < gs.size(); ++g) {
       const int k = gs[g], kb = gs[g + 1] - k;
       const int n3 = N - k - kb;
+      if (n3 > 0 && !rows_on_side(g)) {
+        // the first panel's block row (and its update of the group's other
+        // rows) while the lookahead stream factors the group's other panels
+        MPDOK_HIP(hipStreamWaitEvent(s, h->ev_first, 0));
+        MPDOK_TRY(block_rows_solve<T>(h, s, k, kb, k + kb, W, 0, 1));
+      }
       MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
       if (n3 <= 0) {
         if (u_on_main(g)) MPDOK_TRY(group_u_inverses<T>(h, s, k, kb, W));
         break;
       }
       if (!rows_on_side(g)) {
-        MPDOK_TRY(block_rows_solve<T>(h, s, k, kb, k + kb, W));
+        MPDOK_TRY(block_rows_solve<T>(h, s, k, kb, k + kb, W, 1));
         if (h->precision == LU_OZAKI)
           MPDOK_TRY(slice_step(h, s, k, kb, n3, reinterpret_cast<const double*>(at(k, k + kb)), ld));
       }
@@ -652,7 +669,7 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
       // that it never waits for the host to issue them
       MPDOK_TRY(trailing_block<T>(h, s, k, kb, nxt ? nk : 0, nxt ? n3 - nk : n3, nk, n3 - nk));
       MPDOK_HIP(hipStreamWaitEvent(h->side, h->ev_next_cols, 0));
-      MPDOK_TRY(panel_group_factor<T>(h, h->side, k + kb, nk, Wp));
+      MPDOK_TRY(panel_group_factor<T>(h, h->side, k + kb, nk, Wp, nxt ? nullptr : h->ev_first));
       if (nxt) {
         const int k2 = k + kb, n32 = N - k2 - nk;
         MPDOK_TRY(block_rows_solve<T>(h, h->side, k2, nk, k2 + nk, Wp));
