@@ -417,8 +417,9 @@ class MPDOKSolver:
         slices whose error bound stays within the first update's times
         2^(-slice_margin): -4 lets each later update be up to 16 times less
         accurate than the first, for fewer digit products; GMRES-IR absorbs it
-        with about one more inner iteration (measured at N = 32768: 6 inner
-        iterations instead of 5, 209 ms instead of 216)."""
+        with about one more inner iteration (measured on the N = 32768 Gaussian
+        RBF system: 6 inner iterations instead of 5 at -2, 194.5 ms instead of
+        196.3)."""
         try:
             if precond is None:
                 return self._solve(A, b, tol, maxiter_outer, restart)
