@@ -165,13 +165,15 @@ def test_diag_block_kernels_factor_and_inverses(L, variant, N):
 
 
 @pytest.mark.parametrize("precision", ["ozaki", "fp64"])
-@pytest.mark.parametrize("N,nb", [(2048, 128), (5000, 512)])
+@pytest.mark.parametrize("N,nb", [(2048, 128), (5000, 512), (4096, 512)])
 def test_row_lookahead_gives_the_same_factors(L, precision, N, nb):
     """Moving block-row solves, U12 slicing and slice-count decisions to the
     lookahead stream (and updating the next step's rows first) changes where
     and in which launch each product is computed, not its operands or
     summation order: the factors, slice counts and solutions match the plain
-    schedule bit for bit (also with a ragged last panel group)."""
+    schedule bit for bit (also with a ragged last panel group, and at N = 4096
+    with 512-column panel pairs, where the main stream starts a pair's first
+    block row after its first panel and the solves use 1024-wide blocks)."""
     X = np.random.default_rng(5).uniform(0, 1, (N, 2))
     A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
     from mpdok_amd.mpdok_ops import _LUHandle
