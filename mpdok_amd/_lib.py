@@ -141,10 +141,12 @@ def lib():
     return _lib if _lib is not None else load()
 
 
-def check(status, what=""):
+def check(status, what="", from_lib=None):
+    """Raise for a nonzero status, with the message of the library instance
+    that returned it (from_lib; default: the one load() returned first)."""
     if status == 0:
         return
-    msg = lib().mpdok_last_error().decode(errors="replace")
+    msg = (from_lib or lib()).mpdok_last_error().decode(errors="replace")
     if status == -2:
         raise MPDOKMemoryError(f"{what}: {msg}")
     raise MPDOKError(f"{what} failed (status {status}): {msg}")

@@ -78,8 +78,9 @@ class RBFValidationLoss(torch.autograd.Function):
         alpha = solver.solve(K, y, **kw)
         _cross.gamma = gamma
         f = _cross(Xv, X, alpha)
-        r = f - yv
-        loss = torch.dot(r, r) / M
+        r = _ops.axpby(1.0, f, -1.0, yv)                        # f_v - yv
+        loss = torch.tensor(float(_ops.dot(r, r).item()) / M, dtype=torch.float64,
+                            device=alpha.device)
         ctx.state = (solver, K, kw, alpha, r, e, sigma, gamma)
         ctx.save_for_backward(X, Xv)
         ctx.forward_info = solver.last_info
@@ -90,14 +91,14 @@ class RBFValidationLoss(torch.autograd.Function):
         X, Xv = ctx.saved_tensors
         solver, K, kw, alpha, r, e, sigma, gamma = ctx.state
         M = Xv.shape[0]
-        g = r * (2.0 / M) * float(gL)
+        g = _ops.axpby(2.0 / M * float(gL), r)
         _cross.gamma = gamma
         q = _cross(X, Xv, g)                                     # K_v^T g
         lam = solver.solve(K, q, reuse_factor=True, **kw)         # adjoint solve, same factor
         t_v = _dbilinear(Xv, X, g, alpha, gamma)                  # g^T dK_v/dgamma alpha
         t_k = _dbilinear(X, X, lam, alpha, gamma)                 # lambda^T dK/dgamma alpha
         d_eps = 2.0 * e * (t_v - t_k)
-        d_logsig = -sigma * float(torch.dot(lam, alpha))
+        d_logsig = -sigma * float(_ops.dot(lam, alpha).item())
         solver.free_fp32_buf()
         ctx.state = None
         dev = alpha.device

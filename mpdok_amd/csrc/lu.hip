@@ -629,9 +629,13 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
     MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
     // row lookahead for group g (see LUHandle::rows_side_min); the first
     // group's rows stay on the main stream (its copy runs there)
+    // The int8-sliced update takes row ranges that start on a 128-row tile,
+    // so a group whose width is not a multiple of 128 keeps its rows on the
+    // main stream.
     auto rows_on_side = [&](size_t g) {
       return g >= 1 && g + 1 < gs.size() && gs[g + 1] < N && h->rows_side_min > 0 &&
-             N - gs[g] >= h->rows_side_min && (adaptive_on(h) || h->precision != LU_OZAKI);
+             N - gs[g] >= h->rows_side_min && (adaptive_on(h) || h->precision != LU_OZAKI) &&
+             (h->precision != LU_OZAKI || (gs[g + 1] - gs[g]) % 128 == 0);
     };
     // The U inverses of group g go to the stream that waits for the other:
     // the lookahead stream while the trailing updates are long (the first
@@ -722,10 +726,16 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
   build_schedule(h);
   const bool split = lookahead_on(h);
   MPDOK_TRY(copy_columns(h, s, A, lda, order, 0, split ? h->group_start[1] : N));
-  if (fp64_storage(h->precision))
-    MPDOK_TRY(factor_impl<double>(h, s, split ? A : nullptr, lda, order));
-  else
-    MPDOK_TRY(factor_impl<float>(h, s, split ? A : nullptr, lda, order));
+  const int st = fp64_storage(h->precision)
+                     ? factor_impl<double>(h, s, split ? A : nullptr, lda, order)
+                     : factor_impl<float>(h, s, split ? A : nullptr, lda, order);
+  if (st != OK) {
+    // work already enqueued on either stream may still use the handle's
+    // buffers: drain both before the caller can reuse or destroy them
+    if (h->side) (void)hipStreamSynchronize(h->side);
+    (void)hipStreamSynchronize(s);
+    return st;
+  }
   int hinfo = 0;
   MPDOK_HIP(hipMemcpyAsync(&hinfo, h->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
   const int nsteps = (int)h->group_start.size() - 1;

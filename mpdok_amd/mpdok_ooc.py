@@ -61,7 +61,10 @@ class MPDOKOOCSolver:
         self.gamma = None
         self.reg = None
         self._coords = None      # device (N, D) FP64
-        self._sq = None          # device (N,) FP64 squared norms
+        # The reference keeps the squared norms of the coordinates here (its
+        # "bring your own matrix" recipe sets them). The fused kernels here
+        # recompute them per tile, so the attribute is accepted and unused.
+        self._sq = None
         self._kind = GAUSSIAN
         self._pinned = None      # torch tensor owning the page-locked _ram_buf
         self._dev_tiles = None   # device workspace: two tiles
@@ -95,7 +98,6 @@ class MPDOKOOCSolver:
         self.N, self.gamma, self.reg = N, gamma, reg
         self._store = store
         self._coords = coords
-        self._sq = (coords * coords).sum(dim=1)
         tr = self.tile_rows
         n_tiles = (N + tr - 1) // tr
         if verbose:

@@ -20,8 +20,10 @@ Differences a caller can see:
     the refinement history of the last solve.
 """
 import ctypes
+import itertools
 import os
 import warnings
+import weakref
 
 import numpy as np
 import torch
@@ -85,9 +87,64 @@ def _wrap_fortran_f64(ptr, N):
     return torch.as_tensor(_CAI(ptr, (N, N), (8, 8 * N)), device=_dev.device())
 
 
+# The native library keeps one managed buffer per kind ("gmres" for
+# py_alloc_managed, "lu" for py_lu_alloc_managed) and frees the previous one
+# when it allocates the next, whichever solver asked for it. Each slot here
+# records which wrapper holds the live buffer (a token per allocation), so a
+# wrapper whose buffer the library already freed never frees the one that
+# replaced it.
+_MANAGED_SLOTS = {}
+_MANAGED_TOKENS = itertools.count(1)
+
+
+def _managed_fns(lib, kind):
+    if kind == "lu":
+        return lib.py_lu_alloc_managed, lib.py_lu_free_managed, "py_lu_alloc_managed"
+    return lib.py_alloc_managed, lib.py_free_managed, "py_alloc_managed"
+
+
+def _mark_slot_freed(lib, kind):
+    """The library has freed (or is about to free) the slot's buffer: its
+    wrapper, if alive, must not free it again."""
+    key = (lib._handle, kind)
+    prev = _MANAGED_SLOTS.pop(key, None)
+    if prev is not None:
+        w = prev[1]()
+        if w is not None:
+            w._freed = True
+
+
+def _alloc_managed(lib, kind, N):
+    alloc, free, name = _managed_fns(lib, kind)
+    key = (lib._handle, kind)
+    _mark_slot_freed(lib, kind)       # the native frees it before allocating
+    ptr = ctypes.c_void_p()
+    alloc(ctypes.c_int(N), ctypes.byref(ptr))
+    if not ptr.value:
+        raise MemoryError(f"{name} failed for N={N}")
+    token = next(_MANAGED_TOKENS)
+
+    def free_fn():
+        cur = _MANAGED_SLOTS.get(key)
+        if cur is not None and cur[0] == token:
+            del _MANAGED_SLOTS[key]
+            free()
+
+    managed = _ManagedArray(_wrap_fortran_f64(ptr.value, N), free_fn)
+    _MANAGED_SLOTS[key] = (token, weakref.ref(managed))
+    return managed
+
+
+def _free_managed_slot(lib, kind):
+    """free_managed() without a wrapper: free whatever the slot holds (the
+    reference's behaviour) and disarm the wrapper that held it."""
+    _mark_slot_freed(lib, kind)
+    _managed_fns(lib, kind)[1]()
+
+
 def _resolve_lib(lib_path, legacy_name):
     if lib_path is None:
-        lib_path = _lib.DEFAULT_LIB
+        lib_path = os.environ.get("MPDOK_LIB", _lib.DEFAULT_LIB)
     if not os.path.exists(lib_path):
         raise FileNotFoundError(
             f"{os.path.basename(lib_path) or legacy_name} not found at {lib_path}. "
@@ -96,7 +153,7 @@ def _resolve_lib(lib_path, legacy_name):
 
 
 def _set_stream(lib):
-    check(lib.mpdok_set_stream(_dev.stream()), "set_stream")
+    check(lib.mpdok_set_stream(_dev.stream()), "set_stream", from_lib=lib)
 
 
 def _is_oom(exc):
@@ -110,7 +167,7 @@ class _LUHandle:
         self._lib = lib
         h = ctypes.c_void_p()
         check(self._lib.mpdok_lu_create(N, PRECISIONS[precision], PIVOTS[pivot], nb, slices,
-                                        ctypes.byref(h)), "lu_create")
+                                        ctypes.byref(h)), "lu_create", from_lib=self._lib)
         self.ptr = h.value
         self.N = N
         if precision == "ozaki" and slices <= 5:
@@ -121,7 +178,7 @@ class _LUHandle:
         mpdok_lu_set_adaptive): each trailing update multiplies only as many
         digits as keep its error bound within the first update's."""
         check(self._lib.mpdok_lu_set_adaptive(self.ptr, 1 if on else 0, int(smin), int(delta)),
-              "lu_set_adaptive")
+              "lu_set_adaptive", from_lib=self._lib)
 
     def set_update_panels(self, panels, panels_early=None, early_rows=20480):
         """Panels per trailing update of the int8-sliced variant (see
@@ -131,13 +188,14 @@ class _LUHandle:
         panels times nb."""
         pe = panels if panels_early is None else panels_early
         check(self._lib.mpdok_lu_set_update_panels(self.ptr, int(panels), int(pe), int(early_rows)),
-              "lu_set_update_panels")
+              "lu_set_update_panels", from_lib=self._lib)
 
     def set_row_lookahead(self, min_rows):
         """Row lookahead (see mpdok_lu_set_row_lookahead): while the trailing
         matrix has at least min_rows rows, block-row solves, U12 slicing and
         slice-count decisions run on the lookahead stream; 0 turns it off."""
-        check(self._lib.mpdok_lu_set_row_lookahead(self.ptr, int(min_rows)), "lu_set_row_lookahead")
+        check(self._lib.mpdok_lu_set_row_lookahead(self.ptr, int(min_rows)), "lu_set_row_lookahead",
+              from_lib=self._lib)
 
     def step_starts(self):
         """First columns of the last factorisation's steps (one trailing
@@ -145,7 +203,7 @@ class _LUHandle:
         out = (ctypes.c_int * 4096)()
         n = ctypes.c_int(0)
         check(self._lib.mpdok_lu_step_starts(self.ptr, out, 4096, ctypes.byref(n)),
-              "lu_step_starts")
+              "lu_step_starts", from_lib=self._lib)
         return list(out[:min(n.value, 4096)])
 
     def slice_counts(self):
@@ -153,30 +211,32 @@ class _LUHandle:
         out = (ctypes.c_int * 4096)()
         n = ctypes.c_int(0)
         check(self._lib.mpdok_lu_slice_counts(self.ptr, out, 4096, ctypes.byref(n)),
-              "lu_slice_counts")
+              "lu_slice_counts", from_lib=self._lib)
         return list(out[:min(n.value, 4096)])
 
     def factor(self, A):
         order, ld = _dev.layout(A)
         info = ctypes.c_int(0)
         f = self._lib.mpdok_lu_factor
-        check(f(self.ptr, _dev.stream(), A.data_ptr(), ld, order, ctypes.byref(info)), "lu_factor")
+        check(f(self.ptr, _dev.stream(), A.data_ptr(), ld, order, ctypes.byref(info)), "lu_factor",
+              from_lib=self._lib)
         return info.value
 
     def set_profiling(self, on):
-        check(self._lib.mpdok_lu_set_profiling(self.ptr, 1 if on else 0), "lu_set_profiling")
+        check(self._lib.mpdok_lu_set_profiling(self.ptr, 1 if on else 0), "lu_set_profiling",
+              from_lib=self._lib)
 
     def stats(self):
         """HIP-event totals of the trailing updates of the last factorisation."""
         ms, ops, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         check(self._lib.mpdok_lu_stats(self.ptr, ctypes.byref(ms), ctypes.byref(ops),
-                                       ctypes.byref(n)), "lu_stats")
+                                       ctypes.byref(n)), "lu_stats", from_lib=self._lib)
         return {"trailing_ms": ms.value, "trailing_ops": ops.value, "launches": n.value}
 
     def solve(self, b, out=None):
         x = out if out is not None else torch.empty_like(b)
         f = self._lib.mpdok_lu_solve
-        check(f(self.ptr, _dev.stream(), b.data_ptr(), x.data_ptr()), "lu_solve")
+        check(f(self.ptr, _dev.stream(), b.data_ptr(), x.data_ptr()), "lu_solve", from_lib=self._lib)
         return x
 
     def close(self):
@@ -226,12 +286,9 @@ class LUIRSolver:
 
     # -- managed memory ---------------------------------------------------
     def alloc_managed(self, N):
-        """(N, N) FP64 Fortran-order array in managed memory (one per library)."""
-        ptr = ctypes.c_void_p()
-        self._lib.py_lu_alloc_managed(ctypes.c_int(N), ctypes.byref(ptr))
-        if not ptr.value:
-            raise MemoryError(f"py_lu_alloc_managed failed for N={N}")
-        managed = _ManagedArray(_wrap_fortran_f64(ptr.value, N), self._lib.py_lu_free_managed)
+        """(N, N) FP64 Fortran-order array in managed memory (one per library:
+        a second allocation frees the first, and its wrapper is marked freed)."""
+        managed = _alloc_managed(self._lib, "lu", N)
         self._managed = managed
         return managed
 
@@ -240,7 +297,7 @@ class LUIRSolver:
             self._managed.free()
             self._managed = None
         else:
-            self._lib.py_lu_free_managed()
+            _free_managed_slot(self._lib, "lu")
 
     # -- factor once, solve many -------------------------------------------
     def factor(self, A):
@@ -274,7 +331,7 @@ class LUIRSolver:
                                     N, b_gpu.data_ptr(), x_gpu.data_ptr(), float(tol),
                                     int(maxiter_outer), RESIDUALS[self.residual],
                                     ctypes.byref(conv), hist, maxiter_outer + 2,
-                                    ctypes.byref(n_hist)), "lu_ir")
+                                    ctypes.byref(n_hist)), "lu_ir", from_lib=self._lib)
         torch.cuda.current_stream().synchronize()
         self.last_info = {"history": list(hist[:n_hist.value]), "converged": bool(conv.value)}
         return x_gpu, conv.value
@@ -333,7 +390,18 @@ class LUIRSolver:
             raise ValueError(f"A must be square (N×N), got {tuple(A_gpu.shape)}")
         if tuple(b_gpu.shape) != (N,):
             raise ValueError(f"b must be (N,), got {tuple(b_gpu.shape)}")
-        self.factor(A_gpu)
+        try:
+            self.factor(A_gpu)
+        except RuntimeError as exc:
+            if "py_lu_factor failed" not in str(exc):
+                raise
+            # the reference's one-call py_lu_ir reports a failed factorisation
+            # only through converged = 0 (mpdok_ops.py:304-318): warn, return x
+            warnings.warn(
+                f"LU-IR did not converge to tol={tol:.1e} within "
+                f"{maxiter_outer} outer iterations.",
+                RuntimeWarning, stacklevel=2)
+            return _dev.zeros(N, torch.float64)
         x_gpu, conv = self._refine(A_gpu, b_gpu, tol, maxiter_outer)
         if not conv:
             warnings.warn(
@@ -382,11 +450,9 @@ class MPDOKSolver:
 
     # -- managed memory ---------------------------------------------------
     def alloc_managed(self, N):
-        ptr = ctypes.c_void_p()
-        self._lib.py_alloc_managed(ctypes.c_int(N), ctypes.byref(ptr))
-        if not ptr.value:
-            raise MemoryError(f"py_alloc_managed failed for N={N}")
-        managed = _ManagedArray(_wrap_fortran_f64(ptr.value, N), self._lib.py_free_managed)
+        """(N, N) FP64 Fortran-order array in managed memory (one per library:
+        a second allocation frees the first, and its wrapper is marked freed)."""
+        managed = _alloc_managed(self._lib, "gmres", N)
         self._managed = managed
         return managed
 
@@ -395,7 +461,7 @@ class MPDOKSolver:
             self._managed.free()
             self._managed = None
         else:
-            self._lib.py_free_managed()
+            _free_managed_slot(self._lib, "gmres")
 
     def free_fp32_buf(self):
         """Drop the FP32 scratch copy of A (and any cached preconditioner)."""
@@ -483,7 +549,11 @@ class MPDOKSolver:
             raise ValueError(f"precond must be None or 'lu', got {precond!r}")
         A_gpu, b_gpu, N = self._prep(A, b)
         order, ld = _dev.layout(A_gpu)
-        key = (N, factor_precision, slices, pivot, nb, bool(adaptive_slices), int(slice_margin))
+        # the factor belongs to one matrix: its storage and torch's version
+        # counter of that storage (bumped by every in-place write) are part
+        # of the key, so reuse_factor never preconditions another matrix
+        key = (N, factor_precision, slices, pivot, nb, bool(adaptive_slices), int(slice_margin),
+               A_gpu.data_ptr(), A_gpu._version)
         if not (reuse_factor and self._precond is not None and self._precond[0] == key):
             if self._precond is not None:
                 self._precond[1].close()
@@ -514,7 +584,7 @@ class MPDOKSolver:
             b_gpu.data_ptr(), x_gpu.data_ptr(), float(tol), int(maxiter_outer), int(restart),
             float(inner_tol), 2, RESIDUALS[residual], 1 if body == "fp64" else 0,
             ctypes.byref(conv), hist, maxiter_outer + 2, ctypes.byref(n_hist),
-            ctypes.byref(inner)), "gmres_ir")
+            ctypes.byref(inner)), "gmres_ir", from_lib=self._lib)
         torch.cuda.current_stream().synchronize()
         self.last_info = {"history": list(hist[:n_hist.value]), "converged": bool(conv.value),
                           "inner_iterations": inner.value,

@@ -27,6 +27,11 @@ from . import _dev, _ops
 
 GAUSSIAN, MATERN32 = 0, 1
 
+# A managed `out` whose FP64 size exceeds this fraction of device memory is
+# filled through its host pages instead of on the device (rbf_kernel.py:190-205
+# uses 90%). Module-level so that a test can force that branch at small N.
+_HOST_BUILD_FRACTION = 0.90
+
 
 def gamma_from_eps(eps):
     """Shape parameter eps of exp(-(eps r)^2) -> gamma of exp(-gamma r^2)."""
@@ -102,7 +107,7 @@ def build_rbf_kernel(coords, gamma=None, reg=1e-6, out=None, chunk=1024, verbose
     # size exceeds 90% of device memory -> host build into its pages.
     fp64_bytes = N * N * 8
     vram_total = torch.cuda.get_device_properties(_dev.device()).total_memory
-    if is_managed and fp64_bytes > vram_total * 0.90 and kind == GAUSSIAN:
+    if is_managed and fp64_bytes > vram_total * _HOST_BUILD_FRACTION and kind == GAUSSIAN:
         coords_np = coords.cpu().numpy()
         buf = (ctypes.c_uint8 * fp64_bytes).from_address(int(A_raw.data_ptr()))
         A_np = np.frombuffer(buf, dtype=np.float64).reshape((N, N), order='F')

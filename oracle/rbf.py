@@ -1,7 +1,8 @@
 """Oracle: kernel-matrix assembly (test infrastructure only).
 
-Restates rbf_kernel.py:57-68 (_estimate_gamma), rbf_kernel.py:158-227
-(build_rbf_kernel, GPU path), mpdok_ooc.py:39-46,118-123 (_build_fp32_tile and
+Restates rbf_kernel.py:41-54 (synthetic_coords, NumPy PCG64 in place of
+CuPy's generator), rbf_kernel.py:57-68 (_estimate_gamma), rbf_kernel.py:158-227
+(build_rbf_kernel, GPU path), rbf_kernel.py:230-252 (weather_front), mpdok_ooc.py:39-46,118-123 (_build_fp32_tile and
 the FP32-after-cast nugget) and mpdok_ooc.py:205-227 (FP64 tile recompute).
 """
 import numpy as np
@@ -76,6 +77,13 @@ def matvec_fp64(X, v, kind="gaussian", param=1.0, reg=0.0, tile_rows=4096):
         tile[k, i + k] += reg
         y[i:i + rows] = tile @ v
     return y
+
+
+def synthetic_coords(N, D=2, seed=42, scale=100.0):
+    """N points uniform in [0, scale]^D (rbf_kernel.py:41-54). The reference
+    draws them from CuPy's default_rng, whose stream NumPy cannot reproduce;
+    this restatement (and the product) use NumPy's PCG64 default_rng(seed)."""
+    return np.random.default_rng(seed).uniform(0.0, scale, size=(N, D)).astype(np.float64)
 
 
 def weather_front(X, t, noise_std=0.05, seed=None):

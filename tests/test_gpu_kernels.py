@@ -177,3 +177,31 @@ def test_cast_matrix_orders(ops):
         for oo in "CF":
             y = ops.cast_matrix(dev(X, order=oi), torch.float32, oo).cpu().numpy()
             assert np.array_equal(y, X.astype(np.float32))
+
+
+def test_synthetic_coords_matches_oracle():
+    """rbf_kernel.py:41-54: (N, D) FP64 points uniform in [0, scale]^D on the
+    device, drawn from NumPy PCG64 default_rng(seed) (the reference's CuPy
+    stream cannot be reproduced; the oracle states the same generator)."""
+    from mpdok_amd.rbf_kernel import synthetic_coords
+    for N, D, seed, scale in [(1, 2, 42, 100.0), (1000, 2, 0, 100.0), (777, 3, 9, 1.0)]:
+        c = synthetic_coords(N, D=D, seed=seed, scale=scale)
+        assert c.is_cuda and c.dtype == torch.float64 and tuple(c.shape) == (N, D)
+        assert np.array_equal(c.cpu().numpy(), orb.synthetic_coords(N, D, seed, scale))
+    assert tuple(synthetic_coords(5).shape) == (5, 2)
+
+
+def test_weather_front_matches_oracle():
+    """rbf_kernel.py:230-252 on device coordinates, with and without noise,
+    the noise seed defaulting to t."""
+    from mpdok_amd.rbf_kernel import synthetic_coords, weather_front
+    c = synthetic_coords(2048, seed=0)
+    X = c.cpu().numpy()
+    for t, noise, seed in [(0, 0.05, None), (3, 0.05, None), (3, 0.2, 11), (7, 0.0, None)]:
+        b = weather_front(c, t, noise_std=noise, seed=seed)
+        assert b.is_cuda and b.dtype == torch.float64 and tuple(b.shape) == (2048,)
+        ref = orb.weather_front(X, t, noise_std=noise, seed=seed)
+        assert np.allclose(b.cpu().numpy(), ref, rtol=0, atol=1e-13)
+    # host coordinates are accepted too
+    assert np.allclose(weather_front(X, 1).cpu().numpy(), orb.weather_front(X, 1), rtol=0,
+                       atol=1e-13)

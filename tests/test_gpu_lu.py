@@ -522,4 +522,78 @@ def test_reference_symbols_through_raw_ctypes(L):
     lib.py_gmres_ir(Ad.data_ptr(), bd.data_ptr(), x.data_ptr(), A32.data_ptr(), N, 1e-11, 5, 50,
                     ctypes.byref(conv))
     assert conv.value == 1
+    xg = x.cpu().numpy()
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(b - A @ xg) / np.linalg.norm(b) < 1e-11
+    assert np.linalg.norm(xg - xr) / np.linalg.norm(xr) < 1e-12
+    # the FP32 scratch holds A rounded to FP32, column-major like A
+    assert np.array_equal(A32.cpu().numpy().reshape(N, N).T, A.astype(np.float32))
+    lib.py_lu_factor(Ad.data_ptr(), N, ctypes.byref(conv))
+    assert conv.value == 0
+    x.zero_()
+    lib.py_lu_solve_factored(Ad.data_ptr(), bd.data_ptr(), x.data_ptr(), N, 1e-11, 3,
+                             ctypes.byref(conv))
+    assert conv.value == 1
+    assert np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr) < 1e-12
     lib.py_lu_free_factored()
+
+
+def _refined_reference(A, b):
+    """LAPACK FP64 solve plus one step of long-double residual refinement,
+    and the forward error of the unrefined solve against it."""
+    x64 = sla.solve(A, b)
+    xl = x64 + sla.solve(A, olu.residual_longdouble(A, x64, b))
+    return xl, np.linalg.norm(x64 - xl) / np.linalg.norm(xl)
+
+
+def test_headline_solver_settings_on_the_3d_generator(L):
+    """MPDOKSolver.solve with the exact keywords of the N = 32768 benchmark
+    (at most 5 slices, adaptive counts with margin -4, nb = 512 panel pairs,
+    the default row lookahead) on the same generator at N = 8192: long-double
+    relres <= 1e-12 and forward error within 10x of LAPACK's."""
+    from mpdok_amd.mpdok_ops import MPDOKSolver
+    N = 8192
+    X = np.random.default_rng(42).uniform(0, 1, (N, 3))
+    A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
+    b = (np.sin(2 * np.pi * X[:, 0]) * np.cos(2 * np.pi * X[:, 1]) * np.cos(2 * np.pi * X[:, 2])
+         + 1e-4 * np.random.default_rng(43).standard_normal(N))
+    solver = MPDOKSolver()
+    x = solver.solve(A, b, tol=1e-12, maxiter_outer=20, restart=50, precond="lu",
+                     factor_precision="ozaki", slices=5, pivot="none", body="fp64",
+                     residual="dot2", nb=512).cpu().numpy()
+    info = solver.last_info
+    assert info["converged"], info
+    counts = info["slice_counts"]
+    assert counts[0] == 5 and min(counts[:-1]) >= 2 and max(counts) <= 5
+    starts = info["step_starts"]
+    assert all(b_ - a == 1024 for a, b_ in zip(starts[:-1], starts[1:]))   # panel pairs
+    r = olu.residual_longdouble(A, x, b)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
+    xl, err64 = _refined_reference(A, b)
+    assert np.linalg.norm(x - xl) / np.linalg.norm(xl) <= 10 * max(err64, 1e-15)
+
+
+def test_cpu_sized_2d_case_int8_lu_gmres_ir_and_reference_lu_ir(L):
+    """The CPU-runnable configuration: N = 4096 points in [0,1]^2 (seed 42),
+    eps = 8 (gamma = 64), nugget 1e-8, b = sin(2 pi x) cos(2 pi y) + 1e-4
+    noise (seed 43). kappa ~ 2e10: 6 fixed slices precondition GMRES-IR,
+    which reaches the FP64 floor of this system (||x||/||b|| ~ 2e4: rounding
+    the exact solution alone leaves relres ~ 2e-11), with a forward error
+    within 10x of LAPACK's. The reference's FP32 LU-IR (py_lu_ir) cannot
+    precondition kappa >> 1/u32 and reports non-convergence."""
+    from mpdok_amd.mpdok_ops import LUIRSolver, MPDOKSolver
+    N = 4096
+    X = np.random.default_rng(42).uniform(0, 1, (N, 2))
+    A = orb.build_kernel(X, "gaussian", 64.0, 1e-8)
+    b = (np.sin(2 * np.pi * X[:, 0]) * np.cos(2 * np.pi * X[:, 1])
+         + 1e-4 * np.random.default_rng(43).standard_normal(N))
+    solver = MPDOKSolver()
+    x = solver.solve(A, b, tol=1e-11, maxiter_outer=20, precond="lu", factor_precision="ozaki",
+                     slices=6).cpu().numpy()
+    assert solver.last_info["converged"], solver.last_info
+    rel = np.linalg.norm(olu.residual_longdouble(A, x, b)) / np.linalg.norm(b)
+    assert rel <= 2e-11
+    xl, err64 = _refined_reference(A, b)
+    assert np.linalg.norm(x - xl) / np.linalg.norm(xl) <= 10 * max(err64, 1e-15)
+    with pytest.warns(RuntimeWarning, match="did not converge"):
+        LUIRSolver().solve(A, b, tol=1e-12, maxiter_outer=10)

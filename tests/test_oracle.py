@@ -54,6 +54,48 @@ def test_fp32_tile_adds_nugget_after_cast():
     assert np.array_equal(t[off], full[16:24][off].astype(np.float32))
 
 
+def test_synthetic_coords_bounds_moments_and_seeds():
+    X = orb.synthetic_coords(200000, D=3, seed=42, scale=100.0)
+    assert X.shape == (200000, 3) and X.dtype == np.float64
+    assert X.min() >= 0.0 and X.max() < 100.0
+    # uniform on [0, L]: mean L/2, variance L^2/12 (5 sigma of the sample error)
+    assert np.allclose(X.mean(axis=0), 50.0, atol=5 * 100 / np.sqrt(12 * 200000))
+    assert np.allclose(X.var(axis=0), 100.0 ** 2 / 12, rtol=0.01)
+    assert np.array_equal(orb.synthetic_coords(50, seed=7), orb.synthetic_coords(50, seed=7))
+    assert not np.array_equal(orb.synthetic_coords(50, seed=7), orb.synthetic_coords(50, seed=8))
+    assert orb.synthetic_coords(10, D=2, scale=1.0).max() < 1.0
+
+
+def test_weather_front_closed_form_points_bounds_and_noise():
+    """rbf_kernel.py:230-252: 15 + 5 sin((x + y)/20 - t/2) + noise."""
+    pi = np.pi
+    # (x + y)/20 - t/2 = pi/2 -> 20, = 3 pi/2 -> 10, = 0 -> 15
+    P = np.array([[5 * pi, 5 * pi], [15 * pi, 15 * pi], [0.0, 0.0], [7.0, -7.0],
+                  [10.0 + 5 * pi, 10.0 + 5 * pi]])
+    f0 = orb.weather_front(P, 0, noise_std=0.0)
+    assert np.allclose(f0[:4], [20.0, 10.0, 15.0, 15.0], rtol=0, atol=1e-13)
+    # t = 2 shifts the phase by -1: x + y = 20 + 10 pi peaks at t = 2
+    assert orb.weather_front(P, 2, noise_std=0.0)[4] == pytest.approx(20.0, abs=1e-13)
+    # symmetric in x and y; columns after the second are ignored
+    X = orb.synthetic_coords(1000, D=3, seed=3)
+    f = orb.weather_front(X, 4, noise_std=0.0)
+    assert np.array_equal(f, orb.weather_front(X[:, [1, 0, 2]], 4, noise_std=0.0))
+    assert np.array_equal(f, orb.weather_front(X[:, :2], 4, noise_std=0.0))
+    assert f.min() >= 10.0 and f.max() <= 20.0
+    # period 4 pi in t
+    assert np.allclose(orb.weather_front(X, 4 + 4 * pi, noise_std=0.0), f, atol=1e-12)
+    # noise: seed defaults to t, std noise_std, zero mean
+    n = orb.weather_front(X, 4) - f
+    assert np.array_equal(orb.weather_front(X, 4), orb.weather_front(X, 4, seed=4))
+    assert not np.array_equal(orb.weather_front(X, 4), orb.weather_front(X, 5) - (
+        orb.weather_front(X, 5, noise_std=0.0) - f))
+    big = orb.synthetic_coords(100000, seed=1)
+    nb = orb.weather_front(big, 1, noise_std=0.3) - orb.weather_front(big, 1, noise_std=0.0)
+    assert np.std(nb) == pytest.approx(0.3, rel=0.02)
+    assert abs(np.mean(nb)) < 5 * 0.3 / np.sqrt(100000)
+    assert np.std(n) == pytest.approx(0.05, rel=0.1)
+
+
 def test_estimate_gamma_known_answer_uniform_square():
     # E[mean d2] for uniform [0, L]^D with the diagonal zeros is D L^2 / 6 (1 - 1/n)
     vals = []
@@ -389,7 +431,26 @@ def test_residual_exact_catches_cancellation_and_longdouble_agrees():
     r = olu.residual_exact(A, x, b)
     assert r[0] == 0.0  # 1 - (1e16 + 1 - 1e16) = 0 exactly
     assert r[1] == pytest.approx(-1e-8, rel=1e-15)
-    naive = b - A @ x
-    assert naive[0] != 0.0 or True  # FP64 may lose it; the exact one may not
+    # FP64 summed left to right loses it: (1e16 + 1) rounds to 1e16
+    naive0 = b[0] - ((A[0, 0] * x[0] + A[0, 1] * x[1]) + A[0, 2] * x[2])
+    assert naive0 == 1.0
     rl = olu.residual_longdouble(A, x, b)
+    assert rl[0] == 0.0  # a 64-bit significand holds 1e16 + 1 exactly
     assert abs(rl[1] - r[1]) < 1e-18
+
+
+def test_forward_and_backward_error_closed_forms():
+    x_ref = np.array([3.0, -4.0, 12.0])            # ||x_ref|| = 13
+    assert olu.forward_error(x_ref, x_ref) == 0.0
+    assert olu.forward_error(2.0 * x_ref, x_ref) == pytest.approx(1.0, rel=1e-15)
+    assert olu.forward_error(x_ref + np.array([0.0, 0.0, 1.3]), x_ref) == pytest.approx(
+        0.1, rel=1e-14)
+    # backward error: exact solution -> 0; x = 0 -> ||b|| / ||b|| = 1
+    A = np.array([[2.0, 0.0], [0.0, 3.0]])
+    b = np.array([2.0, 3.0])
+    assert olu.backward_error(A, np.array([1.0, 1.0]), b) == 0.0
+    assert olu.backward_error(A, np.zeros(2), b) == pytest.approx(1.0, rel=1e-15)
+    # a perturbation r = b - A x of norm 1 with ||A||_F = sqrt(13), ||x|| = sqrt(2)
+    x = np.array([1.0, 1.0 + 1.0 / 3.0])
+    assert olu.backward_error(A, x, b) == pytest.approx(
+        1.0 / (np.sqrt(13.0) * np.linalg.norm(x) + np.sqrt(13.0)), rel=1e-14)
