@@ -112,6 +112,17 @@ int mpdok_residual_f64(hipStream_t s, const double* A, int64_t lda, int order, i
                        const double* x, const double* b, double* r, int mode, void* work,
                        size_t work_bytes);
 
+/* r = b - A x exactly as mpdok_residual_f64 (same kernels, same bits), with
+ * no workspace argument and no allocation: the column-major partial sums go
+ * to a 16 MiB array the library holds on each GPU, so a column-major N needs
+ * mpdok_residual_workspace_bytes(N, 1) <= 16 MiB (N <= 1048576; any N
+ * row-major). Larger N is rejected with an error before any GPU work. Calls
+ * that overlap on one GPU share that array and race. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc). */
+int mpdok_residual_f64_noalloc(hipStream_t s, const double* A, int64_t lda, int order, int N,
+                               const double* x, const double* b, double* r, int mode);
+
 /* Kernel-matrix tile: rows [row0,row0+rows) x cols [col0,col0+cols) of
  * k(c_i, c_j) + reg*delta_ij from coords (N x D, row-major FP64). kind 0 =
  * Gaussian exp(-p d2), 1 = Matern-3/2 with length scale p. reg_mode 0 = nugget in

@@ -51,6 +51,7 @@ SIGNATURES = {
     "mpdok_gemv_f32": (I, [P, P, I64, I, I, I, P, P, P, SZ, P]),
     "mpdok_gemv_f64": (I, [P, P, I64, I, I, I, P, P, P, SZ, P]),
     "mpdok_residual_f64": (I, [P, P, I64, I, I, P, P, P, I, P, SZ]),
+    "mpdok_residual_f64_noalloc": (I, [P, P, I64, I, I, P, P, P, I]),
     # RBF assembly
     "mpdok_rbf_assemble": (I, [P, P, I, I, I, D, D, I, I, I, I, I, P, I, I64, I]),
     "mpdok_rbf_mean_d2": (I, [P, P, I, I, P]),

@@ -15,6 +15,8 @@
 // dense_krylov.py:65-79 and :391,:416 (FP64 residual), mpdok_ooc.py:158-201
 // (tile SGEMV), the cublasDgemv residual inside lu_ir.so/mpdok.so
 // (benchmark_mpdok.ipynb:991).
+#include <mutex>
+
 #include "common.h"
 #include "gemv.h"
 
@@ -297,6 +299,7 @@ __global__ __launch_bounds__(256) void resid_cm_reduce_kernel(const double* __re
 // ---------------------------------------------------------------------------
 static int cm_chunks(int M, int rows_per_block, int N) {
   const int nrb = (M + rows_per_block - 1) / rows_per_block;
+  if (nrb < 1) return 1;
   int nch = (2048 + nrb - 1) / nrb;
   int maxch = (N + 63) / 64;
   if (nch > maxch) nch = maxch;
@@ -398,6 +401,55 @@ int residual_f64(hipStream_t s, const double* A, int64_t lda, int order, int N, 
   return OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free residual. The column-major residual's partial sums (high
+// and low parts per column chunk and row, resid_workspace_bytes) live in a
+// file-scope device array instead of a caller workspace. Each GPU holds its
+// own copy of the array at its own address; the host looks that address up
+// once per device (hipGetSymbolAddress, with the device current) and passes
+// it to the kernels. The partial kernel writes every (chunk, row) slot that
+// the reduction reads, so nothing needs clearing between calls. Calls that
+// overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kResidScratchBytes = size_t(16) << 20;  // 16 MiB per GPU
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kResidScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device, then cached).
+static int scratch_address(void** out) {
+  int dev = -1;
+  MPDOK_HIP(hipGetDevice(&dev));
+  MPDOK_REQUIRE(dev >= 0 && dev < kMaxDevices, "residual_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    MPDOK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MPDOK_REQUIRE(p != nullptr, "residual_noalloc: no device address for the scratch array");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return OK;
+}
+
+int residual_f64_noalloc(hipStream_t s, const double* A, int64_t lda, int order, int N,
+                         const double* x, const double* b, double* r, int mode) {
+  MPDOK_REQUIRE(N >= 0, "residual_noalloc: N < 0");
+  MPDOK_REQUIRE(order == ROW_MAJOR || order == COL_MAJOR, "residual_noalloc: bad order");
+  const size_t need = resid_workspace_bytes(N, order);
+  MPDOK_REQUIRE(need <= kResidScratchBytes,
+                "residual_noalloc: N = %d (column-major) needs %zu bytes of scratch, more than "
+                "the %zu the library holds per GPU; use mpdok_residual_f64 with a workspace",
+                N, need, kResidScratchBytes);
+  void* work = nullptr;
+  if (need > 0 && N > 0) MPDOK_TRY(scratch_address(&work));
+  return residual_f64(s, A, lda, order, N, x, b, r, mode, work, need);
+}
+
 }  // namespace mpdok
 
 // ---------------------------------------------------------------------------
@@ -418,6 +470,11 @@ MPDOK_API int mpdok_gemv_f64(hipStream_t s, const double* A, int64_t lda, int or
                              const double* x, double* y, void* work, size_t work_bytes,
                              const int* skip) {
   return mpdok::gemv_f64(s, A, lda, order, M, N, x, y, 1.0, work, work_bytes, skip);
+}
+MPDOK_API int mpdok_residual_f64_noalloc(hipStream_t s, const double* A, int64_t lda, int order,
+                                         int N, const double* x, const double* b, double* r,
+                                         int mode) {
+  return mpdok::residual_f64_noalloc(s, A, lda, order, N, x, b, r, mode);
 }
 MPDOK_API int mpdok_residual_f64(hipStream_t s, const double* A, int64_t lda, int order, int N,
                                  const double* x, const double* b, double* r, int mode, void* work,
