@@ -131,6 +131,13 @@ int mpdok_residual_f64_noalloc(hipStream_t s, const double* A, int64_t lda, int 
 int mpdok_rbf_assemble(hipStream_t s, const double* coords, int N, int D, int kind, double p,
                        double reg, int reg_mode, int row0, int rows, int col0, int cols,
                        void* out, int out_dtype, int64_t ld, int order);
+/* Diagonal of the kernel matrix exactly as the assembly and the matrix-free
+ * products form it (elem_dtype F64: nugget in FP64; F32: the FP32 tile's
+ * element, nugget after the cast), written as out_dtype, inverted when
+ * invert != 0 (the Jacobi preconditioner of the out-of-core CG). No
+ * reference counterpart (the reference has no Jacobi-CG). */
+int mpdok_rbf_diag(hipStream_t s, const double* coords, int N, int D, int kind, double p,
+                   double reg, int elem_dtype, int invert, void* out, int out_dtype);
 /* out[0] = mean clamped d2 over the first n points (rbf_kernel.py:57-68). */
 int mpdok_rbf_mean_d2(hipStream_t s, const double* coords, int n, int D, double* out);
 /* Matrix-free y = K[row0:row0+rows, :] x, kernel values recomputed, never
@@ -206,6 +213,20 @@ int mpdok_scale_f64(hipStream_t s, const double* in, double* out, int n, const d
  * work: 4096 doubles. */
 int mpdok_cg_step_updates(hipStream_t s, float* x, float* r, float* p, const float* Ap, int n,
                           const double* rs_old, const double* pAp, double* rs_new, void* work);
+
+/* Preconditioned CG updates with device scalars (dense_krylov.py:191-198 with
+ * a diagonal preconditioner), vectors FP64 (dtype 0) or FP32 (1): alpha =
+ * rz_old/pAp, x += alpha p, r -= alpha Ap, z = dinv .* r, rz_new = r.z,
+ * rr_new = r.r (optional), p = z + (rz_new/rz_old) p. dinv NULL: plain CG (z
+ * is r; z may be NULL). alpha and beta are rounded to the vector type. work:
+ * mpdok_pcg_workspace_bytes(). The inner solver of the out-of-core Jacobi-CG
+ * and the FP64 body of cg(); no reference C symbol. */
+size_t mpdok_pcg_workspace_bytes(void);
+int mpdok_pcg_step_updates(hipStream_t s, int dtype, void* x, void* r, void* z, void* p,
+                           const void* Ap, const void* dinv, int n, const double* rz_old,
+                           const double* pAp, double* rz_new, double* rr_new, void* work);
+/* out = d .* in (dtype 0 FP64 / 1 FP32): the Jacobi preconditioner apply. */
+int mpdok_diag_scale(hipStream_t s, int dtype, const void* d, const void* in, void* out, int n);
 
 /* ---- LU factorisation ------------------------------------------------------
  * Opaque handle replacing the global factor slot of lu_ir.so

@@ -55,6 +55,7 @@ SIGNATURES = {
     # RBF assembly
     "mpdok_rbf_assemble": (I, [P, P, I, I, I, D, D, I, I, I, I, I, P, I, I64, I]),
     "mpdok_rbf_mean_d2": (I, [P, P, I, I, P]),
+    "mpdok_rbf_diag": (I, [P, P, I, I, I, D, D, I, I, P, I]),
     "mpdok_rbf_matvec": (I, [P, P, I, I, I, D, D, I, P, P, I, I, I]),
     "mpdok_rbf_cross_matvec": (I, [P, P, I, P, I, I, I, D, P, P]),
     "mpdok_rbf_dbilinear_workspace_bytes": (SZ, [I]),
@@ -74,6 +75,9 @@ SIGNATURES = {
     "mpdok_scale_f32": (I, [P, P, P, I, P]),
     "mpdok_scale_f64": (I, [P, P, P, I, P]),
     "mpdok_cg_step_updates": (I, [P, P, P, P, P, I, P, P, P, P]),
+    "mpdok_pcg_workspace_bytes": (SZ, []),
+    "mpdok_pcg_step_updates": (I, [P, I, P, P, P, P, P, P, I, P, P, P, P, P]),
+    "mpdok_diag_scale": (I, [P, I, P, P, P, I]),
     # LU factorisation / trailing-update kernels
     "mpdok_lu_create": (I, [I, I, I, I, I, PP]),
     "mpdok_lu_factor": (I, [P, P, P, I64, I, PI]),

@@ -103,3 +103,37 @@ def rbf_mean_d2(coords, n):
     check(lib().mpdok_rbf_mean_d2(_dev.stream(), coords.data_ptr(), int(n), coords.shape[1],
                                   o.data_ptr()), "rbf_mean_d2")
     return o
+
+
+def pcg_step(x, r, z, p, Ap, dinv, rz_old, pAp, rz_new, rr_new=None):
+    """One set of (preconditioned) CG updates on device vectors of one dtype
+    (FP32 or FP64) with device FP64 scalars: see mpdok_pcg_step_updates."""
+    ws = _dev.workspace(lib().mpdok_pcg_workspace_bytes(), "pcg")
+    check(lib().mpdok_pcg_step_updates(_dev.stream(), _dev.dtcode(x), x.data_ptr(), r.data_ptr(),
+                                       _dev.ptr(z), p.data_ptr(), Ap.data_ptr(), _dev.ptr(dinv),
+                                       x.numel(), rz_old.data_ptr(), pAp.data_ptr(),
+                                       rz_new.data_ptr(), _dev.ptr(rr_new), ws.data_ptr()),
+          "pcg_step_updates")
+
+
+def diag_scale(d, x, out=None):
+    """out = d .* x (same dtype, FP32 or FP64)."""
+    out = out if out is not None else torch.empty_like(x)
+    check(lib().mpdok_diag_scale(_dev.stream(), _dev.dtcode(x), d.data_ptr(), x.data_ptr(),
+                                 out.data_ptr(), x.numel()), "diag_scale")
+    return out
+
+
+def rbf_diag(coords, kind, param, reg, elem_dtype=torch.float32, out_dtype=torch.float32,
+             invert=False):
+    """Diagonal of the kernel matrix as the assembly forms it (elem FP32: the
+    stored FP32 tile's element), optionally inverted."""
+    N, D = coords.shape
+    out = torch.empty(N, dtype=out_dtype, device=coords.device)
+    check(lib().mpdok_rbf_diag(_dev.stream(), coords.data_ptr(), N, D, int(kind), float(param),
+                               float(reg), _DTC[elem_dtype], 1 if invert else 0, out.data_ptr(),
+                               _DTC[out_dtype]), "rbf_diag")
+    return out
+
+
+_DTC = {torch.float64: _lib.F64, torch.float32: _lib.F32}

@@ -309,6 +309,106 @@ int cg_step_updates(hipStream_t s, float* x, float* r, float* p, const float* Ap
   return OK;
 }
 
+// ---------------------------------------------------------------------------
+// Preconditioned conjugate gradients with a diagonal (Jacobi) preconditioner,
+// FP32 or FP64 vectors, every scalar on the device (dense_krylov.py:191-198
+// with z = M^-1 r in place of r in the direction update):
+//   pcg_update1: alpha = rz / pAp (rounded to T); x += alpha p; r -= alpha Ap;
+//                z = dinv .* r (no preconditioner: z is r); per-block
+//                partials of r.z and r.r in FP64
+//   pcg_update2: rz_new, rr_new = the partials summed in a fixed order by every
+//                block; beta = rz_new / rz (rounded to T); p = z + beta p
+// rr_new gives the recursive residual norm for the stopping test without a
+// second pass over r.
+// ---------------------------------------------------------------------------
+constexpr int PCG_BLOCKS = 512;
+
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_update1_kernel(
+    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+    const T* __restrict__ Ap, const T* __restrict__ dinv, int n, const double* __restrict__ rz,
+    const double* __restrict__ pAp, double* __restrict__ part) {
+  const T alpha = (T)(rz[0] / pAp[0]);
+  double a_rz = 0.0, a_rr = 0.0;
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+    x[j] = x[j] + alpha * p[j];
+    const T rn = r[j] - alpha * Ap[j];
+    r[j] = rn;
+    T zn = rn;
+    if (dinv) {
+      zn = rn * dinv[j];
+      z[j] = zn;
+    }
+    a_rz = __builtin_fma((double)rn, (double)zn, a_rz);
+    a_rr = __builtin_fma((double)rn, (double)rn, a_rr);
+  }
+  __shared__ double sh[4];
+  a_rz = block_sum_256(a_rz, sh);
+  a_rr = block_sum_256(a_rr, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a_rz;
+    part[PCG_BLOCKS + blockIdx.x] = a_rr;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_update2_kernel(const T* __restrict__ z,
+                                                          T* __restrict__ p, int n,
+                                                          const double* __restrict__ part, int np,
+                                                          const double* __restrict__ rz_old,
+                                                          double* __restrict__ rz_new,
+                                                          double* __restrict__ rr_new) {
+  __shared__ double sh[4];
+  double srz = 0.0, srr = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    srz += part[i];
+    srr += part[PCG_BLOCKS + i];
+  }
+  srz = block_sum_256(srz, sh);
+  srr = block_sum_256(srr, sh);
+  const T beta = (T)(srz / rz_old[0]);
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256)
+    p[j] = z[j] + beta * p[j];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    rz_new[0] = srz;
+    if (rr_new) rr_new[0] = srr;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void diag_scale_kernel(const T* __restrict__ d,
+                                                         const T* __restrict__ in,
+                                                         T* __restrict__ out, int n) {
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) out[j] = in[j] * d[j];
+}
+
+size_t pcg_workspace_bytes() { return 2 * PCG_BLOCKS * sizeof(double); }
+
+template <typename T>
+int pcg_step_updates(hipStream_t s, T* x, T* r, T* z, T* p, const T* Ap, const T* dinv, int n,
+                     const double* rz_old, const double* pAp, double* rz_new, double* rr_new,
+                     void* work) {
+  MPDOK_REQUIRE(x && r && p && Ap && rz_old && pAp && rz_new && work && n >= 1,
+                "pcg_step_updates: bad arguments");
+  MPDOK_REQUIRE(dinv == nullptr || z != nullptr, "pcg_step_updates: dinv needs a z vector");
+  dim3 g = grid_1d(n);
+  if (g.x > PCG_BLOCKS) g.x = PCG_BLOCKS;
+  double* part = static_cast<double*>(work);
+  pcg_update1_kernel<T><<<g, 256, 0, s>>>(x, r, z, p, Ap, dinv, n, rz_old, pAp, part);
+  pcg_update2_kernel<T><<<g, 256, 0, s>>>(dinv ? z : r, p, n, part, g.x, rz_old, rz_new, rr_new);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+template <typename T>
+int diag_scale(hipStream_t s, const T* d, const T* in, T* out, int n) {
+  MPDOK_REQUIRE(d && in && out && n >= 0, "diag_scale: bad arguments");
+  if (n == 0) return OK;
+  diag_scale_kernel<T><<<grid_1d(n), 256, 0, s>>>(d, in, out, n);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
 }  // namespace mpdok
 
 // ---------------------------------------------------------------------------
@@ -361,4 +461,25 @@ MPDOK_API int mpdok_cg_step_updates(hipStream_t s, float* x, float* r, float* p,
                                     int n, const double* rs_old, const double* pAp,
                                     double* rs_new, void* work) {
   return mpdok::cg_step_updates(s, x, r, p, Ap, n, rs_old, pAp, rs_new, work);
+}
+MPDOK_API size_t mpdok_pcg_workspace_bytes(void) { return mpdok::pcg_workspace_bytes(); }
+MPDOK_API int mpdok_pcg_step_updates(hipStream_t s, int dtype, void* x, void* r, void* z, void* p,
+                                     const void* Ap, const void* dinv, int n,
+                                     const double* rz_old, const double* pAp, double* rz_new,
+                                     double* rr_new, void* work) {
+  if (dtype == mpdok::F32)
+    return mpdok::pcg_step_updates<float>(s, (float*)x, (float*)r, (float*)z, (float*)p,
+                                          (const float*)Ap, (const float*)dinv, n, rz_old, pAp,
+                                          rz_new, rr_new, work);
+  MPDOK_REQUIRE(dtype == mpdok::F64, "pcg_step_updates: dtype must be 0 (FP64) or 1 (FP32)");
+  return mpdok::pcg_step_updates<double>(s, (double*)x, (double*)r, (double*)z, (double*)p,
+                                         (const double*)Ap, (const double*)dinv, n, rz_old, pAp,
+                                         rz_new, rr_new, work);
+}
+MPDOK_API int mpdok_diag_scale(hipStream_t s, int dtype, const void* d, const void* in, void* out,
+                               int n) {
+  if (dtype == mpdok::F32)
+    return mpdok::diag_scale<float>(s, (const float*)d, (const float*)in, (float*)out, n);
+  MPDOK_REQUIRE(dtype == mpdok::F64, "diag_scale: dtype must be 0 (FP64) or 1 (FP32)");
+  return mpdok::diag_scale<double>(s, (const double*)d, (const double*)in, (double*)out, n);
 }

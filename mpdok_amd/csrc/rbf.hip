@@ -124,6 +124,55 @@ int rbf_assemble(hipStream_t st, const double* coords, int N, int D, int kind, d
   return OK;
 }
 
+// Diagonal of the kernel matrix, element for element what the tile kernel and
+// the matrix-free products use: d2 of a point with itself by the Gram identity
+// (clamped at zero; rounding can leave it slightly positive), the kernel value,
+// the nugget in FP64 (elem F64) or after the FP32 cast (elem F32). Written as
+// `out_dtype`, optionally inverted (the Jacobi preconditioner's M^-1).
+template <typename TO>
+__global__ __launch_bounds__(256) void rbf_diag_kernel(const double* __restrict__ coords, int N,
+                                                       int D, int kind, double p, double reg,
+                                                       int elem, int invert, TO* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  double c[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) c[d] = d < D ? coords[(int64_t)i * D + d] : 0.0;
+  const double sq = sqnorm(c, D);
+  const double v = kernel_value(kind, p, gram_d2(c, sq, c, sq, D));
+  double e;
+  if (elem == F64) {
+    e = v + reg;
+  } else {
+    float f = (float)v;
+    f = f + (float)reg;
+    e = (double)f;
+  }
+  const TO o = (TO)e;
+  out[i] = invert ? (TO)1 / o : o;
+}
+
+int rbf_diag(hipStream_t st, const double* coords, int N, int D, int kind, double p, double reg,
+             int elem_dtype, int invert, void* out, int out_dtype) {
+  MPDOK_REQUIRE(coords && out && N >= 0, "rbf_diag: bad arguments");
+  MPDOK_REQUIRE(D >= 1 && D <= MAXD, "rbf_diag: D must be in [1, %d], got %d", MAXD, D);
+  MPDOK_REQUIRE(kind == 0 || kind == 1, "rbf_diag: unknown kernel kind %d", kind);
+  MPDOK_REQUIRE(!(kind == 1 && !(p > 0.0)), "rbf_diag: Matern length scale must be > 0");
+  MPDOK_REQUIRE(elem_dtype == F64 || elem_dtype == F32, "rbf_diag: elem dtype must be F64/F32");
+  if (N == 0) return OK;
+  const dim3 g((N + 255) / 256);
+  if (out_dtype == F64)
+    rbf_diag_kernel<double><<<g, 256, 0, st>>>(coords, N, D, kind, p, reg, elem_dtype, invert,
+                                               (double*)out);
+  else if (out_dtype == F32)
+    rbf_diag_kernel<float><<<g, 256, 0, st>>>(coords, N, D, kind, p, reg, elem_dtype, invert,
+                                              (float*)out);
+  else
+    MPDOK_REQUIRE(false, "rbf_diag: unsupported output dtype %d", out_dtype);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
 // Mean of the clamped squared distances over the first n points, diagonal
 // included (rbf_kernel.py:57-68). Single block; n <= 512 in the reference.
 __global__ __launch_bounds__(256) void mean_d2_kernel(const double* __restrict__ coords, int n,
@@ -389,4 +438,9 @@ MPDOK_API int mpdok_rbf_assemble(hipStream_t st, const double* coords, int N, in
 }
 MPDOK_API int mpdok_rbf_mean_d2(hipStream_t st, const double* coords, int n, int D, double* out) {
   return mpdok::rbf_mean_d2(st, coords, n, D, out);
+}
+MPDOK_API int mpdok_rbf_diag(hipStream_t st, const double* coords, int N, int D, int kind,
+                             double p, double reg, int elem_dtype, int invert, void* out,
+                             int out_dtype) {
+  return mpdok::rbf_diag(st, coords, N, D, kind, p, reg, elem_dtype, invert, out, out_dtype);
 }

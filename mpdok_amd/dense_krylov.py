@@ -107,28 +107,52 @@ def _as32(v):
 # ----------------------------------------------------------------------
 # Conjugate gradients (dense_krylov.py:119-205)
 # ----------------------------------------------------------------------
-def cg(op, b, x0=None, tol=1e-6, maxiter=None, check_every=10, verbose=False):
+def cg(op, b, x0=None, tol=1e-6, maxiter=None, check_every=10, verbose=False, body="fp32",
+       precond=None):
     """CG for SPD A: FP32 vectors, FP64 residual check every `check_every`
     iterations before the update, stop on <1% improvement between checks.
 
+    Extensions (defaults reproduce the reference): body='fp64' runs the
+    iteration on the FP64 copy with FP64 vectors (the plain FP64 CG);
+    precond='jacobi' preconditions with diag(A)^-1.
+
     Returns (x FP64 (N,), history [(iter, relres)], converged).
     """
+    if body not in ("fp32", "fp64"):
+        raise ValueError(f"body must be 'fp32' or 'fp64', got {body!r}")
+    if precond not in (None, "jacobi"):
+        raise ValueError(f"precond must be None or 'jacobi', got {precond!r}")
     N = op.N
     if maxiter is None:
         maxiter = N
+    dt = torch.float32 if body == "fp32" else torch.float64
+    A = op.A_fp32 if body == "fp32" else op.A_fp64
+    if A is None:
+        raise RuntimeError("body='fp64' needs the FP64 copy, which was freed")
     b64 = _as64(b)
-    b32 = _ops.axpby(1.0, b64, dtype=torch.float32)
+    bv = _ops.axpby(1.0, b64, dtype=dt)
     b_norm = _ops.norm(b64)
     if b_norm == 0.0:
         return _dev.zeros(N, torch.float64), [(0, 0.0)], True
 
-    x = _dev.zeros(N, torch.float32) if x0 is None else _as32(x0)
-    r = _ops.axpby(1.0, b32, -1.0, op.matvec(x))
-    p = r.clone()
+    if x0 is None:
+        x = _dev.zeros(N, dt)
+    else:
+        x = _as32(x0) if body == "fp32" else _as64(x0).clone()
+    r = _ops.axpby(1.0, bv, -1.0, _ops.matvec(A, x))
+    dinv = None
+    if precond == "jacobi":
+        # setup only: the diagonal is read once and inverted on the host
+        d = A.diagonal().cpu().numpy().astype(np.float64)
+        dinv = _dev.as_device((1.0 / d).astype(np.float32 if body == "fp32" else np.float64), dt)
+    z = _ops.diag_scale(dinv, r) if dinv is not None else None
+    p = (z if z is not None else r).clone()
     Ap = torch.empty_like(r)
-    rs = [_ops.dot(r, r), torch.empty(1, dtype=torch.float64, device=r.device)]
+    rs = [_ops.dot(r, z if z is not None else r),
+          torch.empty(1, dtype=torch.float64, device=r.device)]
     pAp = torch.empty(1, dtype=torch.float64, device=r.device)
     ws = _dev.workspace(8 * 4096, "cg")
+    reference_kernel = body == "fp32" and dinv is None
 
     history = []
     converged = False
@@ -149,20 +173,54 @@ def cg(op, b, x0=None, tol=1e-6, maxiter=None, check_every=10, verbose=False):
                     print(f"  CG stagnated at iter {k}: {prev_rel:.3e} → {rel:.3e}")
                 break
             prev_rel = rel
-        _ops.matvec(op.A_fp32, p, out=Ap)
+        _ops.matvec(A, p, out=Ap)
         _ops.dot(p, Ap, out=pAp)
         cur, nxt = rs[k % 2], rs[(k + 1) % 2]
-        check(lib().mpdok_cg_step_updates(_dev.stream(), x.data_ptr(), r.data_ptr(),
-                                          p.data_ptr(), Ap.data_ptr(), N, cur.data_ptr(),
-                                          pAp.data_ptr(), nxt.data_ptr(), ws.data_ptr()),
-              "cg_step_updates")
+        if reference_kernel:
+            check(lib().mpdok_cg_step_updates(_dev.stream(), x.data_ptr(), r.data_ptr(),
+                                              p.data_ptr(), Ap.data_ptr(), N, cur.data_ptr(),
+                                              pAp.data_ptr(), nxt.data_ptr(), ws.data_ptr()),
+                  "cg_step_updates")
+        else:
+            _ops.pcg_step(x, r, z, p, Ap, dinv, cur, pAp, nxt)
 
     if not converged:
         rel = op.residual_norm(x, b64) / b_norm
         history.append((k + 1, rel))
         converged = rel < tol
 
-    return _ops.axpby(1.0, x, dtype=torch.float64), history, converged
+    return (_ops.axpby(1.0, x, dtype=torch.float64) if body == "fp32" else x), history, converged
+
+
+def _pcg_inner(matvec, rhs, dtype, dinv=None, tol=1e-6, maxiter=1000, check_every=10):
+    """(Jacobi-)PCG on A e = rhs from e = 0 with vectors in `dtype` and device
+    scalars; stops when the recursive residual satisfies ||r|| < tol ||rhs||
+    (read on the host every `check_every` iterations) or after maxiter
+    iterations. matvec(v) returns A v in `dtype`. Returns (e, iterations,
+    ||r|| / ||rhs||)."""
+    r = _ops.axpby(1.0, rhs, dtype=dtype)
+    beta = _ops.norm(r)
+    N = r.numel()
+    e = _dev.zeros(N, dtype)
+    if beta == 0.0:
+        return e, 0, 0.0
+    z = _ops.diag_scale(dinv, r) if dinv is not None else None
+    p = (z if z is not None else r).clone()
+    rs = [_ops.dot(r, z if z is not None else r),
+          torch.empty(1, dtype=torch.float64, device=r.device)]
+    rr = torch.empty(1, dtype=torch.float64, device=r.device)
+    pAp = torch.empty(1, dtype=torch.float64, device=r.device)
+    rel, it = 1.0, 0
+    for it in range(1, maxiter + 1):
+        Ap = matvec(p)
+        _ops.dot(p, Ap, out=pAp)
+        cur, nxt = rs[(it - 1) % 2], rs[it % 2]
+        _ops.pcg_step(e, r, z, p, Ap, dinv, cur, pAp, nxt, rr)
+        if it % check_every == 0 or it == maxiter:
+            rel = float(rr.item()) ** 0.5 / beta
+            if rel < tol:
+                break
+    return e, it, rel
 
 
 # ----------------------------------------------------------------------

@@ -24,7 +24,10 @@ What runs where:
 Extension (not in the reference): ``store='free'`` keeps nothing but the
 coordinates and recomputes the FP32 tile values inside the inner product as
 well (mpdok_rbf_matvec, FP32 elements: bit for bit the stored tile's values),
-and ``kernel='matern32'`` with ``length_scale`` selects the Matern-3/2 kernel.
+``kernel='matern32'`` with ``length_scale`` selects the Matern-3/2 kernel, and
+``solve(..., inner='cg', precond='jacobi')`` refines with a Jacobi-preconditioned
+FP32 CG as the inner solver (SPD kernels; the FP64 residual, update and stopping
+test stay as in the reference's outer loop).
 
 Deviation kept from the other solvers here: on an Arnoldi breakdown the
 rotation of the last Hessenberg column is applied before stopping (the
@@ -69,6 +72,7 @@ class MPDOKOOCSolver:
         self._pinned = None      # torch tensor owning the page-locked _ram_buf
         self._dev_tiles = None   # device workspace: two tiles
         self._stage = None       # two page-locked tile buffers for the on-disk store
+        self.last_info = None    # outer history and inner iterations of the last solve
 
     # ---- build (mpdok_ooc.py:75-154) ----------------------------------------
     def build(self, coords, gamma=None, reg=1e-6, store='ram', path=None, verbose=True,
@@ -318,15 +322,55 @@ class MPDOKOOCSolver:
                                            y_dev.data_ptr(), e.data_ptr(), N, 1), "basis_update")
         return e
 
+    # ---- inner FP32 Jacobi-CG (extension) -----------------------------------
+    def _jacobi_dinv(self):
+        """diag(A_fp32)^-1 in FP32: from the caller's host matrix for a
+        user-supplied _ram_buf, otherwise recomputed from the coordinates
+        exactly as the stored FP32 tiles hold it (nugget after the cast)."""
+        buf = self._ram_buf
+        own = (self._pinned is not None and isinstance(buf, np.ndarray)
+               and buf.ctypes.data == self._pinned.data_ptr())
+        if self._store == 'ram' and buf is not None and not own:
+            d = np.asarray(np.diagonal(buf), dtype=np.float32)
+            return _dev.as_device(np.float32(1.0) / d, torch.float32)
+        c = self._coords_dev()
+        return _ops.rbf_diag(c, self._kind, self._param(), self.reg, torch.float32,
+                             torch.float32, invert=True)
+
+    def _inner_cg(self, rhs_fp64, tol=1e-6, maxiter=None, precond='jacobi', check_every=10):
+        """FP32 conjugate gradients on A_fp32 e = rhs from e = 0 (CG updates
+        of dense_krylov.py:191-198 with z = M^-1 r, M = diag(A_fp32) for
+        precond='jacobi'), products through _tiled_sgemv. Stops when the
+        recursive residual ||r|| < tol ||rhs||. Returns (e FP32, iterations)."""
+        from .dense_krylov import _pcg_inner
+        dinv = self._jacobi_dinv() if precond == 'jacobi' else None
+        rhs = _dev.as_device(rhs_fp64, torch.float64)
+        e, it, _ = _pcg_inner(self._tiled_sgemv, rhs, torch.float32, dinv, tol,
+                              self.N if maxiter is None else int(maxiter), check_every)
+        return e, it
+
     # ---- outer FP64 refinement (mpdok_ooc.py:315-373) ----------------------
-    def solve(self, b, tol=1e-5, maxiter_outer=20, restart=50, verbose=True):
-        """GMRES-IR: FP64 residual (matrix-free) + inner FP32 GMRES.
-        Returns x (N,) FP64 on the device."""
+    def solve(self, b, tol=1e-5, maxiter_outer=20, restart=50, verbose=True, inner='gmres',
+              precond=None, inner_tol=1e-6, inner_maxiter=None):
+        """Iterative refinement: FP64 residual (matrix-free) + inner FP32 solve.
+        Returns x (N,) FP64 on the device.
+
+        inner='gmres' (default, the reference's): GMRES(restart) with DGKS.
+        inner='cg' (extension, SPD kernels): CG with vectors in FP32, Jacobi
+        preconditioned for precond='jacobi', to ||r|| < inner_tol ||rhs|| or
+        inner_maxiter iterations (default N). last_info records the outer
+        relres history and the inner iteration counts."""
         if self.N is None:
             raise RuntimeError("Call build() before solve()")
+        if inner not in ('gmres', 'cg'):
+            raise ValueError(f"inner must be 'gmres' or 'cg', got {inner!r}")
+        if precond not in (None, 'jacobi'):
+            raise ValueError(f"precond must be None or 'jacobi', got {precond!r}")
         b_gpu = _dev.as_device(b, torch.float64)
         b_norm = _ops.norm(b_gpu)
+        self.last_info = {"history": [], "inner_iterations": [], "converged": False}
         if b_norm == 0.0:
+            self.last_info["converged"] = True
             return _dev.zeros(self.N, torch.float64)
         x = _dev.zeros(self.N, torch.float64)
         converged = False
@@ -335,6 +379,7 @@ class MPDOKOOCSolver:
             t0 = time.perf_counter()
             r = _ops.axpby(1.0, b_gpu, -1.0, self._tiled_dgemv_fp64(x))
             rel = _ops.norm(r) / b_norm
+            self.last_info["history"].append(rel)
             t_dgemv = time.perf_counter() - t0
             if verbose:
                 print(f"  outer {outer}: rel_res={rel:.2e}  (FP64 DGEMV {t_dgemv:.1f}s)",
@@ -343,10 +388,18 @@ class MPDOKOOCSolver:
                 converged = True
                 break
             t0 = time.perf_counter()
-            e = self._inner_gmres(r, restart)
-            if verbose:
-                print(f"           inner GMRES {time.perf_counter() - t0:.1f}s", flush=True)
+            if inner == 'cg':
+                e, its = self._inner_cg(r, inner_tol, inner_maxiter, precond)
+                self.last_info["inner_iterations"].append(its)
+                if verbose:
+                    print(f"           inner CG {its} iterations "
+                          f"{time.perf_counter() - t0:.1f}s", flush=True)
+            else:
+                e = self._inner_gmres(r, restart)
+                if verbose:
+                    print(f"           inner GMRES {time.perf_counter() - t0:.1f}s", flush=True)
             x = _ops.axpby(1.0, x, 1.0, e, dtype=torch.float64)
+        self.last_info["converged"] = converged
         if not converged:
             warnings.warn(
                 f"MPDOKOOCSolver did not converge within {maxiter_outer} outer "

@@ -2,7 +2,10 @@
 
 CPU restatements of the reference algorithms with the reference's precisions:
 FP32 Krylov vectors, FP64 residual checks and updates.
-  cg        dense_krylov.py:119-205
+  cg        dense_krylov.py:119-205 (plus the FP64 body and the diagonal
+            preconditioner the product offers as extensions)
+  pcg / cg_ir  FP32 Jacobi-PCG and its FP64 refinement loop (the outer loop
+            of mpdok_ooc.py:315-373 around a CG inner solver)
   gmres     dense_krylov.py:212-331 (one-pass batched CGS, lstsq on (m+1) x m H)
   gmres_ir  dense_krylov.py:338-421 (trailing (maxiter_outer, rel) history entry)
   inner_gmres / ooc_solve   mpdok_ooc.py:231-373 (two-pass DGKS, Givens,
@@ -29,18 +32,25 @@ class DenseOp:
         return float(np.linalg.norm(b64 - self.A64 @ x.astype(np.float64)))
 
 
-def cg(op, b, x0=None, tol=1e-6, maxiter=None, check_every=10):
+def cg(op, b, x0=None, tol=1e-6, maxiter=None, check_every=10, body="fp32", dinv=None):
+    """dense_krylov.py:119-205. body='fp64' runs the same loop on the FP64
+    matrix with FP64 vectors; dinv (a vector) preconditions with z = dinv r
+    (p = z + beta p, scalars r.z): the product's extensions."""
     N = op.N
     maxiter = N if maxiter is None else maxiter
+    dt = np.float32 if body == "fp32" else np.float64
+    mv = op.matvec if body == "fp32" else (lambda v: op.A64 @ v)
     b64 = np.asarray(b, np.float64)
-    b32 = b64.astype(np.float32)
+    bv = b64.astype(dt)
     b_norm = float(np.linalg.norm(b64))
     if b_norm == 0.0:
         return np.zeros(N), [(0, 0.0)], True
-    x = np.zeros(N, np.float32) if x0 is None else np.asarray(x0, np.float32).copy()
-    r = b32 - op.matvec(x)
-    p = r.copy()
-    rs = float(np.dot(r.astype(np.float64), r))
+    x = np.zeros(N, dt) if x0 is None else np.asarray(x0, dt).copy()
+    dv = None if dinv is None else np.asarray(dinv, dt)
+    r = bv - mv(x)
+    z = r if dv is None else r * dv
+    p = z.copy()
+    rs = float(np.dot(r.astype(np.float64), z))
     history, converged, prev_rel = [], False, float("inf")
     k = -1
     for k in range(maxiter):
@@ -53,19 +63,75 @@ def cg(op, b, x0=None, tol=1e-6, maxiter=None, check_every=10):
             if rel > prev_rel * (1.0 - 1e-2):
                 break
             prev_rel = rel
-        Ap = op.matvec(p)
-        alpha = np.float32(rs / float(np.dot(p.astype(np.float64), Ap)))
+        Ap = mv(p)
+        alpha = dt(rs / float(np.dot(p.astype(np.float64), Ap)))
         x = x + alpha * p
         r = r - alpha * Ap
-        rs_new = float(np.dot(r.astype(np.float64), r))
-        beta = np.float32(rs_new / rs)
-        p = r + beta * p
+        z = r if dv is None else r * dv
+        rs_new = float(np.dot(r.astype(np.float64), z))
+        beta = dt(rs_new / rs)
+        p = z + beta * p
         rs = rs_new
     if not converged:
         rel = op.residual_norm(x, b64) / b_norm
         history.append((k + 1, rel))
         converged = rel < tol
     return x.astype(np.float64), history, converged
+
+
+def pcg(matvec, b, dinv=None, tol=1e-6, maxiter=1000, check_every=10, dtype=np.float32):
+    """(Jacobi-)PCG on A e = b from e = 0 with vectors in `dtype` and alpha,
+    beta rounded to it (the CG updates of dense_krylov.py:191-198 with z =
+    dinv r). Stops when the recursive residual ||r|| < tol ||b||, tested
+    every `check_every` iterations and at maxiter. Returns (e, iterations)."""
+    r = np.asarray(b, np.float64).astype(dtype)
+    beta0 = float(np.linalg.norm(r.astype(np.float64)))
+    e = np.zeros(r.shape[0], dtype)
+    if beta0 == 0.0:
+        return e, 0
+    dv = None if dinv is None else np.asarray(dinv, dtype)
+    z = r if dv is None else r * dv
+    p = z.copy()
+    rz = float(np.dot(r.astype(np.float64), z))
+    it = 0
+    for it in range(1, maxiter + 1):
+        Ap = np.asarray(matvec(p), dtype)
+        alpha = dtype(rz / float(np.dot(p.astype(np.float64), Ap)))
+        e = e + alpha * p
+        r = r - alpha * Ap
+        z = r if dv is None else r * dv
+        rz_new = float(np.dot(r.astype(np.float64), z))
+        p = z + dtype(rz_new / rz) * p
+        rz = rz_new
+        if it % check_every == 0 or it == maxiter:
+            if float(np.linalg.norm(r.astype(np.float64))) < tol * beta0:
+                break
+    return e, it
+
+
+def cg_ir(matvec32, matvec64, b, dinv=None, tol=1e-10, maxiter_outer=20, inner_tol=1e-6,
+          inner_maxiter=None, check_every=10):
+    """The outer FP64 refinement of mpdok_ooc.py:315-373 (r = b - A x in FP64,
+    stop when ||r||/||b|| < tol, x += e in FP64) with pcg in FP32 as the
+    inner solver. Returns (x, converged, outer relres history, inner
+    iteration counts)."""
+    b = np.asarray(b, np.float64)
+    b_norm = float(np.linalg.norm(b))
+    x = np.zeros_like(b)
+    hist, inner = [], []
+    if b_norm == 0.0:
+        return x, True, hist, inner
+    mi = b.shape[0] if inner_maxiter is None else inner_maxiter
+    for _ in range(maxiter_outer):
+        r = b - matvec64(x)
+        rel = float(np.linalg.norm(r)) / b_norm
+        hist.append(rel)
+        if rel < tol:
+            return x, True, hist, inner
+        e, it = pcg(matvec32, r, dinv, inner_tol, mi, check_every, np.float32)
+        inner.append(it)
+        x = x + e.astype(np.float64)
+    return x, False, hist, inner
 
 
 def gmres(op, b, x0=None, tol=1e-6, maxiter=None, restart=50):

@@ -146,6 +146,80 @@ def test_cg_identity_one_step_and_zero_rhs():
     assert conv and hist == [(0, 0.0)] and not x.any()
 
 
+def test_pcg_jacobi_solves_a_diagonal_system_in_one_iteration():
+    d = np.linspace(1.0, 50.0, 40)
+    b = np.random.default_rng(1).standard_normal(40)
+    e, it = ok.pcg(lambda v: d * v, b, dinv=1.0 / d, tol=1e-14, maxiter=10, check_every=1,
+                   dtype=np.float64)
+    assert it == 1
+    assert np.allclose(e, b / d, rtol=1e-14, atol=0)
+    # without the preconditioner the 40 distinct eigenvalues need many steps
+    _, it2 = ok.pcg(lambda v: d * v, b, tol=1e-14, maxiter=100, check_every=1, dtype=np.float64)
+    assert it2 > 10
+
+
+def test_pcg_three_distinct_eigenvalues_and_scipy_agreement():
+    rng = np.random.default_rng(2)
+    Q, _ = np.linalg.qr(rng.standard_normal((30, 30)))
+    A = Q @ np.diag(np.repeat([1.0, 4.0, 9.0], 10)) @ Q.T
+    b = rng.standard_normal(30)
+    e, it = ok.pcg(lambda v: A @ v, b, tol=1e-12, maxiter=30, check_every=1, dtype=np.float64)
+    assert it == 3
+    assert np.allclose(A @ e, b, atol=1e-10)
+    # Jacobi-preconditioned, against SciPy's cg with M = diag(A)^-1
+    X = rng.uniform(0, 1, (300, 3))
+    K = orb.build_kernel(X, "matern32", 0.1, 1e-2) * np.linspace(1.0, 3.0, 300)[:, None]
+    K = (K + K.T) / 2
+    dinv = 1.0 / np.diag(K)
+    bb = rng.standard_normal(300)
+    e, _ = ok.pcg(lambda v: K @ v, bb, dinv=dinv, tol=1e-12, maxiter=2000, check_every=1,
+                  dtype=np.float64)
+    M = spla.LinearOperator((300, 300), matvec=lambda v: dinv * v)
+    es, info = spla.cg(K, bb, M=M, rtol=1e-12, maxiter=2000)
+    assert info == 0
+    ref = np.linalg.solve(K, bb)
+    assert np.linalg.norm(e - ref) / np.linalg.norm(ref) < 1e-8
+    assert np.linalg.norm(es - ref) / np.linalg.norm(ref) < 1e-8
+
+
+def test_cg_fp64_body_and_jacobi_match_scipy():
+    rng = np.random.default_rng(3)
+    X = rng.uniform(0, 1, (400, 2))
+    A = orb.build_kernel(X, "gaussian", 4.0, 1e-1)
+    rng.standard_normal(400)
+    b = rng.standard_normal(400)
+    op = ok.DenseOp(A)
+    ref = np.linalg.solve(A, b)
+    x, hist, conv = ok.cg(op, b, tol=1e-12, maxiter=4000, body="fp64")
+    assert conv and hist[-1][1] < 1e-12
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-9
+    # the FP32 body cannot reach 1e-12: the FP64 checks see it stagnate
+    _, hist32, conv32 = ok.cg(op, b, tol=1e-12, maxiter=4000)
+    assert not conv32 and hist32[-1][1] > 1e-9
+    # a diagonal system is solved by one Jacobi-preconditioned step
+    d = np.linspace(1.0, 9.0, 50)
+    opd = ok.DenseOp(np.diag(d))
+    bd = rng.standard_normal(50)
+    xd, hd, cd = ok.cg(opd, bd, tol=1e-13, body="fp64", dinv=1.0 / d, check_every=1)
+    assert cd and hd[-1][0] == 1
+    assert np.allclose(xd, bd / d, rtol=1e-14, atol=0)
+
+
+def test_cg_ir_reaches_1e10_with_fp32_inner_cg():
+    X = np.random.default_rng(4).uniform(0, 1, (600, 3))
+    A = orb.build_kernel(X, "matern32", 0.1, 1e-4)
+    A32 = orb.build_fp32_tile(X, 0, 600, "matern32", 0.1, 1e-4)
+    b = np.sin(2 * np.pi * X[:, 0])
+    dinv = (np.float32(1.0) / np.diag(A32)).astype(np.float32)
+    x, conv, hist, inner = ok.cg_ir(lambda v: A32 @ v.astype(np.float32), lambda v: A @ v, b,
+                                    dinv=dinv, tol=1e-10)
+    assert conv and hist[-1] < 1e-10 and len(hist) >= 2
+    assert all(h1 < h0 for h0, h1 in zip(hist, hist[1:]))
+    ref = np.linalg.solve(A, b)
+    kappa = np.linalg.cond(A)
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < kappa * 1e-10
+
+
 def test_gmres_three_distinct_eigenvalues_solve_in_three_steps():
     # The Krylov space of A with 3 distinct eigenvalues has dimension 3: one
     # restart cycle of 3 steps reaches the FP32 floor.
