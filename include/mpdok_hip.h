@@ -67,14 +67,15 @@ void py_lu_free_factored(void);
 const char* mpdok_last_error(void);
 /* Stream used by the reference-compatible symbols (NULL = null stream). */
 int mpdok_set_stream(hipStream_t s);
-/* Kernel-shape choices picked by the tuner. Key 0: block shape of the
- * int8-sliced GEMM, 0..14 (0 = pipelined 128 x 64 tiles, 2 workgroups per CU
- * (default); 1-5 the unpipelined kernel's shapes; 6-9, 13 and 14 other
- * pipelined shapes; 10-12 persistent kernels). Key 1: row tiles per group of its tile
- * order (0 = 8). Key 2: start-up stagger of the persistent shapes (0..64).
- * Key 3: 64 x 64 diagonal LU kernel (0 = blocked, 1 = unblocked). Key 4: row
- * slicing (0 = two passes over a 2-D grid, 1 = one workgroup per 64 rows).
- * No reference counterpart. */
+/* Kernel-shape choices picked by the tuner (0 is every key's default). Key 0:
+ * block shape of the int8-sliced trailing update, 0..3 (0 = 128 x 128 tiles,
+ * 8 wavefronts, four stages; 1 = 128 x 64 tiles, two workgroups per CU; 2 =
+ * 128 x 128, two stages; 3 = 256 x 64). Key 1: row tiles per group of its
+ * tile order (0 = 8). Key 3: 64 x 64 diagonal LU kernel (0 = blocked, 1 =
+ * unblocked). Key 4: row slicing (0 = two passes over a 2-D grid, 1 = one
+ * workgroup per 64 rows). Key 5: the trailing update's C (0 = read into
+ * registers at the start of each tile for slice counts up to 3, 1 = read
+ * after the K loop). Keys 2, 6, 7 are unused. No reference counterpart. */
 int mpdok_set_tuning(int key, int value);
 int mpdok_get_tuning(int key);
 size_t mpdok_reduce_workspace_bytes(int n);

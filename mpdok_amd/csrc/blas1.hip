@@ -20,11 +20,11 @@ int tuning_set(int key, int value) {
   MPDOK_REQUIRE(key >= 0 && key < TUNE_COUNT, "tuning: unknown key %d", key);
   MPDOK_REQUIRE(key != TUNE_OZAKI_GROUP || (value >= 0 && value <= 256),
                 "tuning: group %d out of range", value);
-  MPDOK_REQUIRE(key != TUNE_OZAKI_SHAPE || (value >= 0 && value <= 15),
-                "tuning: key %d out of range", key);
-  MPDOK_REQUIRE(key != TUNE_OZAKI_STAGGER || (value >= 0 && value <= 64),
-                "tuning: stagger %d out of range", value);
-  MPDOK_REQUIRE((key != TUNE_DIAG_KERNEL && key != TUNE_SLICE_ROWS) || (value >= 0 && value <= 1),
+  MPDOK_REQUIRE(key != TUNE_OZAKI_SHAPE || (value >= 0 && value <= 3),
+                "tuning: trailing-update shape must be in [0, 3], got %d", value);
+  MPDOK_REQUIRE(key != 2 && key != 6 && key != 7, "tuning: key %d is not used", key);
+  MPDOK_REQUIRE((key != TUNE_DIAG_KERNEL && key != TUNE_SLICE_ROWS && key != TUNE_OZAKI_EPI) ||
+                    (value >= 0 && value <= 1),
                 "tuning: key %d takes 0 or 1, got %d", key, value);
   g_tuning[key] = value;
   return OK;

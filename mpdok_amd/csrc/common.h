@@ -28,11 +28,12 @@ const char* error_string();  // this thread's last error message
 // Process-wide kernel-shape choices (set by the tuner through
 // mpdok_set_tuning; read at launch time).
 enum TuneKey : int {
-  TUNE_OZAKI_SHAPE = 0,  // int8-sliced GEMM block shape (waves M x N, stages, workgroups per CU)
+  TUNE_OZAKI_SHAPE = 0,  // trailing update's block shape (0: 128 x 128, four stages; 1: 128 x 64,
+                         // two workgroups per CU; 2: 128 x 128, two stages; 3: 256 x 64)
   TUNE_OZAKI_GROUP = 1,  // int8-sliced GEMM: row tiles per group of the tile order (0: 8)
-  TUNE_OZAKI_STAGGER = 2,  // persistent int8-sliced GEMM: start-up stagger unit (x 8128 cycles)
   TUNE_DIAG_KERNEL = 3,    // 64 x 64 diagonal LU: 0 = blocked, 1 = unblocked
   TUNE_SLICE_ROWS = 4,     // int8 row slicing: 0 = 2-D grid (two passes), 1 = one workgroup per 64 rows
+  TUNE_OZAKI_EPI = 5,      // trailing update's C: 0 = read at the tile's start (S <= 3), 1 = after the K loop
   TUNE_COUNT = 8,
 };
 int tuning_get(int key);

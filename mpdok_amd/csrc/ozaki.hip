@@ -20,16 +20,22 @@
 // record per (row, 32-wide K block), so that the bytes one K step of one block
 // tile needs are a single contiguous run per slice, copied straight into LDS.
 //
-// Kernel shape (default): 256 threads = 2 x 2 wavefronts, two workgroups per
-// CU; wavefront tile 64 x 32 (two v_mfma_i32_32x32x32_i8 tiles), block tile
-// 128 x 64, K step 32. One K step of all S slices of both operands is a
-// pipeline stage in LDS, filled by global_load_lds_dwordx4 (global memory to
-// LDS without registers) one stage ahead of its use; a stage is waited for
-// with s_waitcnt vmcnt and a plain s_barrier, so the copies of the next stage
-// stay in flight while the matrix cores work on the current one. All S(S+1)/2
-// slice pairs of a stage are multiplied from LDS, with one int32 accumulator
-// set per level (32 S accumulator registers per lane). Other block shapes
-// (more wavefronts, stages, one workgroup per CU) are selectable for tuning.
+// Trailing update of the LU (the runtime-count kernel, ozaki_gemmd_kernel):
+// 512 threads = 2 x 4 wavefronts of 64 x 32 (two v_mfma_i32_32x32x32_i8
+// tiles each), block tile 128 x 128, K step 32, one workgroup per CU. One K
+// step of all S slices of both operands is a pipeline stage in LDS, filled by
+// global_load_lds_dwordx4 (global memory to LDS without registers) three
+// stages ahead of its use; a stage is waited for with s_waitcnt vmcnt and a
+// plain s_barrier, so the copies of the next stages stay in flight while the
+// matrix cores work on the current one. All S(S+1)/2 slice pairs of a stage
+// are multiplied from LDS, with one int32 accumulator set per level. For
+// slice counts up to 3 the products take the U-panel slices as the first
+// operand, so C's rows run along the lanes, and the tile's FP64 C values are
+// read into registers right after the first stages' copies: the read of C
+// overlaps the K loop and the write-back only stores. Other shapes (tuning
+// key 0) and the read of C after the K loop (key 5) are selectable.
+// Products with a fixed slice count (ozaki_gemm) use 128 x 64 tiles, two
+// workgroups per CU, C read after the K loop.
 #include "common.h"
 #include "lu.h"
 
@@ -553,7 +559,7 @@ __device__ __forceinline__ void ozp_issue(signed char* __restrict__ st, const si
 // are read from st_nxt. sched_barrier keeps the compiler from moving the
 // next step's reads back next to their uses or the barrier ahead of the
 // first pair.
-template <class Cfg, int S, int TA, int TB>
+template <class Cfg, int S, int TA, int TB, bool SWAP = false>
 __device__ __forceinline__ void ozp_step(signed char* __restrict__ st_cur,
                                          const signed char* __restrict__ st_nxt,
                                          v16i (&acc)[S][TA][TB], v4i (&fa)[S][TA],
@@ -569,7 +575,10 @@ __device__ __forceinline__ void ozp_step(signed char* __restrict__ st_cur,
 #pragma unroll
         for (int tb = 0; tb < TB; ++tb)
           acc[i + j][ta][tb] =
-              __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][ta], fb[j][tb], acc[i + j][ta][tb], 0, 0, 0);
+              SWAP ? __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[j][tb], fa[i][ta], acc[i + j][ta][tb],
+                                                           0, 0, 0)
+                   : __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][ta], fb[j][tb], acc[i + j][ta][tb],
+                                                           0, 0, 0);
       __builtin_amdgcn_sched_barrier(0);
       if (i == 0 && j == S - 1) {
         // copies of step k + 1 landed (those of the later steps stay in
@@ -600,7 +609,11 @@ __device__ __forceinline__ void ozp_step(signed char* __restrict__ st_cur,
 // at least Cfg::STAGE; one array per stage, so the compiler tells the stages
 // apart and waits only for the copies into the stage being read) belong to
 // the calling kernel, which may size them for more slices than S.
-template <int S, int WM, int WN, int TA, int TB, int NST, int OCC, int SB>
+// EPI 0: after the K loop, C is read, updated through an LDS transpose and
+// written (MFMA operands in the L-panel, U-panel order). EPI 1: operands
+// swapped (U-panel slices first, so C's rows run along the lanes) and C read
+// into registers at the start of the tile (see below).
+template <int S, int WM, int WN, int TA, int TB, int NST, int OCC, int SB, int EPI = 0>
 __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, int col0, int ncols,
                                          const signed char* __restrict__ As,
                                          const signed char* __restrict__ Bs,
@@ -643,7 +656,35 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
     ozp_issue<Cfg, S>(st2, Ab + 2 * kstepA, Bb + 2 * kstepB, slabA, slabB, tid);
     ozp_issue<Cfg, S>(st3, Ab + 3 * kstepA, Bb + 3 * kstepB, slabA, slabB, tid);
   }
-  __builtin_amdgcn_s_waitcnt(oz_vmcnt((NST - 1) * Cfg::COPIES));
+  // EPI 1: the tile's C values are read now, behind the first stages'
+  // copies, into the registers they are updated in (swapped operands: lane
+  // = row), so the FP64 read of C overlaps the whole K loop and the
+  // write-back only stores. The copies of stage 0 stay the oldest loads, so
+  // the first wait below still covers them.
+  constexpr bool PRE = EPI == 1;
+  double cpre[PRE ? TA : 1][PRE ? TB : 1][16];
+  if constexpr (PRE) {
+    const int rl = lane & 31, u = lane >> 5;
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta) {
+      const int i = bm + wm + 32 * ta + rl;
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb) {
+        const int jb = bn + wn + 32 * tb + 4 * u;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int jo = 8 * (r >> 2) + (r & 3);
+          cpre[ta][tb][r] = (i < m && jb + jo < col0 + ncols)
+                                ? __builtin_nontemporal_load(C + i + (int64_t)(jb + jo) * ldc)
+                                : 0.0;
+        }
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // this thread's copies of stage 0 landed: the loads issued after them
+  // (the later stages' copies and, for EPI 1, the C values) may be in flight
+  __builtin_amdgcn_s_waitcnt(oz_vmcnt((NST - 1) * Cfg::COPIES + (PRE ? TA * TB * 16 : 0)));
   __builtin_amdgcn_s_barrier();
   v4i fa[S][TA], fb[S][TB];
 #pragma unroll
@@ -661,23 +702,46 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
   // the loop body is the same every time
   for (int t = 0; t < nk; t += NST) {
     int k = min(t + NST, nk - 1);
-    ozp_step<Cfg, S, TA, TB>(st0, st1, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA, slabB,
+    ozp_step<Cfg, S, TA, TB, (EPI > 0)>(st0, st1, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA, slabB,
                              offA, offB, tid);
     k = min(t + NST + 1, nk - 1);
-    ozp_step<Cfg, S, TA, TB>(st1, NST > 2 ? st2 : st0, acc, fa, fb, Ab + k * kstepA,
+    ozp_step<Cfg, S, TA, TB, (EPI > 0)>(st1, NST > 2 ? st2 : st0, acc, fa, fb, Ab + k * kstepA,
                              Bb + k * kstepB, slabA, slabB, offA, offB, tid);
     if constexpr (NST > 2) {
       k = min(t + NST + 2, nk - 1);
-      ozp_step<Cfg, S, TA, TB>(st2, st3, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA,
+      ozp_step<Cfg, S, TA, TB, (EPI > 0)>(st2, st3, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA,
                                slabB, offA, offB, tid);
       k = min(t + NST + 3, nk - 1);
-      ozp_step<Cfg, S, TA, TB>(st3, st0, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA,
+      ozp_step<Cfg, S, TA, TB, (EPI > 0)>(st3, st0, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA,
                                slabB, offA, offB, tid);
     }
   }
   __builtin_amdgcn_s_waitcnt(oz_vmcnt(0) & OZ_LGKM0);
   __builtin_amdgcn_s_barrier();
 
+  if constexpr (PRE) {
+    const int rl = lane & 31, u = lane >> 5;
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta) {
+      const int i = bm + wm + 32 * ta + rl;
+      const int ei = ea[i];  // i < mp
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb) {
+        const int jb = bn + wn + 32 * tb + 4 * u;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int jo = 8 * (r >> 2) + (r & 3);
+          double c = 0.0;
+#pragma unroll
+          for (int l = 0; l < S; ++l) c += (double)acc[l][ta][tb][r] * oz_level_scale(l);
+          const double v = cpre[ta][tb][r] - ldexp(c, ei + eb[jb + jo]);  // jb + jo < np
+          if (i < m && jb + jo < col0 + ncols)
+            __builtin_nontemporal_store(v, C + i + (int64_t)(jb + jo) * ldc);
+        }
+      }
+    }
+    return;
+  }
   // Epilogue, one 32 x 32 MFMA tile (ta, tb) at a time: the next tile's C
   // values are loaded before this tile's stores; the levels are folded (FP64,
   // l ascending) in the MFMA layout, staged through LDS unscaled with the
@@ -748,28 +812,33 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmp_kernel(
 // The same with the slice count read on the device (*Sdev, 2 .. SMAX): the
 // slices are those of an SMAX-slice split (only the leading *Sdev digits
 // are used), so one launch serves whatever count the adaptive rule picked.
-template <int SMAX, int WM, int WN, int TA, int TB, int NST, int OCC>
+template <int SMAX, int WM, int WN, int TA, int TB, int NST, int OCC, int EPI = 0>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmd_kernel(
     int m, int n, int mp, int np, int Kp, int col0, int ncols, const signed char* __restrict__ As,
     const signed char* __restrict__ Bs, const int* __restrict__ ea, const int* __restrict__ eb,
     double* __restrict__ C, int64_t ldc, int group, const int* __restrict__ Sdev) {
-  constexpr int SB = OzPCfg<SMAX, WM, WN, TA, TB, NST, OCC>::STAGE;
+  typedef OzPCfg<SMAX, WM, WN, TA, TB, NST, OCC> Cfg;
+  // a stage array also stages the epilogue's FP64 tiles (EPI 0)
+  constexpr int SBE = (WM * WN + NST - 1) / NST * Cfg::EPI_WAVE;
+  constexpr int SB = Cfg::STAGE > SBE ? Cfg::STAGE : SBE;
+  static_assert(OCC * NST * SB <= 160 * 1024, "LDS");
   __shared__ __attribute__((aligned(16))) signed char st0[SB];
   __shared__ __attribute__((aligned(16))) signed char st1[SB];
   __shared__ __attribute__((aligned(16))) signed char st2[NST > 2 ? SB : 16];
   __shared__ __attribute__((aligned(16))) signed char st3[NST > 2 ? SB : 16];
   const int S = __builtin_amdgcn_readfirstlane(*Sdev);
+  // the C prefetch (EPI 1) holds 32 TA TB more registers: S <= 3 only
   if (S <= 2)
-    ozp_body<2, WM, WN, TA, TB, NST, OCC, SB>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
+    ozp_body<2, WM, WN, TA, TB, NST, OCC, SB, EPI>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
                                               ldc, group, st0, st1, st2, st3);
   else if (SMAX > 3 && S == 3)
-    ozp_body<3, WM, WN, TA, TB, NST, OCC, SB>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
+    ozp_body<3, WM, WN, TA, TB, NST, OCC, SB, EPI>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
                                               ldc, group, st0, st1, st2, st3);
   else if (SMAX > 4 && S == 4)
-    ozp_body<(SMAX > 4 ? 4 : SMAX), WM, WN, TA, TB, NST, OCC, SB>(
+    ozp_body<(SMAX > 4 ? 4 : SMAX), WM, WN, TA, TB, NST, OCC, SB, 0>(
         m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C, ldc, group, st0, st1, st2, st3);
   else
-    ozp_body<SMAX, WM, WN, TA, TB, NST, OCC, SB>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
+    ozp_body<SMAX, WM, WN, TA, TB, NST, OCC, SB, (SMAX > 3 ? 0 : EPI)>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
                                                  ldc, group, st0, st1, st2, st3);
 }
 
@@ -819,242 +888,6 @@ __global__ __launch_bounds__(1024) void oz_decide_kernel(const int* __restrict__
     }
     *s_out = S;
   }
-}
-
-// ---------------------------------------------------------------------------
-// Persistent variant (the default for S <= 5): one workgroup per CU walks a
-// strided sequence of block tiles (tile p, p + G, ... of the XCD-aware order,
-// G = grid size), and its stage pipeline runs on across tile boundaries: the
-// copies of the next tile's first steps are in flight while the current tile
-// is written back, so a tile pays no pipeline fill. The products are taken
-// with the operands swapped (U-panel slices as the MFMA's first operand),
-// which puts C's rows along the lanes: for accumulator register r, lanes 0-31
-// hold 32 consecutive rows of one column and lanes 32-63 of another, so the
-// FP64 read-modify-write of C is coalesced straight from the accumulators
-// (no LDS transpose) and the epilogue leaves LDS to the pipeline.
-//   lane l, register r of MFMA tile (ta, tb): row bm + wm + 32 ta + l % 32,
-//   column bn + wn + 32 tb + 8 (r / 4) + 4 (l / 32) + r % 4.
-// READ_NEXT = false for a tile's last step: the next tile's first fragments
-// are read after the write-back, so they hold no registers during it.
-template <class Cfg, int S, int TA, int TB, bool READ_NEXT = true>
-__device__ __forceinline__ void ozq_step(signed char* __restrict__ st_cur,
-                                         const signed char* __restrict__ st_nxt,
-                                         v16i (&acc)[S][TA][TB], v4i (&fa)[S][TA],
-                                         v4i (&fb)[S][TB], const signed char* a,
-                                         const signed char* b, size_t slabA, size_t slabB,
-                                         int offA, int offB, int tid) {
-#pragma unroll
-  for (int i = 0; i < S; ++i) {
-#pragma unroll
-    for (int j = S - 1 - i; j >= 0; --j) {
-#pragma unroll
-      for (int ta = 0; ta < TA; ++ta)
-#pragma unroll
-        for (int tb = 0; tb < TB; ++tb)
-          acc[i + j][ta][tb] =
-              __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[j][tb], fa[i][ta], acc[i + j][ta][tb], 0, 0, 0);
-      __builtin_amdgcn_sched_barrier(0);
-      if (i == 0 && j == S - 1) {
-        __builtin_amdgcn_s_waitcnt(oz_vmcnt((Cfg::NST - 2) * Cfg::COPIES) & OZ_LGKM0);
-        __builtin_amdgcn_s_barrier();
-        ozp_issue<Cfg, S>(st_cur, a, b, slabA, slabB, tid);
-      }
-      if (!READ_NEXT) {
-        __builtin_amdgcn_sched_barrier(0);
-        continue;
-      }
-      if (j == S - 1 - i) {
-#pragma unroll
-        for (int tb = 0; tb < TB; ++tb)
-          fb[j][tb] = *reinterpret_cast<const v4i*>(st_nxt + Cfg::ABYTES + j * Cfg::BN * OZ_BK +
-                                                    offB + tb * 32 * 16);
-      }
-      if (j == 0) {
-#pragma unroll
-        for (int ta = 0; ta < TA; ++ta)
-          fa[i][ta] = *reinterpret_cast<const v4i*>(st_nxt + i * Cfg::BM * OZ_BK + offA +
-                                                    ta * 32 * 16);
-      }
-      __builtin_amdgcn_sched_barrier(0);
-    }
-  }
-}
-
-// Kp: a multiple of 4 K steps (128 bytes). Launch with at most tm * tn
-// workgroups (tiles_launch rounding is not needed: every workgroup id below
-// the grid owns at least one tile when grid <= tiles).
-//
-// Every workgroup's tiles take the same time, so workgroups started together
-// stay in step: all multiply, then all write back at once (matrix cores idle
-// while HBM saturates). `stagger` (units of 8128 cycles of s_sleep) delays
-// workgroup b by ((b / 8) mod 4) units before its first copies, spreading the
-// write-backs over a tile's duration.
-template <int S, int WM, int WN, int TA, int TB, int NST, int OCC>
-__global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmq_kernel(
-    int m, int n, int mp, int np, int Kp, int col0, int ncols, const signed char* __restrict__ As,
-    const signed char* __restrict__ Bs, const int* __restrict__ ea, const int* __restrict__ eb,
-    double* __restrict__ C, int64_t ldc, int group, int stagger) {
-  typedef OzPCfg<S, WM, WN, TA, TB, NST, OCC> Cfg;
-  __shared__ __attribute__((aligned(16))) signed char st0[Cfg::STAGE];
-  __shared__ __attribute__((aligned(16))) signed char st1[Cfg::STAGE];
-  __shared__ __attribute__((aligned(16))) signed char st2[NST > 2 ? Cfg::STAGE : 16];
-  __shared__ __attribute__((aligned(16))) signed char st3[NST > 2 ? Cfg::STAGE : 16];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave % WM) * 32 * TA, wn = (wave / WM) * 32 * TB;
-  const int tm = mp / Cfg::BM, tn = (ncols + Cfg::BN - 1) / Cfg::BN;
-  const int G = gridDim.x;
-  // tiles of this workgroup: p = blockIdx.x + s G for s = 0 .. nmine - 1 (ids
-  // past the XCD-aware order's surplus are skipped by tile_of)
-  const int T = (tm * tn + NXCD - 1) / NXCD * NXCD;  // ids of the XCD-aware order
-  const size_t slabA = (size_t)mp * Kp, slabB = (size_t)np * Kp;
-  const size_t kstepA = (size_t)mp * OZ_BK, kstepB = (size_t)np * OZ_BK;
-  const int r32 = lane & 31, hf = lane >> 5;
-  const int offA = hf * Cfg::BM * 16 + (wm + r32) * 16;
-  const int offB = hf * Cfg::BN * 16 + (wn + r32) * 16;
-  const int nk = Kp / OZ_BK;  // a multiple of 4
-
-  // first tile with work
-  int p = blockIdx.x, ti = 0, tj = 0;
-  while (p < T && !tile_of(p, tm, tn, ti, tj, group)) p += G;
-  if (p >= T) return;
-  // next tile with work (or none)
-  auto next_tile = [&](int from, int& nti, int& ntj) {
-    int q = from + G;
-    while (q < T && !tile_of(q, tm, tn, nti, ntj, group)) q += G;
-    return q;
-  };
-  int nti = ti, ntj = tj;
-  int pn = next_tile(p, nti, ntj);
-  const signed char* Ab = As + (size_t)(ti * Cfg::BM) * OZ_BK;
-  const signed char* Bb = Bs + (size_t)(col0 + tj * Cfg::BN) * OZ_BK;
-  // source of the copies past the tile's last step: the next tile's first
-  // steps, or (last tile) the last step again (never read)
-  const signed char* An = pn < T ? As + (size_t)(nti * Cfg::BM) * OZ_BK : Ab + (nk - 1) * kstepA;
-  const signed char* Bn = pn < T ? Bs + (size_t)(col0 + ntj * Cfg::BN) * OZ_BK : Bb + (nk - 1) * kstepB;
-  size_t nstepA = pn < T ? kstepA : 0, nstepB = pn < T ? kstepB : 0;
-
-  v16i acc[S][TA][TB];
-#pragma unroll
-  for (int l = 0; l < S; ++l)
-#pragma unroll
-    for (int ta = 0; ta < TA; ++ta)
-#pragma unroll
-      for (int tb = 0; tb < TB; ++tb) acc[l][ta][tb] = v16i(0);
-
-  for (int z = ((blockIdx.x >> 3) & 3) * stagger; z > 0; --z) __builtin_amdgcn_s_sleep(127);
-  ozp_issue<Cfg, S>(st0, Ab, Bb, slabA, slabB, tid);
-  ozp_issue<Cfg, S>(st1, Ab + kstepA, Bb + kstepB, slabA, slabB, tid);
-  if constexpr (NST > 2) {
-    ozp_issue<Cfg, S>(st2, Ab + 2 * kstepA, Bb + 2 * kstepB, slabA, slabB, tid);
-    ozp_issue<Cfg, S>(st3, Ab + 3 * kstepA, Bb + 3 * kstepB, slabA, slabB, tid);
-  }
-  __builtin_amdgcn_s_waitcnt(oz_vmcnt((NST - 1) * Cfg::COPIES));
-  __builtin_amdgcn_s_barrier();
-  v4i fa[S][TA], fb[S][TB];
-  const int rl = lane & 31, u = lane >> 5;
-  for (;;) {
-    // first fragments of the tile (stage 0 holds its step 0: landed and
-    // visible since the last step's wait and barrier, or the prologue's)
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-#pragma unroll
-      for (int ta = 0; ta < TA; ++ta)
-        fa[q][ta] = *reinterpret_cast<const v4i*>(st0 + q * Cfg::BM * OZ_BK + offA + ta * 32 * 16);
-#pragma unroll
-      for (int tb = 0; tb < TB; ++tb)
-        fb[q][tb] = *reinterpret_cast<const v4i*>(st0 + Cfg::ABYTES + q * Cfg::BN * OZ_BK + offB +
-                                                  tb * 32 * 16);
-    }
-    // source of the copy of step k (k may run past the tile into the next one)
-    auto srcA = [&](int k) { return k < nk ? Ab + k * kstepA : An + (k - nk) * nstepA; };
-    auto srcB = [&](int k) { return k < nk ? Bb + k * kstepB : Bn + (k - nk) * nstepB; };
-    int t = 0;
-    for (; t + NST < nk; t += NST) {
-      ozq_step<Cfg, S, TA, TB>(st0, st1, acc, fa, fb, srcA(t + NST), srcB(t + NST), slabA, slabB,
-                               offA, offB, tid);
-      ozq_step<Cfg, S, TA, TB>(st1, NST > 2 ? st2 : st0, acc, fa, fb, srcA(t + NST + 1),
-                               srcB(t + NST + 1), slabA, slabB, offA, offB, tid);
-      if constexpr (NST > 2) {
-        ozq_step<Cfg, S, TA, TB>(st2, st3, acc, fa, fb, srcA(t + NST + 2), srcB(t + NST + 2),
-                                 slabA, slabB, offA, offB, tid);
-        ozq_step<Cfg, S, TA, TB>(st3, st0, acc, fa, fb, srcA(t + NST + 3), srcB(t + NST + 3),
-                                 slabA, slabB, offA, offB, tid);
-      }
-    }
-    // the tile's last NST steps; the very last reads no next fragments
-    if constexpr (NST > 2) {
-      ozq_step<Cfg, S, TA, TB>(st0, st1, acc, fa, fb, srcA(t + NST), srcB(t + NST), slabA, slabB,
-                               offA, offB, tid);
-      ozq_step<Cfg, S, TA, TB>(st1, st2, acc, fa, fb, srcA(t + NST + 1), srcB(t + NST + 1), slabA,
-                               slabB, offA, offB, tid);
-      ozq_step<Cfg, S, TA, TB>(st2, st3, acc, fa, fb, srcA(t + NST + 2), srcB(t + NST + 2), slabA,
-                               slabB, offA, offB, tid);
-      ozq_step<Cfg, S, TA, TB, false>(st3, st0, acc, fa, fb, srcA(t + NST + 3), srcB(t + NST + 3),
-                                      slabA, slabB, offA, offB, tid);
-    } else {
-      ozq_step<Cfg, S, TA, TB>(st0, st1, acc, fa, fb, srcA(t + NST), srcB(t + NST), slabA, slabB,
-                               offA, offB, tid);
-      ozq_step<Cfg, S, TA, TB, false>(st1, st0, acc, fa, fb, srcA(t + NST + 1), srcB(t + NST + 1),
-                                      slabA, slabB, offA, offB, tid);
-    }
-    // write back this tile while the next tile's first copies are in flight;
-    // interior tiles (the uniform common case) skip the bounds checks
-    const int bm = ti * Cfg::BM, bn = col0 + tj * Cfg::BN;
-    const bool inside = bm + Cfg::BM <= m && bn + Cfg::BN <= col0 + ncols;
-#pragma unroll
-    for (int ta = 0; ta < TA; ++ta) {
-      const int i = bm + wm + 32 * ta + rl;
-      const int ei = ea[i];  // i < mp
-#pragma unroll
-      for (int tb = 0; tb < TB; ++tb) {
-        const int jb = bn + wn + 32 * tb + 4 * u;
-        double* Ct = C + i + (int64_t)jb * ldc;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {  // registers 4 q .. 4 q + 3: columns jb + 8 q + 0..3
-          double cv[4];
-          int ej[4];
-#pragma unroll
-          for (int rr = 0; rr < 4; ++rr) {
-            const int j = jb + 8 * q + rr;
-            ej[rr] = eb[j];  // j < np
-            // C streams through once per update: non-temporal accesses keep
-            // it from evicting the slices the other tiles still read from L2
-            if (inside)
-              cv[rr] = __builtin_nontemporal_load(Ct + (int64_t)(8 * q + rr) * ldc);
-            else
-              cv[rr] = (i < m && j < col0 + ncols)
-                           ? __builtin_nontemporal_load(Ct + (int64_t)(8 * q + rr) * ldc)
-                           : 0.0;
-          }
-#pragma unroll
-          for (int rr = 0; rr < 4; ++rr) {
-            const int j = jb + 8 * q + rr;
-            double c = 0.0;
-#pragma unroll
-            for (int l = 0; l < S; ++l) c += (double)acc[l][ta][tb][4 * q + rr] * oz_level_scale(l);
-            const double v = cv[rr] - ldexp(c, ei + ej[rr]);
-            if (inside || (i < m && j < col0 + ncols))
-              __builtin_nontemporal_store(v, Ct + (int64_t)(8 * q + rr) * ldc);
-          }
-        }
-#pragma unroll
-        for (int l = 0; l < S; ++l) acc[l][ta][tb] = v16i(0);
-      }
-    }
-    if (pn >= T) break;
-    p = pn;
-    ti = nti;
-    tj = ntj;
-    Ab = An;
-    Bb = Bn;
-    pn = next_tile(p, nti, ntj);
-    An = pn < T ? As + (size_t)(nti * Cfg::BM) * OZ_BK : Ab + (nk - 1) * kstepA;
-    Bn = pn < T ? Bs + (size_t)(col0 + ntj * Cfg::BN) * OZ_BK : Bb + (nk - 1) * kstepB;
-    nstepA = pn < T ? kstepA : 0;
-    nstepB = pn < T ? kstepB : 0;
-  }
-  // drain the copies past the last tile (never read) before the workgroup ends
-  __builtin_amdgcn_s_waitcnt(oz_vmcnt(0) & OZ_LGKM0);
 }
 
 // Slice arrays: rows padded to a multiple of 128; columns to a multiple of 128
@@ -1136,91 +969,45 @@ static int ozaki_launchp(hipStream_t s, const OzLayout& L, int m, int n, int col
   return OK;
 }
 
-static int oz_cus() {
-  static int ncu = 0;
-  if (ncu == 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        ncu <= 0)
-      ncu = 256;
-  }
-  return ncu;
-}
-
-template <int S, int WM, int WN, int TA, int TB, int NST, int OCC>
-static int ozaki_launchq(hipStream_t s, const OzLayout& L, int m, int n, int col0, int ncols,
-                         double* C, int64_t ldc) {
-  typedef OzPCfg<S, WM, WN, TA, TB, NST, OCC> Cfg;
-  const int tm = L.mp / Cfg::BM, tn = (ncols + Cfg::BN - 1) / Cfg::BN;
-  const int tiles = tm * tn;
-  // a multiple of the XCD count, so each workgroup stays on one XCD's tiles
-  int grid = oz_cus() * OCC;
-  if (grid > tiles_launch(tm, tn)) grid = tiles_launch(tm, tn);
-  const int g = tuning_get(TUNE_OZAKI_GROUP);
-  // stagger only when every workgroup has several tiles to amortise it over
-  const int stagger = tiles >= 4 * grid ? tuning_get(TUNE_OZAKI_STAGGER) : 0;
-  ozaki_gemmq_kernel<S, WM, WN, TA, TB, NST, OCC><<<grid, Cfg::NT, 0, s>>>(
-      m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, g > 0 ? g : GROUP_M,
-      stagger);
-  MPDOK_HIP(hipGetLastError());
-  return OK;
-}
-
 template <int S>
 static int ozaki_apply_s(hipStream_t s, int m, int n, int K, int col0, int ncols, double* C,
                          int64_t ldc, void* work) {
   const OzLayout L = oz_layout(S, m, n, K, work);
-  // shape 0 (default): the single-barrier pipelined kernel with 128 x 64
-  // block tiles (4 wavefronts of 64 x 32), two stages, two workgroups per CU,
-  // so that one workgroup's epilogue (the FP64 read-modify-write of C)
-  // overlaps the other's matrix-core work; S = 6, 7 (too many accumulators
-  // for it) use the two-barrier kernel with two workgroups per CU (S = 6) or
-  // one (S = 7). Shapes 1-5 select the two-barrier kernel, 6-9 other
-  // pipelined shapes, for tuning and tests.
-  switch (tuning_get(TUNE_OZAKI_SHAPE)) {
-    case 1: return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 1 per CU
-    case 2: return ozaki_launch<S, 2, 4, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 128 x 128
-    case 3: return ozaki_launch<S, 2, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);  // four stages
-    case 4: return ozaki_launch<S, 2, 4, 4, 1>(s, L, m, n, col0, ncols, C, ldc);  // 128 x 128, four stages
-    case 5:
-      if constexpr (S <= 6) return ozaki_launch<S, 2, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 6:  // pipelined, 8 wavefronts of 64 x 32, four stages, one workgroup per CU
-      if constexpr (S <= 5) return ozaki_launchp<S, 2, 4, 2, 1, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 7:  // pipelined, 8 wavefronts of 32 x 64, four stages, one workgroup per CU
-      if constexpr (S <= 5) return ozaki_launchp<S, 4, 2, 1, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 8:  // pipelined, 4 wavefronts of 64 x 64 (S <= 4), four stages, one workgroup per CU
-      if constexpr (S <= 4) return ozaki_launchp<S, 2, 2, 2, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 9:  // pipelined, 4 wavefronts of 32 x 64, two stages, two workgroups per CU
-      if constexpr (S <= 5) return ozaki_launchp<S, 2, 2, 1, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 10:  // persistent, 8 wavefronts of 64 x 32, four stages
-      if constexpr (S <= 5) return ozaki_launchq<S, 2, 4, 2, 1, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 11:  // persistent, 4 wavefronts of 64 x 32, two stages, two workgroups per CU
-      if constexpr (S <= 5) return ozaki_launchq<S, 2, 2, 2, 1, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 13:  // pipelined, four stages, two workgroups per CU (S <= 3)
-      if constexpr (S <= 3) return ozaki_launchp<S, 2, 2, 2, 1, 4, 2>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 14:  // pipelined, two stages, three workgroups per CU (S <= 3)
-      if constexpr (S <= 3) return ozaki_launchp<S, 2, 2, 2, 1, 2, 3>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 15:  // pipelined, 4 wavefronts of 64 x 64, two stages, two workgroups per CU (S <= 3)
-      if constexpr (S <= 3) return ozaki_launchp<S, 2, 2, 2, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    case 12:  // persistent, 4 wavefronts of 64 x 64 (S <= 4), four stages
-      if constexpr (S <= 4) return ozaki_launchq<S, 2, 2, 2, 2, 4, 1>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);
-    default:
-      if constexpr (S <= 5) return ozaki_launchp<S, 2, 2, 2, 1, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
-      else if constexpr (S == 6) return ozaki_launch<S, 2, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
-      else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 7 S accumulators
+  // the single-barrier pipelined kernel with 128 x 64 block tiles (4
+  // wavefronts of 64 x 32), two stages, two workgroups per CU; S = 6, 7 (too
+  // many accumulators for it) use the two-barrier kernel with two workgroups
+  // per CU (S = 6) or one (S = 7)
+  if constexpr (S <= 5) return ozaki_launchp<S, 2, 2, 2, 1, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
+  else if constexpr (S == 6) return ozaki_launch<S, 2, 2, 2, 2>(s, L, m, n, col0, ncols, C, ldc);
+  else return ozaki_launch<S, 2, 2, 2, 1>(s, L, m, n, col0, ncols, C, ldc);  // 7 S accumulators
+}
+
+template <int SMAX, int WM, int WN, int TA, int TB, int NST, int OCC, int EPI>
+static int oz_dyn_launch(hipStream_t s, const OzLayout& L, int m, int n, int col0, int ncols,
+                         double* C, int64_t ldc, int group, const int* Sdev) {
+  typedef OzPCfg<SMAX, WM, WN, TA, TB, NST, OCC> Cfg;
+  const int grid = tiles_launch((m + Cfg::BM - 1) / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN);
+  ozaki_gemmd_kernel<SMAX, WM, WN, TA, TB, NST, OCC, EPI><<<grid, Cfg::NT, 0, s>>>(
+      m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, group, Sdev);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+// sel = (smax - 3) * 2 + epi
+template <int WM, int WN, int TA, int TB, int NST, int OCC>
+static int oz_dyn_pick(hipStream_t s, int sel, const OzLayout& L, int m, int n, int col0, int ncols,
+                       double* C, int64_t ldc, int group, const int* Sdev) {
+#define OZ_DYN(SM, EP) \
+  oz_dyn_launch<SM, WM, WN, TA, TB, NST, OCC, EP>(s, L, m, n, col0, ncols, C, ldc, group, Sdev)
+  switch (sel) {
+    case 0: return OZ_DYN(3, 0);
+    case 1: return OZ_DYN(3, 1);
+    case 2: return OZ_DYN(4, 0);
+    case 3: return OZ_DYN(4, 1);
+    case 5: return OZ_DYN(5, 1);
+    default: return OZ_DYN(5, 0);
   }
+#undef OZ_DYN
 }
 
 // C[:, col0 : col0 + ncols] -= the product of the leading *Sdev slices of the
@@ -1240,23 +1027,21 @@ int ozaki_apply_dyn(hipStream_t s, int smax, int m, int n, int K, int row0, int 
   m = nrows;
   const int g = tuning_get(TUNE_OZAKI_GROUP);
   const int group = g > 0 ? g : GROUP_M;
-  // 128 x 64 tiles, 4 wavefronts of 64 x 32, two stages, two workgroups per CU
-  const int grid = tiles_launch((m + 127) / 128, (ncols + 63) / 64);
-  switch (smax) {
-    case 3:
-      ozaki_gemmd_kernel<3, 2, 2, 2, 1, 2, 2><<<grid, 256, 0, s>>>(
-          m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, group, Sdev);
-      break;
-    case 4:
-      ozaki_gemmd_kernel<4, 2, 2, 2, 1, 2, 2><<<grid, 256, 0, s>>>(
-          m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, group, Sdev);
-      break;
-    default:
-      ozaki_gemmd_kernel<5, 2, 2, 2, 1, 2, 2><<<grid, 256, 0, s>>>(
-          m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, group, Sdev);
+  // Block shapes (tuning key 0): 0 (default) 128 x 128 tiles, 8 wavefronts
+  // of 64 x 32, four stages, one workgroup per CU; 1: 128 x 64 tiles, 4
+  // wavefronts, two stages, two workgroups per CU; 2: 128 x 128, two stages;
+  // 3: 256 x 64, 8 wavefronts, two stages. C handling (key 5): 0 (default)
+  // C read into registers at the start of each tile (S <= 3; larger counts
+  // read it after the K loop), 1: read after the K loop.
+  const int shape = tuning_get(TUNE_OZAKI_SHAPE);
+  const int epi = tuning_get(TUNE_OZAKI_EPI) == 0 ? 1 : 0;
+  const int sel = (smax - 3) * 2 + epi;  // 0 .. 5
+  switch (shape) {
+    case 1: return oz_dyn_pick<2, 2, 2, 1, 2, 2>(s, sel, L, m, n, col0, ncols, C, ldc, group, Sdev);
+    case 2: return oz_dyn_pick<2, 4, 2, 1, 2, 1>(s, sel, L, m, n, col0, ncols, C, ldc, group, Sdev);
+    case 3: return oz_dyn_pick<4, 2, 2, 1, 2, 1>(s, sel, L, m, n, col0, ncols, C, ldc, group, Sdev);
+    default: return oz_dyn_pick<2, 4, 2, 1, 4, 1>(s, sel, L, m, n, col0, ncols, C, ldc, group, Sdev);
   }
-  MPDOK_HIP(hipGetLastError());
-  return OK;
 }
 
 // Slice count for the product of the last ozaki_slice call (smax slices,

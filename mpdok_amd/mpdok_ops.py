@@ -475,17 +475,17 @@ class MPDOKSolver:
     def solve(self, A, b, tol=1e-11, maxiter_outer=5, restart=50, precond=None,
               factor_precision="ozaki", slices=5, pivot="none", body="fp64", residual="dot2",
               inner_tol=1e-6, nb=512, reuse_factor=False, adaptive_slices=True,
-              slice_margin=-4):
+              slice_margin=-8):
         """Solve A x = b by GMRES-IR; returns x (FP64 device tensor (N,)).
 
         slice_margin (int8-sliced LU with adaptive slice counts): the `delta`
         of mpdok_lu_set_adaptive. A later trailing update takes the fewest
         slices whose error bound stays within the first update's times
-        2^(-slice_margin): -4 lets each later update be up to 16 times less
+        2^(-slice_margin): -8 lets each later update be up to 256 times less
         accurate than the first, for fewer digit products; GMRES-IR absorbs it
-        with about one more inner iteration (measured on the N = 32768 Gaussian
-        RBF system: 6 inner iterations instead of 5 at -2, 194.5 ms instead of
-        196.3)."""
+        with one more inner iteration (measured on the N = 32768 Gaussian RBF
+        system: 7 inner iterations instead of 6 at -4, 178.7 ms instead of
+        186.6)."""
         try:
             if precond is None:
                 return self._solve(A, b, tol, maxiter_outer, restart)
@@ -544,7 +544,7 @@ class MPDOKSolver:
 
     def _solve_precond(self, A, b, tol, maxiter_outer, restart, precond, factor_precision,
                        slices, pivot, body, residual, inner_tol, nb, reuse_factor,
-                       adaptive_slices=True, slice_margin=-4):
+                       adaptive_slices=True, slice_margin=-8):
         if precond != "lu":
             raise ValueError(f"precond must be None or 'lu', got {precond!r}")
         A_gpu, b_gpu, N = self._prep(A, b)

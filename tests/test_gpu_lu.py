@@ -91,27 +91,28 @@ def test_ozaki_gemm_bit_exact_vs_oracle(L, S, m, n, K):
     assert np.array_equal(Cd.cpu().numpy(), ref)
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14])
-@pytest.mark.parametrize("S,m,n,K", [(5, 257, 129, 300), (6, 640, 384, 512), (3, 130, 260, 96),
-                                     (4, 300, 520, 640)])
-def test_ozaki_gemm_block_shapes_bit_exact(L, shape, S, m, n, K):
-    """Every block shape of the int8-sliced GEMM (tuning key 0) gives the
-    oracle's bits: shapes only change tiling, pipelining and occupancy."""
-    rng = np.random.default_rng(shape * 7 + S)
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("epi", [0, 1])
+@pytest.mark.parametrize("smax,m,n,K", [(5, 257, 129, 300), (3, 130, 260, 96), (4, 300, 520, 640),
+                                        (5, 640, 384, 1024)])
+def test_trailing_update_shapes_and_c_handling_bit_exact(L, shape, epi, smax, m, n, K):
+    """Every block shape (tuning key 0) and both ways of handling C (key 5:
+    read into registers at the start of a tile, or after its K loop) give the
+    oracle's bits for every slice count: they change tiling, pipelining,
+    operand order and when C is read, never the sums."""
+    rng = np.random.default_rng(shape * 7 + smax + 31 * epi)
     A = rng.standard_normal((m, K)) * np.exp(rng.uniform(-3, 3, (m, 1)))
     B = rng.standard_normal((K, n)) * np.exp(rng.uniform(-3, 3, (1, n)))
     C = rng.standard_normal((m, n))
-    ws = torch.empty(L.mpdok_ozaki_workspace_bytes(m, n, K, S), dtype=torch.uint8, device="cuda")
-    Ad, Bd, Cd = dev(A), dev(B), dev(C)
-    assert L.mpdok_set_tuning(0, shape) == 0
+    assert L.mpdok_set_tuning(0, shape) == 0 and L.mpdok_set_tuning(5, epi) == 0
     try:
-        st = L.mpdok_ozaki_gemm_sub(stream(), S, m, n, K, Ad.data_ptr(), m, Bd.data_ptr(), K,
-                                    Cd.data_ptr(), m, ws.data_ptr())
-        torch.cuda.synchronize()
+        for S in range(2, smax + 1):
+            out, used, _ = _ozaki_adaptive(L, A, B, C, smax, s_fixed=S)
+            assert used == S
+            assert np.array_equal(out, C - olu.ozaki_matmul_lead(A, B, smax, S)), (S, shape, epi)
     finally:
         L.mpdok_set_tuning(0, 0)
-    assert st == 0, L.mpdok_last_error()
-    assert np.array_equal(Cd.cpu().numpy(), C - olu.ozaki_matmul(A, B, S))
+        L.mpdok_set_tuning(5, 0)
 
 
 @pytest.mark.parametrize("variant", [0, 1])
