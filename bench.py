@@ -55,23 +55,124 @@ def measured_traffic(N, slices, nb):
     return None
 
 
-def cpu_baseline(N_full, n_sample):
-    """SciPy FP64 LAPACK solve (getrf/getrs, the reference's CPU comparator) on
-    the first n_sample points of the same workload, extrapolated by N^3."""
+def _cores():
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else n
+
+
+def cpu_baseline(N_full, n_sample, A_dev=None, b_full=None, gmres_iters=20):
+    """The reference's CPU path (SciPy FP64) on a bounded sample of the same
+    workload: LAPACK getrf/getrs (scipy.linalg.solve, rbf_demo.ipynb:159) on
+    the first n_sample points, extrapolated by (N / n_sample)^3 to the full
+    system; and, on the full N x N matrix (copied from the GPU, where it was
+    assembled), SciPy GMRES(restart) for a fixed iteration budget
+    (dense_krylov.py:617,689 call it with restart=50, maxiter=N), reporting the
+    time per iteration and the relres reached."""
     import scipy.linalg as sla
+    import scipy.sparse.linalg as spla
     from oracle import rbf as orb
-    X = np.random.default_rng(42).uniform(0.0, 1.0, (N_full, 3))[:n_sample]
-    A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
-    b = rhs_of(np.random.default_rng(42).uniform(0.0, 1.0, (N_full, 3)))[:n_sample]
+    X = np.random.default_rng(42).uniform(0.0, 1.0, (N_full, 3))
+    b_all = rhs_of(X)
+    Xs = X[:n_sample]
+    A = orb.build_kernel(Xs, "gaussian", 25.0, 1e-6)
+    b = b_all[:n_sample]
     t0 = time.perf_counter()
     x = sla.solve(A, b, check_finite=False)
     t = time.perf_counter() - t0
     rel = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
-    cores = len(os.sched_getaffinity(0))
-    return {"value": t * (N_full / n_sample) ** 3, "unit": "s", "cores": cores, "kind": "port",
-            "sample": (f"scipy.linalg.solve FP64 (LAPACK getrf/getrs) on the first {n_sample} "
-                       f"points: {t:.2f} s measured, relres {rel:.1e} (FP64 residual), value "
-                       f"extrapolated x(N/{n_sample})^3 to N={N_full}")}
+    del A
+    out = {"value": t * (N_full / n_sample) ** 3, "unit": "s", "cores": _cores(), "kind": "port",
+           "sample": (f"scipy.linalg.solve FP64 (LAPACK getrf/getrs) on the first {n_sample} of the "
+                      f"{N_full} points: {t:.2f} s measured, relres {rel:.1e} (FP64 residual); value "
+                      f"= that time x (N/{n_sample})^3")}
+    if A_dev is not None and gmres_iters > 0:
+        A_h = A_dev.cpu().numpy()                      # column-major N x N, FP64
+        it = [0]
+
+        def cb(_):
+            it[0] += 1
+        t0 = time.perf_counter()
+        xg, _ = spla.gmres(A_h, b_all, rtol=1e-12, restart=gmres_iters, maxiter=1,
+                           callback=cb, callback_type="pr_norm")
+        tg = time.perf_counter() - t0
+        relg = float(np.linalg.norm(b_all - A_h @ xg) / np.linalg.norm(b_all))
+        out["gmres_full_n"] = {"iterations": it[0], "s": tg, "s_per_iteration": tg / max(it[0], 1),
+                               "relres": relg, "restart": gmres_iters,
+                               "note": "scipy.sparse.linalg.gmres FP64 on the full matrix, one "
+                                       "restart cycle (capped)"}
+        del A_h
+    return out
+
+
+def cpu_full(N):
+    """scipy.linalg.solve on the full N-point system (minutes of CPU)."""
+    import scipy.linalg as sla
+    from oracle import rbf as orb
+    X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
+    A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
+    b = rhs_of(X)
+    log(f"[bench] CPU full solve N={N} on {_cores()} cores ...")
+    t0 = time.perf_counter()
+    x = sla.solve(A, b, check_finite=False, overwrite_a=False)
+    t = time.perf_counter() - t0
+    rel = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+    return {"s": t, "relres_fp64": rel, "cores": _cores(), "N": N}
+
+
+def gemv_profile(N, reps=10):
+    """HIP-event times of the bandwidth-bound kernels on the N-point headline
+    matrix: FP32 GEMV (row- and column-major A, 4 N^2 B), FP64 GEMV and the
+    FP64 / Dot2 residual (column-major, 8 N^2 B), and the read-only ceiling
+    (mpdok_read_ceiling) over the same buffers."""
+    import torch
+    from mpdok_amd import _dev, _ops, _lib
+    L = _lib.load()
+    X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
+    A64 = _dev.empty((N, N), torch.float64, "F")
+    _ops.rbf_assemble(_dev.as_device(X, torch.float64), 0, 25.0, 1e-6, A64)
+    A32c = _ops.cast_matrix(A64, torch.float32, "F")
+    A32r = _ops.cast_matrix(A64, torch.float32, "C")
+    x64 = torch.rand(N, dtype=torch.float64, device=A64.device)
+    x32 = x64.float()
+    b64 = torch.rand(N, dtype=torch.float64, device=A64.device)
+    ws = torch.empty(L.mpdok_read_ceiling_workspace_bytes(), dtype=torch.uint8, device=A64.device)
+    out = torch.empty(1, dtype=torch.float64, device=A64.device)
+
+    def ceiling(t):
+        st = L.mpdok_read_ceiling(_dev.stream(), t.data_ptr(), t.numel() * t.element_size(),
+                                  out.data_ptr(), ws.data_ptr())
+        assert st == 0
+
+    cases = [("read_ceiling_f64", lambda: ceiling(A64), 8.0 * N * N),
+             ("read_ceiling_f32", lambda: ceiling(A32r), 4.0 * N * N),
+             ("gemv_f32_row_major", lambda: _ops.matvec(A32r, x32), 4.0 * N * N),
+             ("gemv_f32_col_major", lambda: _ops.matvec(A32c, x32), 4.0 * N * N),
+             ("gemv_f64_col_major", lambda: _ops.matvec(A64, x64), 8.0 * N * N),
+             ("residual_f64_col_major", lambda: _ops.residual(A64, x64, b64, mode="fp64"),
+              8.0 * N * N),
+             ("residual_dot2_col_major", lambda: _ops.residual(A64, x64, b64, mode="dot2"),
+              8.0 * N * N)]
+    res = {}
+    for name, fn, byts in cases:
+        fn()
+        fn()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+        torch.cuda.synchronize()
+        for r in range(reps):
+            ev[2 * r].record()
+            fn()
+            ev[2 * r + 1].record()
+        torch.cuda.synchronize()
+        ms = min(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps))
+        gbs = byts / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "GBps": gbs, "frac_of_peak": gbs / PEAK_HBM_GBS, "bytes": byts}
+    for name in res:
+        if not name.startswith("read_ceiling"):
+            c = res["read_ceiling_f32" if "f32" in name else "read_ceiling_f64"]["GBps"]
+            res[name]["frac_of_read_ceiling"] = res[name]["GBps"] / c
+    return {"metric": "bandwidth-bound kernels at N=%d" % N, "N": N, "unit": "GB/s",
+            "peak_GBps": PEAK_HBM_GBS, "kernels": res}
 
 
 def main():
@@ -83,7 +184,14 @@ def main():
     ap.add_argument("--slices", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--nb", type=int, default=512)
-    ap.add_argument("--cpu-sample", type=int, default=12288)
+    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--cpu-gmres-iters", type=int, default=20,
+                    help="SciPy GMRES iterations on the full matrix in the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time scipy.linalg.solve on the full system (minutes of CPU)")
+    ap.add_argument("--gemv-profile", action="store_true",
+                    help="only measure the GEMV / residual kernels and the read-only ceiling "
+                         "at N (HIP events) and print them as one JSON line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip torch solve and plain GMRES")
     ap.add_argument("--configs", action="store_true",
@@ -101,6 +209,9 @@ def main():
 
     N, tol = args.n, args.tol
     dev = _dev.device()
+    if args.gemv_profile:
+        print(json.dumps(gemv_profile(N)), flush=True)
+        return 0
     log(f"[bench] device {torch.cuda.get_device_name(0)}  N={N}  slices={args.slices}")
     X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
     b_h = rhs_of(X)
@@ -207,12 +318,13 @@ def main():
     roofline = None
     if st.get("launches"):
         avg_ms = st["trailing_ms"] / st["launches"]
-        achieved = st["trailing_ops"] / (st["trailing_ms"] * 1e-3) / 1e12
-        # per launch: the larger of its int8-MFMA time and its HBM time (FP64 C
-        # read + write) at peak, over the measured time: the fraction of the
-        # mixed roofline (the low slice counts are HBM-bound)
-        # steps: groups of panels, one trailing update (two launches: the next
-        # group's columns, then the rest) with K = the group's width each
+        t_tr = st["trailing_ms"] * 1e-3
+        # algorithmic work: 2 m n k flops per update (the FP64 GEMM the
+        # sliced product stands for), against the dense int8 peak
+        achieved = st["trailing_flops"] / t_tr / 1e12
+        int8_rate = st["trailing_ops"] / t_tr / 1e12
+        # per launch: the larger of its int8-MFMA time and its HBM time (FP64
+        # C read + write) at peak, over the measured time
         t_roof = 0.0
         starts = list(info_mix.get("step_starts") or range(0, N, args.nb)) + [N]
         for step in range(len(starts) - 2):
@@ -223,19 +335,25 @@ def main():
                 ops = 2.0 * n3 * nc * kb * S * (S + 1) / 2
                 byts = 16.0 * n3 * nc
                 t_roof += max(ops / (PEAK_INT8_TOPS * 1e12), byts / (PEAK_HBM_GBS * 1e9))
-        roofline = {"bound": "mfma", "kernel": f"ozaki_gemmd_kernel<{args.slices}, ...>",
+        roofline = {"bound": "mfma", "kernel": "ozaki_gemmd_kernel<5, 2, 4, 2, 1, 4, 1, 1>",
                     "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                     "frac": achieved / PEAK_INT8_TOPS,
                     "traffic": measured_traffic(N, args.slices, args.nb),
-                    "mixed_roofline_frac": t_roof / (st["trailing_ms"] * 1e-3),
-                    "note": ("int8 tera-operations/s of the trailing updates: 2 m n k x "
-                             "S_k (S_k + 1) / 2 slice pairs per launch, S_k the adaptive slice "
-                             f"count of step k (slice_counts), {st['launches']} launches, avg "
-                             f"{avg_ms:.3f} ms (HIP events on the factorisation's stream); "
-                             "mixed_roofline_frac: sum over launches of max(int8 ops / 5.0 POP/s, "
-                             "16 B per C element / 8 TB/s) over the measured time; traffic: HBM "
-                             "bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
-                             "(profiles/ozaki_gemmd_traffic_n32768.md)")}
+                    "algorithmic_bytes_per_launch": 16.0 * st["trailing_flops"] / (2 * 1024) / st["launches"],
+                    "launches": st["launches"], "avg_launch_ms": avg_ms,
+                    "int8_ops_rate_TOPS": int8_rate,
+                    "int8_ops_frac": int8_rate / PEAK_INT8_TOPS,
+                    "mixed_roofline_frac": t_roof / t_tr,
+                    "note": ("achieved: algorithmic flops of the trailing updates, 2 m n k per "
+                             "launch (k = 1024, a panel pair; the FP64 product the int8-sliced "
+                             "one replaces), over their summed HIP-event time on the "
+                             "factorisation's stream, against the 5.0 POP/s dense int8 peak. "
+                             "int8_ops_rate: the int8 operations actually done, 2 m n k "
+                             "S_k (S_k + 1) / 2 per launch (slice_counts). mixed_roofline_frac: "
+                             "sum over launches of max(int8 ops / 5.0 POP/s, 16 B per C element "
+                             "/ 8 TB/s) over the measured time. traffic: HBM bytes per launch "
+                             "(rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/"
+                             "ozaki_gemmd_traffic_n32768.md)")}
     plain_roof = None
     if plain_stats and plain_stats.get("launches"):
         a = plain_stats["trailing_ops"] / (plain_stats["trailing_ms"] * 1e-3) / 1e12
@@ -245,7 +363,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
-        cpu = cpu_baseline(N, min(args.cpu_sample, N))
+        cpu = cpu_baseline(N, min(args.cpu_sample, N), A_dev=A, b_full=b_h,
+                           gmres_iters=args.cpu_gmres_iters)
+        if args.cpu_full:
+            cpu["full_n_solve"] = cpu_full(N)
         log(f"[bench] CPU baseline {cpu}")
 
     out = {
@@ -306,16 +427,61 @@ def other_configs(log):
     x, t = timed(lambda: s.solve(A, bd, **kw))
     lu = LUIRSolver()
     x2, t2 = timed(lambda: lu.solve(A, bd, tol=1e-12, maxiter_outer=10))
+    from mpdok_amd.dense_krylov import DenseLinearOperator, cg
+    import contextlib
+    with contextlib.redirect_stdout(sys.stderr):     # the operator prints its sizes
+        op = DenseLinearOperator(A)
+    (x3, h3, c3), t3 = timed(lambda: cg(op, bd, tol=1e-12, body="fp64"))
     res["n4096_2d"] = {"gmres_ir_int8_lu": {"s": t, "relres": rel(x)},
-                       "lu_ir_fp32_lu_reference_symbol": {"s": t2, "relres": rel(x2)}}
+                       "lu_ir_fp32_lu_reference_symbol": {"s": t2, "relres": rel(x2)},
+                       "cg_fp64": {"s": t3, "relres": rel(x3), "iterations": h3[-1][0],
+                                   "converged": bool(c3),
+                                   "note": "stops by the reference's rule (<1% improvement "
+                                           "between checks every 10 iterations)"}}
+    del op
     log(f"[configs] n4096 {res['n4096_2d']}")
     del A, s, lu
     torch.cuda.empty_cache()
 
-    # out-of-core: Matern-3/2, length scale 0.1, nugget 1e-4, N = 65536 (the
-    # 262144-point FP32 operator, 275 GB, exceeds this process's host memory)
+    # out-of-core: Matern-3/2, length scale 0.1, nugget 1e-4. N = 262144
+    # (275 GB in FP32, more host memory than this process may use) runs
+    # matrix-free: Jacobi-preconditioned FP32 CG inside the FP64 refinement,
+    # capped at one outer step of 50 CG iterations (the full solve needs tens
+    # of thousands of iterations at this condition number)
     from mpdok_amd.mpdok_ooc import MPDOKOOCSolver
+    N = 262144
+    X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
+    bo = np.sin(2 * np.pi * X[:, 0]) * np.cos(2 * np.pi * X[:, 1]) * np.cos(2 * np.pi * X[:, 2])
+    o = MPDOKOOCSolver(tile_rows=4096)
+    o.build(X, reg=1e-4, store="free", verbose=False, kernel="matern32", length_scale=0.1)
+    v64 = torch.rand(N, dtype=torch.float64, device=_dev.device())
+    o._tiled_dgemv_fp64(v64)
+    _, tf64 = timed(lambda: o._tiled_dgemv_fp64(v64))
+    _, tf32 = timed(lambda: o._tiled_sgemv(v64.float()))
+    its = 50
+    _, tcg = timed(lambda: o.solve(bo, tol=1e-10, maxiter_outer=1, inner="cg", precond="jacobi",
+                                   inner_maxiter=its, verbose=False))
+    res["ooc_matern_n262144_matrix_free_cg"] = {
+        "matrix_free_fp64_product_s": tf64, "matrix_free_fp32_product_s": tf32,
+        "kernel_evals_per_s_fp32": N * N / tf32, "kernel_evals_per_s_fp64": N * N / tf64,
+        "capped_solve_s": tcg, "inner_cg_iterations": o.last_info["inner_iterations"],
+        "s_per_cg_iteration": tcg / max(1, sum(o.last_info["inner_iterations"])),
+        "relres_history": o.last_info["history"],
+        "note": "one outer FP64 step, 50 Jacobi-CG iterations in FP32; capped, not converged"}
+    log(f"[configs] ooc c3 {res['ooc_matern_n262144_matrix_free_cg']}")
+    o.free()
+    del o
+    torch.cuda.empty_cache()
+
+    # streamed FP32 operator from page-locked host memory, at the largest N
+    # this process's host memory holds comfortably
     N = 65536
+    try:
+        avail = int([l for l in open("/proc/meminfo") if l.startswith("MemAvailable")][0].split()[1]) * 1024
+        if avail > 3 * 131072 ** 2 * 4:
+            N = 131072
+    except (OSError, IndexError, ValueError):
+        pass
     X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
     o = MPDOKOOCSolver(tile_rows=4096)
     _, tb = timed(lambda: o.build(X, reg=1e-4, store="ram", verbose=False, kernel="matern32",
@@ -325,10 +491,10 @@ def other_configs(log):
     _, ts = timed(lambda: o._tiled_sgemv(v))
     v64 = v.double()
     _, tf = timed(lambda: o._tiled_dgemv_fp64(v64))
-    res["ooc_matern_n65536"] = {"build_s": tb, "stream_product_s": ts,
-                                "stream_GBps": N * N * 4 / ts / 1e9,
-                                "matrix_free_fp64_product_s": tf}
-    log(f"[configs] ooc {res['ooc_matern_n65536']}")
+    res[f"ooc_matern_stream_n{N}"] = {"build_s": tb, "stream_product_s": ts,
+                                      "stream_GBps": N * N * 4 / ts / 1e9,
+                                      "matrix_free_fp64_product_s": tf}
+    log(f"[configs] ooc stream {res[f'ooc_matern_stream_n{N}']}")
     o.free()
     del o
     torch.cuda.empty_cache()

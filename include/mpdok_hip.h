@@ -124,6 +124,14 @@ int mpdok_residual_f64(hipStream_t s, const double* A, int64_t lda, int order, i
 int mpdok_residual_f64_noalloc(hipStream_t s, const double* A, int64_t lda, int order, int N,
                                const double* x, const double* b, double* r, int mode);
 
+/* out[0] = sum of the FP64 values in `bytes` bytes at p (16-byte aligned, a
+ * multiple of 16), read once with 16-byte non-temporal loads: the read-only
+ * ceiling the GEMV kernels are measured against. work:
+ * mpdok_read_ceiling_workspace_bytes(). No reference counterpart:
+ * measurement. */
+size_t mpdok_read_ceiling_workspace_bytes(void);
+int mpdok_read_ceiling(hipStream_t s, const void* p, size_t bytes, double* out, void* work);
+
 /* Kernel-matrix tile: rows [row0,row0+rows) x cols [col0,col0+cols) of
  * k(c_i, c_j) + reg*delta_ij from coords (N x D, row-major FP64). kind 0 =
  * Gaussian exp(-p d2), 1 = Matern-3/2 with length scale p. reg_mode 0 = nugget in
@@ -251,14 +259,17 @@ int mpdok_lu_destroy(void* handle);
  * sizes without reallocating. No reference counterpart. */
 int mpdok_lu_resize(void* handle, int n);
 /* HIP-event timing of the trailing-update launches of later factorisations;
- * mpdok_lu_stats returns the summed time (ms), the algorithmic operations
- * (int8 operations for the sliced variant, flops otherwise) and the launch
- * count of the last factorisation. No reference counterpart: measurement. */
+ * mpdok_lu_stats returns the summed time (ms), the operations done (int8
+ * operations, 2 m n k S(S+1)/2 per launch, for the sliced variant; flops
+ * otherwise), the launch count and the algorithmic flops of the updates
+ * (2 m n k per launch) of the last factorisation. No reference counterpart:
+ * measurement. */
 int mpdok_lu_set_profiling(void* handle, int on);
 /* Factor the next panel on a side stream while the trailing update of the
  * current step finishes (default on; no-pivot handles only). */
 int mpdok_lu_set_lookahead(void* handle, int on);
-int mpdok_lu_stats(void* handle, double* trailing_ms, double* trailing_ops, int* launches);
+int mpdok_lu_stats(void* handle, double* trailing_ms, double* trailing_ops, int* launches,
+                   double* trailing_flops);
 /* Adaptive slice counts for the int8-sliced variant (default on when
  * slices <= 5): every panel pair is cut into `slices` digits, and each
  * trailing update multiplies only the leading S of them, S the fewest in

@@ -52,6 +52,8 @@ SIGNATURES = {
     "mpdok_gemv_f64": (I, [P, P, I64, I, I, I, P, P, P, SZ, P]),
     "mpdok_residual_f64": (I, [P, P, I64, I, I, P, P, P, I, P, SZ]),
     "mpdok_residual_f64_noalloc": (I, [P, P, I64, I, I, P, P, P, I]),
+    "mpdok_read_ceiling_workspace_bytes": (SZ, []),
+    "mpdok_read_ceiling": (I, [P, P, SZ, P, P]),
     # RBF assembly
     "mpdok_rbf_assemble": (I, [P, P, I, I, I, D, D, I, I, I, I, I, P, I, I64, I]),
     "mpdok_rbf_mean_d2": (I, [P, P, I, I, P]),
@@ -87,7 +89,7 @@ SIGNATURES = {
     "mpdok_lu_factors": (I, [P, PP, PP, PI]),
     "mpdok_lu_set_profiling": (I, [P, I]),
     "mpdok_lu_set_lookahead": (I, [P, I]),
-    "mpdok_lu_stats": (I, [P, ctypes.POINTER(D), ctypes.POINTER(D), PI]),
+    "mpdok_lu_stats": (I, [P, ctypes.POINTER(D), ctypes.POINTER(D), PI, ctypes.POINTER(D)]),
     "mpdok_lu_set_adaptive": (I, [P, I, I, I]),
     "mpdok_lu_slice_counts": (I, [P, PI, I, PI]),
     "mpdok_lu_set_update_panels": (I, [P, I, I, I]),

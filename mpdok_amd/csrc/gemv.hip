@@ -115,26 +115,37 @@ __global__ __launch_bounds__(256) void gemv_cm_partial_kernel(const T* __restric
   if (i0 >= M) return;
   T* out = part + (int64_t)c * M;
   if (VEC && i0 + W <= M) {
-    V acc0 = V(0), acc1 = V(0);
-    int j = j0;
-    for (; j + 1 < j1; j += 2) {
-      V a0 = nt_load(reinterpret_cast<const V*>(A + (int64_t)j * lda + i0));
-      V a1 = nt_load(reinterpret_cast<const V*>(A + (int64_t)(j + 1) * lda + i0));
-      T x0 = x[j], x1 = x[j + 1];
+    // U columns per iteration: U 16-byte loads in flight per lane
+    constexpr int U = 4;
+    V acc[U];
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        acc0[w] = __builtin_fma(a0[w], x0, acc0[w]);
-        acc1[w] = __builtin_fma(a1[w], x1, acc1[w]);
+    for (int u = 0; u < U; ++u) acc[u] = V(0);
+    int j = j0;
+    for (; j + U - 1 < j1; j += U) {
+      V a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        a[u] = nt_load(reinterpret_cast<const V*>(A + (int64_t)(j + u) * lda + i0));
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const T xu = x[j + u];
+#pragma unroll
+        for (int w = 0; w < W; ++w) acc[u][w] = __builtin_fma(a[u][w], xu, acc[u][w]);
       }
     }
-    if (j < j1) {
+    for (int u = 0; j < j1; ++j, ++u) {
       V a0 = nt_load(reinterpret_cast<const V*>(A + (int64_t)j * lda + i0));
       T x0 = x[j];
 #pragma unroll
-      for (int w = 0; w < W; ++w) acc0[w] = __builtin_fma(a0[w], x0, acc0[w]);
+      for (int w = 0; w < W; ++w) acc[u][w] = __builtin_fma(a0[w], x0, acc[u][w]);
     }
+    // the U partial sums in a fixed pairwise order
 #pragma unroll
-    for (int w = 0; w < W; ++w) out[i0 + w] = acc0[w] + acc1[w];
+    for (int h = U / 2; h > 0; h /= 2)
+#pragma unroll
+      for (int u = 0; u < h; ++u) acc[u] = acc[u] + acc[u + h];
+#pragma unroll
+    for (int w = 0; w < W; ++w) out[i0 + w] = acc[0][w];
   } else {
     for (int w = 0; w < W && i0 + w < M; ++w) {
       T acc = T(0);
@@ -402,6 +413,57 @@ int residual_f64(hipStream_t s, const double* A, int64_t lda, int order, int N, 
 }
 
 // ---------------------------------------------------------------------------
+// Read-only ceiling: the sum of `bytes` bytes read as FP64 pairs (16-byte
+// non-temporal loads, grid-stride), partial sums per workgroup, then one
+// workgroup sums the partials in a fixed order. It does only what a GEMV must
+// (stream the matrix in once), so its rate is the ceiling the GEMV kernels
+// are measured against. No reference counterpart: measurement.
+// ---------------------------------------------------------------------------
+constexpr int RC_BLOCKS = 2048;
+
+__global__ __launch_bounds__(256) void read_ceiling_kernel(const d2* __restrict__ p, size_t n2,
+                                                           double* __restrict__ part) {
+  double acc = 0.0;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += (size_t)gridDim.x * 256) {
+    const d2 v = __builtin_nontemporal_load(p + i);
+    acc += v[0] + v[1];
+  }
+  acc = wave_sum(acc);
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ __launch_bounds__(256) void read_ceiling_sum_kernel(const double* __restrict__ part,
+                                                               int np, double* __restrict__ out) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) acc += part[i];
+  acc = wave_sum(acc);
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+size_t read_ceiling_workspace_bytes() { return RC_BLOCKS * sizeof(double); }
+
+int read_ceiling(hipStream_t s, const void* p, size_t bytes, double* out, void* work) {
+  MPDOK_REQUIRE(p && out && work, "read_ceiling: null pointer");
+  MPDOK_REQUIRE((reinterpret_cast<uintptr_t>(p) & 15) == 0 && bytes % 16 == 0,
+                "read_ceiling: the buffer must be 16-byte aligned and a multiple of 16 bytes");
+  const size_t n2 = bytes / 16;
+  int nb = (int)((n2 + 255) / 256);
+  if (nb > RC_BLOCKS) nb = RC_BLOCKS;
+  if (nb < 1) nb = 1;
+  double* part = static_cast<double*>(work);
+  read_ceiling_kernel<<<nb, 256, 0, s>>>(static_cast<const d2*>(p), n2, part);
+  read_ceiling_sum_kernel<<<1, 256, 0, s>>>(part, nb, out);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+// ---------------------------------------------------------------------------
 // Allocation-free residual. The column-major residual's partial sums (high
 // and low parts per column chunk and row, resid_workspace_bytes) live in a
 // file-scope device array instead of a caller workspace. Each GPU holds its
@@ -470,6 +532,13 @@ MPDOK_API int mpdok_gemv_f64(hipStream_t s, const double* A, int64_t lda, int or
                              const double* x, double* y, void* work, size_t work_bytes,
                              const int* skip) {
   return mpdok::gemv_f64(s, A, lda, order, M, N, x, y, 1.0, work, work_bytes, skip);
+}
+MPDOK_API size_t mpdok_read_ceiling_workspace_bytes(void) {
+  return mpdok::read_ceiling_workspace_bytes();
+}
+MPDOK_API int mpdok_read_ceiling(hipStream_t s, const void* p, size_t bytes, double* out,
+                                 void* work) {
+  return mpdok::read_ceiling(s, p, bytes, out, work);
 }
 MPDOK_API int mpdok_residual_f64_noalloc(hipStream_t s, const double* A, int64_t lda, int order,
                                          int N, const double* x, const double* b, double* r,

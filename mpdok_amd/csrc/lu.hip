@@ -95,7 +95,7 @@ struct LUHandle {
   std::vector<hipEvent_t> ev;
   std::vector<double> ev_ops;  // algorithmic operations of each timed launch (per slice pair)
   std::vector<int> ev_step;    // step of each timed launch
-  double trailing_ms = 0.0, trailing_ops = 0.0;
+  double trailing_ms = 0.0, trailing_ops = 0.0, trailing_flops = 0.0;
   int trailing_launches = 0;
 };
 
@@ -749,6 +749,7 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
   if (h->profile) {
     h->trailing_ms = 0.0;
     h->trailing_ops = 0.0;
+    h->trailing_flops = 0.0;
     h->trailing_launches = (int)h->ev_ops.size();
     for (size_t i = 0; i < h->ev_ops.size(); ++i) {
       float ms = 0.f;
@@ -756,6 +757,7 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
       h->trailing_ms += ms;
       const int Sk = h->precision == LU_OZAKI ? h->steps_S[h->ev_step[i]] : 1;
       h->trailing_ops += h->ev_ops[i] * (h->precision == LU_OZAKI ? Sk * (Sk + 1) / 2.0 : 1.0);
+      h->trailing_flops += h->ev_ops[i];
     }
   }
   return OK;
@@ -859,12 +861,14 @@ MPDOK_API int mpdok_lu_set_profiling(void* hv, int on) {
 // Totals of the last profiled factorisation: summed event time of the trailing
 // updates (ms), their algorithmic operations (int8 ops for the sliced variant,
 // flops otherwise) and the number of timed launches.
-MPDOK_API int mpdok_lu_stats(void* hv, double* trailing_ms, double* trailing_ops, int* launches) {
+MPDOK_API int mpdok_lu_stats(void* hv, double* trailing_ms, double* trailing_ops, int* launches,
+                             double* trailing_flops) {
   auto* h = static_cast<mpdok::LUHandle*>(hv);
   MPDOK_REQUIRE(h != nullptr, "lu_stats: null handle");
   if (trailing_ms) *trailing_ms = h->trailing_ms;
   if (trailing_ops) *trailing_ops = h->trailing_ops;
   if (launches) *launches = h->trailing_launches;
+  if (trailing_flops) *trailing_flops = h->trailing_flops;
   return 0;
 }
 // Plain GEMM on the matrix cores (C = alpha A B + beta C, column-major), for

@@ -228,10 +228,12 @@ class _LUHandle:
 
     def stats(self):
         """HIP-event totals of the trailing updates of the last factorisation."""
-        ms, ops, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        ms, ops, n, fl = ctypes.c_double(), ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
         check(self._lib.mpdok_lu_stats(self.ptr, ctypes.byref(ms), ctypes.byref(ops),
-                                       ctypes.byref(n)), "lu_stats", from_lib=self._lib)
-        return {"trailing_ms": ms.value, "trailing_ops": ops.value, "launches": n.value}
+                                       ctypes.byref(n), ctypes.byref(fl)), "lu_stats",
+              from_lib=self._lib)
+        return {"trailing_ms": ms.value, "trailing_ops": ops.value, "launches": n.value,
+                "trailing_flops": fl.value}
 
     def solve(self, b, out=None):
         x = out if out is not None else torch.empty_like(b)
