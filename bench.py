@@ -443,6 +443,36 @@ def other_configs(log):
     del A, s, lu
     torch.cuda.empty_cache()
 
+    # factor once, then 72 right-hand sides (the reference's weather-front time
+    # series, run_demo.py:75-109; rbf_demo.ipynb:247): the reference-default
+    # LUIRSolver (FP32 LU with partial pivoting, FP64 refinement) at N = 8192
+    # and 32768, 72 solve_factored calls one at a time and as one block
+    from mpdok_amd import rbf_kernel
+    for N in (8192, 32768):
+        coords = rbf_kernel.synthetic_coords(N, seed=0)
+        A, gam = rbf_kernel.build_rbf_kernel(coords, reg=1e-2)
+        Bs = torch.stack([rbf_kernel.weather_front(coords, t) for t in range(72)], dim=1)
+        lu = LUIRSolver()
+        lu.factor(A)
+        _, tfac = timed(lambda: lu.factor(A))
+        xs = lu.solve_factored(Bs[:, 0].contiguous())
+        _, tone = timed(lambda: [lu.solve_factored(Bs[:, t].contiguous()) for t in range(72)])
+        Xb = lu.solve_factored(Bs)
+        _, tblk = timed(lambda: lu.solve_factored(Bs))
+        rels = [float(_ops.norm(_ops.residual(A, Xb[:, t].contiguous(), Bs[:, t].contiguous(),
+                                              mode="fp64")) / _ops.norm(Bs[:, t].contiguous()))
+                for t in range(72)]
+        res[f"factor_once_72_rhs_n{N}"] = {
+            "factor_s": tfac, "solve_one_at_a_time_s_per_rhs": tone / 72,
+            "solve_block_s_per_rhs": tblk / 72, "max_relres": max(rels),
+            "all_converged": bool(all(lu.last_info["converged"])), "gamma": gam,
+            "method": "LUIRSolver() defaults: FP32 getrf-style LU with partial pivoting, FP64 "
+                      "residual and update, tol 1e-11, maxiter_outer 3"}
+        log(f"[configs] factor once N={N} {res[f'factor_once_72_rhs_n{N}']}")
+        lu.free_factored()
+        del A, Bs, Xb, lu
+        torch.cuda.empty_cache()
+
     # out-of-core: Matern-3/2, length scale 0.1, nugget 1e-4. N = 262144
     # (275 GB in FP32, more host memory than this process may use) runs
     # matrix-free: Jacobi-preconditioned FP32 CG inside the FP64 refinement,

@@ -124,6 +124,16 @@ int mpdok_residual_f64(hipStream_t s, const double* A, int64_t lda, int order, i
 int mpdok_residual_f64_noalloc(hipStream_t s, const double* A, int64_t lda, int order, int N,
                                const double* x, const double* b, double* r, int mode);
 
+/* R = B - A X for nrhs right-hand sides (column-major A, X, B, R; mode as
+ * mpdok_residual_f64): A is read once per block of up to 8 columns, and each
+ * column of R equals mpdok_residual_f64's bit for bit. work:
+ * mpdok_residual_multi_workspace_bytes(N). Replaces nrhs Dgemv residuals of
+ * the reference's one-at-a-time LU-IR (benchmark_mpdok.ipynb:991). */
+size_t mpdok_residual_multi_workspace_bytes(int N);
+int mpdok_residual_multi_f64(hipStream_t s, const double* A, int64_t lda, int N, const double* X,
+                             int64_t ldx, const double* B, int64_t ldb, double* R, int64_t ldr,
+                             int nrhs, int mode, void* work, size_t work_bytes);
+
 /* out[0] = sum of the FP64 values in `bytes` bytes at p (16-byte aligned, a
  * multiple of 16), read once with 16-byte non-temporal loads: the read-only
  * ceiling the GEMV kernels are measured against. work:
@@ -253,6 +263,14 @@ int mpdok_lu_factor(void* handle, hipStream_t s, const double* A, int64_t lda, i
 /* x = inv(P L U) b (FP64 vectors; x may alias b). Replaces Sgetrs
  * (benchmark_mpdok.ipynb:990,992). */
 int mpdok_lu_solve(void* handle, hipStream_t s, const double* b, double* x);
+/* X[:, 0:nrhs] = inv(P L U) B[:, 0:nrhs] (FP64, column-major, ldb / ldx >= N;
+ * X must not alias B). Blocks of up to 8 right-hand sides share each read of
+ * the factors; every column equals mpdok_lu_solve's result bit for bit. The
+ * multi-right-hand-side Sgetrs the reference's factor-once use case would call
+ * once per field (run_demo.py:75-109 solves 72 right-hand sides with
+ * lu_ir.so::py_lu_solve_factored, mpdok_ops.py:204-243, one at a time). */
+int mpdok_lu_solve_multi(void* handle, hipStream_t s, const double* B, int64_t ldb, double* X,
+                         int64_t ldx, int nrhs);
 int mpdok_lu_destroy(void* handle);
 /* Use the first n <= N (N as created) rows and columns of the handle's
  * buffers for the next factorisation: one handle serves systems of many
@@ -345,12 +363,23 @@ int mpdok_lu_ir(hipStream_t s, const double* A64, int64_t lda, int order, void* 
                 const double* b, double* x, double tol, int maxiter_outer, int resid_mode,
                 int* converged, double* hist, int hist_cap, int* n_hist);
 
+/* mpdok_lu_ir for nrhs right-hand sides (column-major B, X, ldb / ldx >= N):
+ * blocks of up to 8 share one residual pass over A and one solve with the
+ * factors per outer step; each column follows mpdok_lu_ir's loop exactly (the
+ * same iterates and result bit for bit, no update once it has converged).
+ * converged, relres: host arrays of nrhs. Synchronises s. The multi-right-hand-
+ * side form of lu_ir.so::py_lu_solve_factored (mpdok_ops.py:204-243). */
+int mpdok_lu_ir_multi(hipStream_t s, const double* A64, int64_t lda, int order, void* lu, int N,
+                      const double* B, int64_t ldb, double* X, int64_t ldx, int nrhs, double tol,
+                      int maxiter_outer, int resid_mode, int* converged, double* relres);
+
 /* Batched uneven kernel systems (north star's batched configuration; no
  * reference counterpart). System i: N[i] points coords[i] (device, N x D
  * row-major), kernel (kind, param[i]) + reg I, nrhs right-hand sides B[i]
  * (device, N x nrhs column-major) solved into X[i] (same layout) to relres <=
  * tol by LU (precision 0 FP64 or 4 int8-sliced, no pivoting) + LU-IR with the
- * Dot2 residual. nstreams workers (host threads with their own streams),
+ * Dot2 residual, all right-hand sides of a system refined together
+ * (mpdok_lu_ir_multi). nstreams workers (host threads with their own streams),
  * largest systems first, longest-processing-time assignment. converged and
  * relres (host, nsys x nrhs) receive the per-right-hand-side outcome. */
 int mpdok_batched_rbf_solve(int nsys, const int* N, const double* const* coords, int D, int kind,

@@ -85,6 +85,10 @@ SIGNATURES = {
     "mpdok_lu_factor": (I, [P, P, P, I64, I, PI]),
     "mpdok_lu_solve": (I, [P, P, P, P]),
     "mpdok_lu_destroy": (I, [P]),
+    "mpdok_lu_solve_multi": (I, [P, P, P, I64, P, I64, I]),
+    "mpdok_residual_multi_workspace_bytes": (SZ, [I]),
+    "mpdok_residual_multi_f64": (I, [P, P, I64, I, P, I64, P, I64, P, I64, I, I, P, SZ]),
+    "mpdok_lu_ir_multi": (I, [P, P, I64, I, P, I, P, I64, P, I64, I, D, I, I, P, P]),
     "mpdok_lu_resize": (I, [P, I]),
     "mpdok_lu_factors": (I, [P, PP, PP, PI]),
     "mpdok_lu_set_profiling": (I, [P, I]),

@@ -12,6 +12,11 @@ int gemv_f64(hipStream_t s, const double* A, int64_t lda, int order, int M, int 
              double* y, double alpha, void* work, size_t work_bytes, const int* skip);
 int residual_f64(hipStream_t s, const double* A, int64_t lda, int order, int N, const double* x,
                  const double* b, double* r, int mode, void* work, size_t work_bytes);
+// R = B - A X for nrhs columns, column-major A (the single residual's bits per column)
+size_t resid_multi_workspace_bytes(int N);
+int residual_multi_f64(hipStream_t s, const double* A, int64_t lda, int N, const double* X,
+                       int64_t ldx, const double* B, int64_t ldb, double* R, int64_t ldr,
+                       int nrhs, int mode, void* work, size_t work_bytes);
 
 // blas1.hip
 size_t reduce_workspace_bytes(int n);

@@ -305,6 +305,85 @@ __global__ __launch_bounds__(256) void resid_cm_reduce_kernel(const double* __re
   }
 }
 
+// R = B - A X for NR right-hand sides (column-major A, X, B, R): each element
+// of A is read once for NR products. For every right-hand side the partial
+// sums run in the order of resid_cm_partial_kernel and the chunks are reduced
+// as resid_cm_reduce_kernel does, so each column of R equals the single
+// residual's bit for bit. Partials of right-hand side q, chunk c:
+// part[(q nch + c) N + i].
+template <int MODE, int NR>
+__global__ __launch_bounds__(256) void resid_cm_multi_partial_kernel(
+    const double* __restrict__ A, int64_t lda, int N, const double* __restrict__ X, int64_t ldx,
+    double* __restrict__ part_hi, double* __restrict__ part_lo, int cpc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y, nch = gridDim.y;
+  const int j0 = c * cpc, j1 = min(N, j0 + cpc);
+  if (i >= N) return;
+  dd acc[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) acc[q] = dd{0.0, 0.0};
+  int j = j0;
+  for (; j + 3 < j1; j += 4) {
+    const double a0 = nt_load(A + (int64_t)j * lda + i);
+    const double a1 = nt_load(A + (int64_t)(j + 1) * lda + i);
+    const double a2 = nt_load(A + (int64_t)(j + 2) * lda + i);
+    const double a3 = nt_load(A + (int64_t)(j + 3) * lda + i);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const double* x = X + (int64_t)q * ldx;
+      if (MODE == 0) {
+        acc[q].hi = __builtin_fma(a0, x[j], acc[q].hi);
+        acc[q].hi = __builtin_fma(a1, x[j + 1], acc[q].hi);
+        acc[q].hi = __builtin_fma(a2, x[j + 2], acc[q].hi);
+        acc[q].hi = __builtin_fma(a3, x[j + 3], acc[q].hi);
+      } else {
+        dd_add_prod(acc[q], a0, x[j]);
+        dd_add_prod(acc[q], a1, x[j + 1]);
+        dd_add_prod(acc[q], a2, x[j + 2]);
+        dd_add_prod(acc[q], a3, x[j + 3]);
+      }
+    }
+  }
+  for (; j < j1; ++j) {
+    const double a0 = A[(int64_t)j * lda + i];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      if (MODE == 0)
+        acc[q].hi = __builtin_fma(a0, X[(int64_t)q * ldx + j], acc[q].hi);
+      else
+        dd_add_prod(acc[q], a0, X[(int64_t)q * ldx + j]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    part_hi[((int64_t)q * nch + c) * N + i] = acc[q].hi;
+    if (MODE == 1) part_lo[((int64_t)q * nch + c) * N + i] = acc[q].lo;
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void resid_cm_multi_reduce_kernel(
+    const double* __restrict__ part_hi, const double* __restrict__ part_lo, int N, int nchunks,
+    const double* __restrict__ B, int64_t ldb, double* __restrict__ R, int64_t ldr) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int q = blockIdx.y;
+  if (i >= N) return;
+  const double* ph = part_hi + (int64_t)q * nchunks * N;
+  const double* pl = part_lo + (int64_t)q * nchunks * N;
+  const double b = B[(int64_t)q * ldb + i];
+  if (MODE == 0) {
+    double s = 0.0;
+    for (int c = 0; c < nchunks; ++c) s += ph[(int64_t)c * N + i];
+    R[(int64_t)q * ldr + i] = b - s;
+  } else {
+    dd s{0.0, 0.0};
+    for (int c = 0; c < nchunks; ++c) s = dd_merge(s, dd{ph[(int64_t)c * N + i], pl[(int64_t)c * N + i]});
+    double h, t;
+    two_sum(b, -s.hi, h, t);
+    R[(int64_t)q * ldr + i] = h + (t - s.lo);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -407,6 +486,57 @@ int residual_f64(hipStream_t s, const double* A, int64_t lda, int order, int N, 
       resid_cm_partial_kernel<1><<<grid, 256, 0, s>>>(A, lda, N, x, ph, pl, cpc);
       resid_cm_reduce_kernel<1><<<(N + 255) / 256, 256, 0, s>>>(ph, pl, N, nch, b, r);
     }
+  }
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+// Up to RESID_MULTI right-hand sides per pass (8, then 4, 2, 1).
+constexpr int RESID_MULTI = 8;
+
+size_t resid_multi_workspace_bytes(int N) {
+  if (N <= 0) return 0;
+  const int nch = cm_chunks(N, 256, N);
+  return (size_t)nch * N * 8 * 2 * RESID_MULTI;
+}
+
+template <int MODE, int NR>
+static void resid_multi_launch(hipStream_t s, const double* A, int64_t lda, int N, const double* X,
+                               int64_t ldx, const double* B, int64_t ldb, double* R, int64_t ldr,
+                               double* ph, double* pl, int nch, int cpc) {
+  resid_cm_multi_partial_kernel<MODE, NR><<<dim3((N + 255) / 256, nch), 256, 0, s>>>(
+      A, lda, N, X, ldx, ph, pl, cpc);
+  resid_cm_multi_reduce_kernel<MODE><<<dim3((N + 255) / 256, NR), 256, 0, s>>>(ph, pl, N, nch, B,
+                                                                             ldb, R, ldr);
+}
+
+int residual_multi_f64(hipStream_t s, const double* A, int64_t lda, int N, const double* X,
+                       int64_t ldx, const double* B, int64_t ldb, double* R, int64_t ldr,
+                       int nrhs, int mode, void* work, size_t work_bytes) {
+  MPDOK_REQUIRE(A && X && B && R && N >= 0 && nrhs >= 0, "residual_multi: bad arguments");
+  MPDOK_REQUIRE(mode == 0 || mode == 1, "residual_multi: mode must be 0 (fp64) or 1 (dot2)");
+  MPDOK_REQUIRE(lda >= N && ldx >= N && ldb >= N && ldr >= N, "residual_multi: ld < N");
+  if (N == 0 || nrhs == 0) return OK;
+  const int nch = cm_chunks(N, 256, N);
+  const size_t need = resid_multi_workspace_bytes(N);
+  MPDOK_REQUIRE(work != nullptr && work_bytes >= need,
+                "residual_multi: needs a workspace of %zu bytes", need);
+  const int cpc = (N + nch - 1) / nch;
+  double* ph = static_cast<double*>(work);
+  double* pl = ph + (size_t)nch * N * RESID_MULTI;
+  for (int c0 = 0; c0 < nrhs;) {
+    const int nr = nrhs - c0 >= 8 ? 8 : nrhs - c0 >= 4 ? 4 : nrhs - c0 >= 2 ? 2 : 1;
+    const double* Xc = X + (int64_t)c0 * ldx;
+    const double* Bc = B + (int64_t)c0 * ldb;
+    double* Rc = R + (int64_t)c0 * ldr;
+#define RM(MD, NRV) resid_multi_launch<MD, NRV>(s, A, lda, N, Xc, ldx, Bc, ldb, Rc, ldr, ph, pl, nch, cpc)
+    if (mode == 0) {
+      if (nr == 8) RM(0, 8); else if (nr == 4) RM(0, 4); else if (nr == 2) RM(0, 2); else RM(0, 1);
+    } else {
+      if (nr == 8) RM(1, 8); else if (nr == 4) RM(1, 4); else if (nr == 2) RM(1, 2); else RM(1, 1);
+    }
+#undef RM
+    c0 += nr;
   }
   MPDOK_HIP(hipGetLastError());
   return OK;
@@ -532,6 +662,16 @@ MPDOK_API int mpdok_gemv_f64(hipStream_t s, const double* A, int64_t lda, int or
                              const double* x, double* y, void* work, size_t work_bytes,
                              const int* skip) {
   return mpdok::gemv_f64(s, A, lda, order, M, N, x, y, 1.0, work, work_bytes, skip);
+}
+MPDOK_API size_t mpdok_residual_multi_workspace_bytes(int N) {
+  return mpdok::resid_multi_workspace_bytes(N);
+}
+MPDOK_API int mpdok_residual_multi_f64(hipStream_t s, const double* A, int64_t lda, int N,
+                                       const double* X, int64_t ldx, const double* B,
+                                       int64_t ldb, double* R, int64_t ldr, int nrhs, int mode,
+                                       void* work, size_t work_bytes) {
+  return mpdok::residual_multi_f64(s, A, lda, N, X, ldx, B, ldb, R, ldr, nrhs, mode, work,
+                                   work_bytes);
 }
 MPDOK_API size_t mpdok_read_ceiling_workspace_bytes(void) {
   return mpdok::read_ceiling_workspace_bytes();

@@ -51,6 +51,11 @@ constexpr int TRSV_BLOCK = 512;  // diagonal block of the inverse-based triangul
 template <typename T, typename TV>
 int trsv_block_inverse(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
                        const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc, int tbw);
+// The same for nr (1, 2, 4 or 8) right-hand sides, column-major N x nr
+// blocks with ld N; acc: 2 (tbw / 64) N nr elements.
+template <typename T, typename TV>
+int trsv_block_inverse_multi(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
+                             const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc, int tbw, int nr);
 // The off-diagonal-free parts of 2 TRSV_BLOCK-wide diagonal blocks of an
 // inverse from its TRSV_BLOCK-wide ones (see lu_kernels.hip).
 template <typename T>
@@ -83,6 +88,9 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
 int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int order, int* info);
 // x = inv(P L U) b  (b, x FP64; x may alias b)
 int lu_solve(LUHandle* h, hipStream_t s, const double* b, double* x);
+// X = inv(P L U) B for k columns (column-major, ldb / ldx >= N)
+int lu_solve_multi(LUHandle* h, hipStream_t s, const double* B, int64_t ldb, double* X,
+                   int64_t ldx, int k);
 void lu_destroy(LUHandle* h);
 int lu_n(const LUHandle* h);
 // Use the first n <= capacity rows / columns of the buffers for the next

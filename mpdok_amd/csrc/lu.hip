@@ -61,6 +61,11 @@ struct LUHandle {
   bool wide_solve = false;
   void* v3 = nullptr;
   void* acc = nullptr;
+  // multi-right-hand-side solves (allocated on first use): three N x
+  // MULTI_NR vector blocks and the accumulators of MULTI_NR solves
+  static constexpr int MULTI_NR = 8;
+  void* mv = nullptr;
+  void* macc = nullptr;
   // lookahead (no pivoting): panel k+1 is factored on `side` while the main
   // stream finishes the trailing update of step k; Wp is the side's scratch
   hipStream_t side = nullptr;
@@ -124,7 +129,7 @@ void lu_destroy(LUHandle* h) {
   void* ptrs[] = {h->LU, h->Linv, h->Uinv, h->W, h->ipiv, h->perm, h->d_info,
                   h->piv_work, h->oz_work, h->oz_work2, h->v1, h->v2, h->Lbinv, h->Ubinv, h->v3,
                   h->Lbinv2, h->Ubinv2,
-                  h->acc,
+                  h->acc, h->mv, h->macc,
                   h->d_S};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -801,6 +806,72 @@ static int solve_impl(LUHandle* h, hipStream_t s, const double* b, double* x) {
   return OK;
 }
 
+// X[:, 0:k] = inv(P L U) B[:, 0:k] (FP64, column-major, leading dimensions
+// ldb / ldx >= N): blocks of up to MULTI_NR right-hand sides go through the
+// triangular solves together (8, then 4, 2, 1), so the factors are read once
+// per block; each column equals the single solve's bit for bit. Without the
+// inverse-based solves (N not a multiple of TRSV_BLOCK) the columns are solved
+// one at a time.
+template <typename T, typename TV>
+static int solve_multi_impl(LUHandle* h, hipStream_t s, const double* B, int64_t ldb, double* X,
+                            int64_t ldx, int k) {
+  const int N = h->N;
+  constexpr int NR = LUHandle::MULTI_NR;
+  const size_t vb = (size_t)N * NR * sizeof(TV);
+  const int tbw = h->wide_solve ? 2 * TRSV_BLOCK : TRSV_BLOCK;
+  if (!h->mv) MPDOK_HIP(hipMalloc(&h->mv, 3 * vb));
+  if (!h->macc) MPDOK_HIP(hipMalloc(&h->macc, 2 * (size_t)(2 * TRSV_BLOCK / 64) * vb));
+  TV* v = static_cast<TV*>(h->mv);
+  TV* v2 = v + (size_t)N * NR;
+  TV* v3 = v2 + (size_t)N * NR;
+  for (int c0 = 0; c0 < k;) {
+    const int nr = k - c0 >= 8 ? 8 : k - c0 >= 4 ? 4 : k - c0 >= 2 ? 2 : 1;
+    if (sizeof(TV) == 8)
+      MPDOK_TRY((copy2d<double, double>(s, B + (int64_t)c0 * ldb, ldb,
+                                        reinterpret_cast<double*>(v), N, N, nr)));
+    else
+      MPDOK_TRY((copy2d<double, float>(s, B + (int64_t)c0 * ldb, ldb,
+                                       reinterpret_cast<float*>(v), N, N, nr)));
+    TV* in = v;
+    if (h->pivot == PIV_PARTIAL) {
+      for (int q = 0; q < nr; ++q)
+        MPDOK_TRY(permute<TV>(s, v + (size_t)q * N, v2 + (size_t)q * N, h->perm, N));
+      in = v2;
+    }
+    TV* out = in == v ? v2 : v;
+    MPDOK_TRY((trsv_block_inverse_multi<T, TV>(
+        s, static_cast<const T*>(h->LU), N, N,
+        static_cast<const T*>(h->wide_solve ? h->Lbinv2 : h->Lbinv),
+        static_cast<const T*>(h->wide_solve ? h->Ubinv2 : h->Ubinv), in, out, v3,
+        static_cast<TV*>(h->macc), tbw, nr)));
+    if (sizeof(TV) == 8)
+      MPDOK_TRY((copy2d<double, double>(s, reinterpret_cast<double*>(out), N,
+                                        X + (int64_t)c0 * ldx, ldx, N, nr)));
+    else
+      MPDOK_TRY((copy2d<float, double>(s, reinterpret_cast<float*>(out), N,
+                                       X + (int64_t)c0 * ldx, ldx, N, nr)));
+    c0 += nr;
+  }
+  return OK;
+}
+
+int lu_solve_multi(LUHandle* h, hipStream_t s, const double* B, int64_t ldb, double* X,
+                   int64_t ldx, int k) {
+  MPDOK_REQUIRE(h && B && X && k >= 0, "lu_solve_multi: bad arguments");
+  MPDOK_REQUIRE(ldb >= h->N && ldx >= h->N, "lu_solve_multi: leading dimension < N");
+  if (!h->factored) {
+    set_error("lu_solve_multi: no factorization available");
+    return ENOFACTOR;
+  }
+  if (k == 0) return OK;
+  if (!h->fast_solve) {
+    for (int c = 0; c < k; ++c) MPDOK_TRY(lu_solve(h, s, B + (int64_t)c * ldb, X + (int64_t)c * ldx));
+    return OK;
+  }
+  if (fp64_storage(h->precision)) return solve_multi_impl<double, double>(h, s, B, ldb, X, ldx, k);
+  return solve_multi_impl<float, float>(h, s, B, ldb, X, ldx, k);
+}
+
 int lu_solve(LUHandle* h, hipStream_t s, const double* b, double* x) {
   MPDOK_REQUIRE(h && b && x, "lu_solve: null argument");
   if (!h->factored) {
@@ -825,6 +896,10 @@ MPDOK_API int mpdok_lu_factor(void* h, hipStream_t s, const double* A, int64_t l
 }
 MPDOK_API int mpdok_lu_solve(void* h, hipStream_t s, const double* b, double* x) {
   return mpdok::lu_solve(static_cast<mpdok::LUHandle*>(h), s, b, x);
+}
+MPDOK_API int mpdok_lu_solve_multi(void* h, hipStream_t s, const double* B, int64_t ldb, double* X,
+                                   int64_t ldx, int nrhs) {
+  return mpdok::lu_solve_multi(static_cast<mpdok::LUHandle*>(h), s, B, ldb, X, ldx, nrhs);
 }
 MPDOK_API int mpdok_lu_resize(void* h, int n) {
   return mpdok::lu_resize(static_cast<mpdok::LUHandle*>(h), n);

@@ -981,6 +981,172 @@ template int trsv_block_inverse<float, float>(hipStream_t, const float*, int64_t
                                               const float*, const float*, float*, float*, float*,
                                               int);
 
+// ---------------------------------------------------------------------------
+// The same solves for NR right-hand sides at once (column-major N x NR
+// blocks, ld N): every matrix element read serves NR products, so the
+// factors stream from HBM once per block of right-hand sides. For each
+// right-hand side the sums run in the single-vector kernels' order, so every
+// column of the result is bit for bit that of a single solve. Accumulators:
+// acc[(r TB_CH + c) N + row] for right-hand side r, column group c (xpart
+// likewise).
+// ---------------------------------------------------------------------------
+template <int TBW, typename T, typename TV, int NR>
+__global__ __launch_bounds__(256) void diag_apply_multi_kernel(const T* __restrict__ Binv, int r0,
+                                                               const TV* __restrict__ v,
+                                                               const TV* __restrict__ acc, int N,
+                                                               TV* __restrict__ xpart, int lower) {
+  constexpr int TB_CH = TBW / TB_CW;
+  __shared__ TV vs[NR][TB_CW];
+  __shared__ TV red[NR][4][64];
+  const int rg = blockIdx.x, cg = blockIdx.y, t = threadIdx.x;
+  if (lower ? cg > rg : cg < rg) {
+    if (t < 64)
+#pragma unroll
+      for (int q = 0; q < NR; ++q) xpart[((size_t)q * TB_CH + cg) * N + r0 + rg * 64 + t] = TV(0);
+    return;
+  }
+  const int r = t & 63, p = t >> 6;
+  const T* col = Binv + rg * 64 + r + (size_t)(cg * TB_CW) * TBW;
+  T bv[TB_CW / 4];
+#pragma unroll
+  for (int q = 0; q < TB_CW / 4; ++q) bv[q] = col[(size_t)(p + 4 * q) * TBW];
+  if (t < TB_CW) {
+    const int g = r0 + cg * TB_CW + t;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      TV sv = v[(size_t)q * N + g];
+#pragma unroll
+      for (int c = 0; c < TB_CH; ++c) sv -= acc[((size_t)q * TB_CH + c) * N + g];
+      vs[q][t] = sv;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    TV a = TV(0);
+#pragma unroll
+    for (int u = 0; u < TB_CW / 4; ++u) a = __builtin_fma((TV)bv[u], vs[q][p + 4 * u], a);
+    red[q][p][r] = a;
+  }
+  __syncthreads();
+  if (t < 64)
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+      xpart[((size_t)q * TB_CH + cg) * N + r0 + rg * 64 + t] =
+          (red[q][0][t] + red[q][1][t]) + (red[q][2][t] + red[q][3][t]);
+}
+
+template <int TBW, typename T, typename TV, int NR>
+__global__ __launch_bounds__(256) void tri_acc_multi_kernel(const T* __restrict__ M, int64_t ldm,
+                                                            int rbeg, int rend, int c0,
+                                                            const TV* __restrict__ xpart,
+                                                            TV* __restrict__ x,
+                                                            TV* __restrict__ acc, int N) {
+  constexpr int TB_CH = TBW / TB_CW;
+  __shared__ TV xs[NR][TB_CW];
+  const int c = blockIdx.y;
+  const int cb = c0 + c * TB_CW;
+  const int row = rbeg + blockIdx.x * 256 + threadIdx.x;
+  const bool active = row < rend;
+  const T* mr = M + row + (int64_t)cb * ldm;
+  T m[32];
+  TV a0[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) a0[q] = TV(0);
+  if (active) {
+#pragma unroll
+    for (int j = 0; j < 32; ++j) m[j] = mr[(int64_t)j * ldm];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) a0[q] = acc[((size_t)q * TB_CH + c) * N + row];
+  }
+  if (threadIdx.x < TB_CW) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      TV sv = TV(0);
+#pragma unroll
+      for (int g = 0; g < TB_CH; ++g) sv += xpart[((size_t)q * TB_CH + g) * N + cb + threadIdx.x];
+      xs[q][threadIdx.x] = sv;
+      if (blockIdx.x == 0) x[(size_t)q * N + cb + threadIdx.x] = sv;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  TV a[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    a[q] = TV(0);
+#pragma unroll
+    for (int j = 0; j < 32; ++j) a[q] = __builtin_fma((TV)m[j], xs[q][j], a[q]);
+  }
+#pragma unroll
+  for (int j = 0; j < 32; ++j) m[j] = mr[(int64_t)(32 + j) * ldm];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+#pragma unroll
+    for (int j = 0; j < 32; ++j) a[q] = __builtin_fma((TV)m[j], xs[q][32 + j], a[q]);
+    acc[((size_t)q * TB_CH + c) * N + row] = a0[q] + a[q];
+  }
+}
+
+template <int TBW, typename T, typename TV, int NR>
+static int trsv_multi_w(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
+                        const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc) {
+  constexpr int TB_CH = TBW / TB_CW;
+  MPDOK_REQUIRE(N % TBW == 0, "trsv_multi: N must be a multiple of %d", TBW);
+  const int nb = N / TBW;
+  TV* xpart = acc + (size_t)TB_CH * N * NR;
+  const dim3 dgrid(TBW / 64, TB_CH);
+  MPDOK_HIP(hipMemsetAsync(acc, 0, sizeof(TV) * (size_t)TB_CH * N * NR, s));
+  for (int i = 0; i < nb; ++i) {
+    const int r0 = i * TBW;
+    diag_apply_multi_kernel<TBW, T, TV, NR><<<dgrid, 256, 0, s>>>(
+        Lbinv + (size_t)i * TBW * TBW, r0, b, acc, N, xpart, 1);
+    tri_acc_multi_kernel<TBW, T, TV, NR>
+        <<<dim3(r0 + TBW < N ? (N - r0 - TBW + 255) / 256 : 1, TB_CH), 256, 0, s>>>(
+            LU, ld, r0 + TBW, N, r0, xpart, v, acc, N);
+  }
+  MPDOK_HIP(hipMemsetAsync(acc, 0, sizeof(TV) * (size_t)TB_CH * N * NR, s));
+  for (int i = nb - 1; i >= 0; --i) {
+    const int r0 = i * TBW;
+    diag_apply_multi_kernel<TBW, T, TV, NR><<<dgrid, 256, 0, s>>>(
+        Ubinv + (size_t)i * TBW * TBW, r0, v, acc, N, xpart, 0);
+    tri_acc_multi_kernel<TBW, T, TV, NR><<<dim3(r0 > 0 ? (r0 + 255) / 256 : 1, TB_CH), 256, 0, s>>>(
+        LU, ld, 0, r0, r0, xpart, x, acc, N);
+  }
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+template <typename T, typename TV>
+int trsv_block_inverse_multi(hipStream_t s, const T* LU, int64_t ld, int N, const T* Lbinv,
+                             const T* Ubinv, const TV* b, TV* x, TV* v, TV* acc, int tbw,
+                             int nr) {
+#define TRSV_M(W, R) return trsv_multi_w<W, T, TV, R>(s, LU, ld, N, Lbinv, Ubinv, b, x, v, acc)
+  MPDOK_REQUIRE(tbw == TRSV_BLOCK || tbw == 2 * TRSV_BLOCK, "trsv_multi: block width %d", tbw);
+  MPDOK_REQUIRE(nr == 1 || nr == 2 || nr == 4 || nr == 8, "trsv_multi: %d right-hand sides", nr);
+  if (tbw == 2 * TRSV_BLOCK) {
+    switch (nr) {
+      case 8: TRSV_M(2 * TRSV_BLOCK, 8);
+      case 4: TRSV_M(2 * TRSV_BLOCK, 4);
+      case 2: TRSV_M(2 * TRSV_BLOCK, 2);
+      default: TRSV_M(2 * TRSV_BLOCK, 1);
+    }
+  }
+  switch (nr) {
+    case 8: TRSV_M(TRSV_BLOCK, 8);
+    case 4: TRSV_M(TRSV_BLOCK, 4);
+    case 2: TRSV_M(TRSV_BLOCK, 2);
+    default: TRSV_M(TRSV_BLOCK, 1);
+  }
+#undef TRSV_M
+}
+template int trsv_block_inverse_multi<double, double>(hipStream_t, const double*, int64_t, int,
+                                                      const double*, const double*, const double*,
+                                                      double*, double*, double*, int, int);
+template int trsv_block_inverse_multi<float, float>(hipStream_t, const float*, int64_t, int,
+                                                    const float*, const float*, const float*,
+                                                    float*, float*, float*, int, int);
+
 // 1024-wide diagonal blocks of the inverses from the 512-wide ones: block b
 // takes inv(A) = in[2b] top-left and inv(D) = in[2b + 1] bottom-right; the
 // off-diagonal block of the zero triangle (top-right for a lower inverse,

@@ -285,6 +285,92 @@ static int lu_ir_impl(hipStream_t s, const double* A64, int64_t lda, int order, 
   return OK;
 }
 
+// LU-IR for nrhs right-hand sides (column-major B, X with ldb, ldx >= N), in
+// blocks of up to 8: one residual pass over A (mpdok_residual_multi_f64) and
+// one multi-right-hand-side solve per outer step serve the whole block. Each
+// column runs the single loop above (x = 0, first residual = b, stop when its
+// relres < tol, no update after that), so its iterates, relres and result are
+// bit for bit those of lu_ir_impl. A row-major A takes the single loop per
+// column. converged / relres: host arrays of nrhs (the last relres checked).
+static int lu_ir_multi_impl(hipStream_t s, const double* A64, int64_t lda, int order, LUHandle* M,
+                            int N, const double* B, int64_t ldb, double* X, int64_t ldx, int nrhs,
+                            double tol, int maxiter_outer, int resid_mode, int* converged,
+                            double* relres) {
+  if (order != COL_MAJOR) {
+    for (int q = 0; q < nrhs; ++q) {
+      IRResult res;
+      MPDOK_TRY(lu_ir_impl(s, A64, lda, order, M, N, B + (int64_t)q * ldb, X + (int64_t)q * ldx,
+                           tol, maxiter_outer, resid_mode, res));
+      converged[q] = res.converged;
+      relres[q] = res.hist.empty() ? NAN : res.hist.back();
+    }
+    return OK;
+  }
+  constexpr int NB = 8;
+  static thread_local DevBuf Rb, Db, Wr, dots;
+  MPDOK_TRY(Rb.get(sizeof(double) * (size_t)N * NB));
+  MPDOK_TRY(Db.get(sizeof(double) * (size_t)N * NB));
+  MPDOK_TRY(Wr.get(resid_multi_workspace_bytes(N) + 64));
+  double* R = static_cast<double*>(Rb.p);
+  double* Dm = static_cast<double*>(Db.p);
+  for (int c0 = 0; c0 < nrhs; c0 += NB) {
+    const int nr = std::min(NB, nrhs - c0);
+    const double* Bc = B + (int64_t)c0 * ldb;
+    double* Xc = X + (int64_t)c0 * ldx;
+    double bn[NB], rn[NB];
+    bool active[NB];
+    int nact = 0;
+    for (int q = 0; q < nr; ++q) {
+      MPDOK_TRY(host_norm(s, Bc + (int64_t)q * ldb, N, dots, &bn[q]));
+      MPDOK_HIP(hipMemsetAsync(Xc + (int64_t)q * ldx, 0, sizeof(double) * N, s));
+      active[q] = bn[q] != 0.0;
+      converged[c0 + q] = active[q] ? 0 : 1;
+      relres[c0 + q] = 0.0;
+      nact += active[q];
+    }
+    for (int it = 0; it < maxiter_outer && nact > 0; ++it) {
+      if (it == 0) {
+        // x = 0: the residual is b exactly, without a pass over A
+        MPDOK_TRY((copy2d<double, double>(s, Bc, ldb, R, N, N, nr)));
+        for (int q = 0; q < nr; ++q) rn[q] = bn[q];
+      } else {
+        MPDOK_TRY(residual_multi_f64(s, A64, lda, N, Xc, ldx, Bc, ldb, R, N, nr, resid_mode, Wr.p,
+                                     Wr.n));
+        for (int q = 0; q < nr; ++q)
+          if (active[q]) MPDOK_TRY(host_norm(s, R + (size_t)q * N, N, dots, &rn[q]));
+      }
+      for (int q = 0; q < nr; ++q) {
+        if (!active[q]) continue;
+        relres[c0 + q] = rn[q] / bn[q];
+        if (rn[q] / bn[q] < tol) {
+          converged[c0 + q] = 1;
+          active[q] = false;
+          --nact;
+        }
+      }
+      if (nact == 0) break;
+      MPDOK_TRY(lu_solve_multi(M, s, R, N, Dm, N, nr));
+      for (int q = 0; q < nr; ++q)
+        if (active[q])
+          MPDOK_TRY(axpby(s, N, 1.0, Xc + (int64_t)q * ldx, F64, 1.0, Dm + (size_t)q * N, F64,
+                          Xc + (int64_t)q * ldx, F64));
+    }
+    if (nact > 0) {
+      // out of outer steps: the last update's residual decides, as in lu_ir_impl
+      MPDOK_TRY(residual_multi_f64(s, A64, lda, N, Xc, ldx, Bc, ldb, R, N, nr, resid_mode, Wr.p,
+                                   Wr.n));
+      for (int q = 0; q < nr; ++q) {
+        if (!active[q]) continue;
+        MPDOK_TRY(host_norm(s, R + (size_t)q * N, N, dots, &rn[q]));
+        relres[c0 + q] = rn[q] / bn[q];
+        converged[c0 + q] = rn[q] / bn[q] < tol;
+      }
+    }
+  }
+  MPDOK_HIP(hipStreamSynchronize(s));
+  return OK;
+}
+
 // ---------------------------------------------------------------------------
 // State of the reference-compatible symbols (one slot each, like the natives).
 // ---------------------------------------------------------------------------
@@ -344,6 +430,19 @@ MPDOK_API int mpdok_lu_ir(hipStream_t s, const double* A64, int64_t lda, int ord
                        maxiter_outer, resid_mode, r));
   if (converged) *converged = r.converged;
   return fill_hist(r, hist, hist_cap, n_hist);
+}
+
+MPDOK_API int mpdok_lu_ir_multi(hipStream_t s, const double* A64, int64_t lda, int order, void* lu,
+                                int N, const double* B, int64_t ldb, double* X, int64_t ldx,
+                                int nrhs, double tol, int maxiter_outer, int resid_mode,
+                                int* converged, double* relres) {
+  MPDOK_REQUIRE(lu != nullptr, "lu_ir_multi: null factorization handle");
+  MPDOK_REQUIRE(A64 && B && X && converged && relres && nrhs >= 0 && N >= 1,
+                "lu_ir_multi: bad arguments");
+  MPDOK_REQUIRE(ldb >= N && ldx >= N, "lu_ir_multi: leading dimension < N");
+  MPDOK_REQUIRE(lu_n(static_cast<LUHandle*>(lu)) == N, "lu_ir_multi: handle holds another N");
+  return lu_ir_multi_impl(s, A64, lda, order, static_cast<LUHandle*>(lu), N, B, ldb, X, ldx, nrhs,
+                          tol, maxiter_outer, resid_mode, converged, relres);
 }
 
 // ---- reference-compatible symbols (void, out-pointers, synchronous) -------
@@ -507,19 +606,18 @@ MPDOK_API int mpdok_batched_rbf_solve(int nsys, const int* N, const double* cons
         MPDOK_TRY(lu_resize(h, n));
         int info = 0;
         MPDOK_TRY(lu_factor(h, s, A, n, COL_MAJOR, &info));
-        for (int r = 0; r < nrhs; ++r) {
-          const int k = i * nrhs + r;
-          if (info != 0) {
-            converged[k] = 0;
-            relres[k] = NAN;
-            continue;
+        if (info != 0) {
+          for (int r = 0; r < nrhs; ++r) {
+            converged[i * nrhs + r] = 0;
+            relres[i * nrhs + r] = NAN;
           }
-          IRResult res;
-          MPDOK_TRY(lu_ir_impl(s, A, n, COL_MAJOR, h, n, B[i] + (size_t)r * n,
-                               X[i] + (size_t)r * n, tol, maxiter_outer, 1, res));
-          converged[k] = res.converged;
-          relres[k] = res.hist.empty() ? NAN : res.hist.back();
+          continue;
         }
+        // all right-hand sides of the system in one block: the matrix and
+        // the factors are read once per outer step for all of them
+        MPDOK_TRY(lu_ir_multi_impl(s, A, n, COL_MAJOR, h, n, B[i], n, X[i], n, nrhs, tol,
+                                   maxiter_outer, 1, converged + (size_t)i * nrhs,
+                                   relres + (size_t)i * nrhs));
       }
       MPDOK_HIP(hipStreamSynchronize(s));
       return OK;

@@ -339,10 +339,16 @@ class LUIRSolver:
         return x_gpu, conv.value
 
     def solve_factored(self, b, tol=1e-11, maxiter_outer=3):
-        """Solve with the factors of the last factor() call (O(N^2) per step)."""
+        """Solve with the factors of the last factor() call (O(N^2) per step).
+
+        b may also be an (N, k) block of right-hand sides (extension): they
+        are refined together (mpdok_lu_ir_multi), each column exactly as a
+        call with that column alone, and X comes back as an (N, k) tensor."""
         if not hasattr(self, "_factored_A"):
             raise RuntimeError("Call factor(A) before solve_factored(b).")
         A_gpu = self._factored_A
+        if getattr(b, "ndim", 1) == 2:
+            return self._refine_multi(A_gpu, b, tol, maxiter_outer)
         b_gpu = _dev.as_device(b, torch.float64)
         x_gpu, conv = self._refine(A_gpu, b_gpu, tol, maxiter_outer)
         if not conv:
@@ -351,6 +357,29 @@ class LUIRSolver:
                 f"within {maxiter_outer} outer iterations.",
                 RuntimeWarning, stacklevel=2)
         return x_gpu
+
+    def _refine_multi(self, A_gpu, B, tol, maxiter_outer):
+        N = int(A_gpu.shape[0])
+        B_gpu = _dev.as_device(B, torch.float64, "F")
+        if B_gpu.shape[0] != N:
+            raise ValueError(f"b must be (N,) or (N, k), got {tuple(B_gpu.shape)}")
+        k = int(B_gpu.shape[1])
+        X = _dev.empty((N, k), torch.float64, "F")
+        order, ld = _dev.layout(A_gpu)
+        conv = (ctypes.c_int * max(k, 1))()
+        rel = (ctypes.c_double * max(k, 1))()
+        check(self._lib.mpdok_lu_ir_multi(_dev.stream(), A_gpu.data_ptr(), ld, order,
+                                          self._handle.ptr, N, B_gpu.data_ptr(), B_gpu.stride(1),
+                                          X.data_ptr(), X.stride(1), k, float(tol),
+                                          int(maxiter_outer), RESIDUALS[self.residual], conv, rel),
+              "lu_ir_multi", from_lib=self._lib)
+        self.last_info = {"converged": [bool(c) for c in conv[:k]], "relres": list(rel[:k])}
+        if not all(self.last_info["converged"]):
+            warnings.warn(
+                f"LU-IR solve_factored did not converge to tol={tol:.1e} "
+                f"within {maxiter_outer} outer iterations.",
+                RuntimeWarning, stacklevel=3)
+        return X
 
     def _drop_handle(self):
         if self._handle is not None:

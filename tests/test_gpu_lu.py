@@ -598,3 +598,87 @@ def test_cpu_sized_2d_case_int8_lu_gmres_ir_and_reference_lu_ir(L):
     assert np.linalg.norm(x - xl) / np.linalg.norm(xl) <= 10 * max(err64, 1e-15)
     with pytest.warns(RuntimeWarning, match="did not converge"):
         LUIRSolver().solve(A, b, tol=1e-12, maxiter_outer=10)
+
+
+def _handle(L, N, precision, pivot, nb=512, slices=5):
+    h = ctypes.c_void_p()
+    assert L.mpdok_lu_create(N, precision, pivot, nb, slices, ctypes.byref(h)) == 0
+    return h
+
+
+@pytest.mark.parametrize("N,precision,pivot", [(2048, 4, 0), (1536, 1, 1), (1000, 0, 0),
+                                               (3072, 0, 1)])
+def test_lu_solve_multi_equals_single_solves(L, N, precision, pivot):
+    """mpdok_lu_solve_multi: every column bit for bit the single solve's, for
+    the 1024-wide (N = 2048, 3072) and 512-wide (1536) inverse-based solves,
+    pivoted or not, and for the per-column fallback (N = 1000); 11 columns
+    exercise the blocks of 8, 2 and 1."""
+    rng = np.random.default_rng(N + precision)
+    X = rng.uniform(0, 1, (N, 2))
+    A = orb.build_kernel(X, "gaussian", 10.0, 1e-3) + (0 if not pivot else
+                                                      0.01 * rng.standard_normal((N, N)))
+    h = _handle(L, N, precision, pivot)
+    try:
+        info = ctypes.c_int(-1)
+        Ad = dev(A)
+        assert L.mpdok_lu_factor(h, stream(), Ad.data_ptr(), N, 1, ctypes.byref(info)) == 0
+        assert info.value == 0
+        k = 11
+        B = dev(rng.standard_normal((N, k)))
+        Xm = torch.full((N, k), float("nan"), dtype=torch.float64, device="cuda").t().contiguous().t()
+        st = L.mpdok_lu_solve_multi(h, stream(), B.data_ptr(), N, Xm.data_ptr(), N, k)
+        assert st == 0, L.mpdok_last_error()
+        for q in range(k):
+            xs = torch.empty(N, dtype=torch.float64, device="cuda")
+            assert L.mpdok_lu_solve(h, stream(), B[:, q].contiguous().data_ptr(), xs.data_ptr()) == 0
+            torch.cuda.synchronize()
+            assert torch.equal(Xm[:, q], xs), q
+    finally:
+        L.mpdok_lu_destroy(h)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_residual_multi_equals_single_residuals(L, mode):
+    N, k = 1500, 13
+    rng = np.random.default_rng(mode)
+    A = dev(rng.standard_normal((N, N)))
+    Xv = dev(rng.standard_normal((N, k)))
+    Bv = dev(rng.standard_normal((N, k)))
+    R = torch.empty((k, N), dtype=torch.float64, device="cuda").t()
+    nb = L.mpdok_residual_multi_workspace_bytes(N)
+    ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    st = L.mpdok_residual_multi_f64(stream(), A.data_ptr(), N, N, Xv.data_ptr(), N, Bv.data_ptr(), N,
+                                    R.data_ptr(), N, k, mode, ws.data_ptr(), nb)
+    assert st == 0, L.mpdok_last_error()
+    nb1 = L.mpdok_residual_workspace_bytes(N, 1)
+    ws1 = torch.empty(nb1, dtype=torch.uint8, device="cuda")
+    for q in range(k):
+        r = torch.empty(N, dtype=torch.float64, device="cuda")
+        x = Xv[:, q].contiguous()
+        b = Bv[:, q].contiguous()
+        assert L.mpdok_residual_f64(stream(), A.data_ptr(), N, 1, N, x.data_ptr(), b.data_ptr(),
+                                    r.data_ptr(), mode, ws1.data_ptr(), nb1) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(R[:, q], r), q
+
+
+@pytest.mark.parametrize("precision,pivot,residual", [("fp32", "partial", "fp64"),
+                                                      ("ozaki", "none", "dot2")])
+def test_solve_factored_block_equals_column_by_column(L, precision, pivot, residual):
+    """Factor once, then a block of 10 right-hand sides (the weather-front time
+    series of rbf_kernel.py:230-252): the block equals ten single
+    solve_factored calls bit for bit and every column reaches the tolerance."""
+    from mpdok_amd.mpdok_ops import LUIRSolver
+    N = 2048
+    X = orb.synthetic_coords(N, seed=3)
+    A = orb.build_kernel(X, "gaussian", 1.5e-4, 1e-2)
+    B = np.stack([orb.weather_front(X, t) for t in range(10)], axis=1)
+    s = LUIRSolver(precision=precision, pivot=pivot, residual=residual)
+    s.factor(A)
+    Xb = s.solve_factored(B, tol=1e-11, maxiter_outer=5)
+    assert tuple(Xb.shape) == (N, 10) and all(s.last_info["converged"])
+    for t in range(10):
+        xt = s.solve_factored(B[:, t], tol=1e-11, maxiter_outer=5)
+        assert torch.equal(Xb[:, t], xt), t
+        assert np.linalg.norm(B[:, t] - A @ xt.cpu().numpy()) / np.linalg.norm(B[:, t]) < 1e-11
+    s.free_factored()
