@@ -443,6 +443,38 @@ def other_configs(log):
     del A, s, lu
     torch.cuda.empty_cache()
 
+    # the precision ladder on the headline system (N = 32768, 3-D, gamma 25,
+    # nugget 1e-6): LU preconditioners of each precision, GMRES-IR with the
+    # FP64 body and Dot2 residual (and the FP32 body with the int8-sliced LU)
+    N = 32768
+    X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
+    b = _dev.as_device(rhs_of(X), torch.float64)
+    A = _dev.empty((N, N), torch.float64, "F")
+    _ops.rbf_assemble(_dev.as_device(X, torch.float64), 0, 25.0, 1e-6, A)
+    ladder = {}
+    for prec, body in [("ozaki", "fp64"), ("ozaki", "fp32"), ("fp64", "fp64"), ("fp32", "fp64"),
+                       ("bf16", "fp64"), ("fp16", "fp64"), ("fp8", "fp64")]:
+        s = MPDOKSolver()
+        s.profile = True
+        kwl = dict(tol=1e-12, maxiter_outer=4, restart=50, precond="lu", factor_precision=prec,
+                   slices=5, pivot="none", body=body, residual="dot2", nb=512)
+        _, tl = timed(lambda: s.solve(A, b, **kwl))
+        st = s.last_info["factor_stats"] or {}
+        ladder[f"{prec}_lu_{body}_body"] = {
+            "s": tl, "converged": s.last_info["converged"],
+            "inner_iterations": s.last_info["inner_iterations"],
+            "history": [float(h) for h in s.last_info["history"]],
+            "trailing_update_ms": st.get("trailing_ms"),
+            "trailing_update_TFLOPs": (st["trailing_flops"] / st["trailing_ms"] / 1e9
+                                       if st.get("trailing_ms") else None)}
+        s.free_fp32_buf()
+        del s
+        torch.cuda.empty_cache()
+    res["precision_ladder_n32768"] = ladder
+    log(f"[configs] ladder {ladder}")
+    del A
+    torch.cuda.empty_cache()
+
     # factor once, then 72 right-hand sides (the reference's weather-front time
     # series, run_demo.py:75-109; rbf_demo.ipynb:247): the reference-default
     # LUIRSolver (FP32 LU with partial pivoting, FP64 refinement) at N = 8192

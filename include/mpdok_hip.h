@@ -250,8 +250,10 @@ int mpdok_diag_scale(hipStream_t s, int dtype, const void* d, const void* in, vo
 /* ---- LU factorisation ------------------------------------------------------
  * Opaque handle replacing the global factor slot of lu_ir.so
  * (mpdok_ops.py:122-141,245-247). precision: 0 = FP64, 1 = FP32, 2 = BF16
- * operands / FP32 accumulation, 3 = FP16 operands / FP32 accumulation,
- * 4 = int8-sliced trailing updates with `slices` digits and FP64 storage.
+ * trailing updates on the BF16 matrix cores (FP32 storage and accumulation),
+ * 3 = the same in FP16, 4 = int8-sliced trailing updates with `slices` digits
+ * and FP64 storage, 5 = FP8 (E4M3) operands with one power-of-two scale per
+ * operand of each update (the precision ladder's FP8 experiment).
  * pivot: 0 = none (SPD / diagonally dominant), 1 = partial (getrf).
  * nb: outer block (multiple of 64; 0 = 512). */
 int mpdok_lu_create(int N, int precision, int pivot, int nb, int slices, void** handle);
@@ -321,8 +323,11 @@ int mpdok_lu_set_update_panels(void* handle, int panels, int panels_early, int e
  * permutation ((P b)[i] = b[perm[i]], NULL without pivoting). */
 int mpdok_lu_factors(void* handle, void** LU, int** perm, int* elem_size);
 /* C = alpha A B + beta C on the matrix cores (column-major); dtype 0 FP64,
- * 1 FP32; round 0/1/2 = none/BF16/FP16 operands (FP32 only). The trailing
- * update of the factorisation (K13 in place of cuSOLVER's internal GEMM). */
+ * 1 FP32; round (FP32 only) 0 = FP32 products, 1 = BF16 operands on the BF16
+ * matrix cores, 2 = FP16, 3 = E4M3 values after one power-of-two scale per
+ * operand (on the BF16 matrix cores, where E4M3 values are exact); FP32
+ * accumulation. The trailing update of the factorisation (in place of
+ * cuSOLVER's internal GEMM). */
 int mpdok_gemm(hipStream_t s, int dtype, int m, int n, int k, double alpha, const void* A,
                int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc,
                int round);

@@ -7,6 +7,13 @@ namespace mpdok {
 template <typename T>
 int gemm(hipStream_t s, int m, int n, int k, T alpha, const T* A, int64_t lda, const T* B,
          int64_t ldb, T beta, T* C, int64_t ldc, int round);
+// C = alpha A B + beta C with FP32 operands rounded to BF16 (round 1), FP16
+// (2) or scaled E4M3 (3; scales: gemm_lowp_scale_floats() device floats,
+// the two scales first) on the matrix cores
+size_t gemm_lowp_scale_floats();
+int gemm_lowp(hipStream_t s, int m, int n, int k, float alpha, const float* A, int64_t lda,
+              const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int round,
+              float* scales);
 // One product C = alpha A B + beta C (column-major) of a grouped launch.
 template <typename T>
 struct GemmArgs {
@@ -80,7 +87,9 @@ int ozaki_apply(hipStream_t s, int S, int m, int n, int K, int col0, int ncols, 
                 int64_t ldc, void* work);
 
 // lu.hip
-enum LUPrecision : int { LU_FP64 = 0, LU_FP32 = 1, LU_BF16 = 2, LU_FP16 = 3, LU_OZAKI = 4 };
+enum LUPrecision : int {
+  LU_FP64 = 0, LU_FP32 = 1, LU_BF16 = 2, LU_FP16 = 3, LU_OZAKI = 4, LU_FP8 = 5
+};
 enum LUPivot : int { PIV_NONE = 0, PIV_PARTIAL = 1 };
 
 struct LUHandle;

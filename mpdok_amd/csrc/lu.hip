@@ -27,6 +27,8 @@
 #include <vector>
 #include "common.h"
 #include "gemv.h"
+#include <type_traits>
+
 #include "lu.h"
 
 namespace mpdok {
@@ -94,6 +96,7 @@ struct LUHandle {
   // stream updates the group's rows before the rest of its trailing update)
   int rows_side_min = 0;
   int* d_S = nullptr;
+  float* d_sc = nullptr;  // FP8 operand scales of the running trailing update
   std::vector<int> steps_S;  // host copy after a factorisation
   // optional HIP-event timing of the trailing-update launches
   bool profile = false;
@@ -130,7 +133,7 @@ void lu_destroy(LUHandle* h) {
                   h->piv_work, h->oz_work, h->oz_work2, h->v1, h->v2, h->Lbinv, h->Ubinv, h->v3,
                   h->Lbinv2, h->Ubinv2,
                   h->acc, h->mv, h->macc,
-                  h->d_S};
+                  h->d_S, h->d_sc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -140,7 +143,7 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
   MPDOK_REQUIRE(out != nullptr, "lu_create: null out");
   *out = nullptr;
   MPDOK_REQUIRE(N >= 1, "lu_create: N must be >= 1");
-  MPDOK_REQUIRE(precision >= LU_FP64 && precision <= LU_OZAKI, "lu_create: bad precision %d",
+  MPDOK_REQUIRE(precision >= LU_FP64 && precision <= LU_FP8, "lu_create: bad precision %d",
                 precision);
   MPDOK_REQUIRE(pivot == PIV_NONE || pivot == PIV_PARTIAL, "lu_create: bad pivot %d", pivot);
   if (nb <= 0) nb = 512;
@@ -184,6 +187,7 @@ int lu_create(int N, int precision, int pivot, int nb, int slices, LUHandle** ou
     alloc(&h->oz_work2, ozaki_workspace_bytes(N, N, LUHandle::MAX_GROUP * nb, slices));
     alloc((void**)&h->d_S, sizeof(int) * (N / nb + 3));
   }
+  if (precision == LU_FP8) alloc((void**)&h->d_sc, gemm_lowp_scale_floats() * sizeof(float));
   h->adaptive = precision == LU_OZAKI && slices <= 5;
   // row lookahead while the trailing matrix has at least 20480 rows:
   // measured 175 -> 172 ms for the N = 32768 factorisation (the early
@@ -472,7 +476,10 @@ static int trailing_block(LUHandle* h, hipStream_t s, int k, int kb, int r_off, 
   T* A = static_cast<T*>(h->LU);
   const int64_t ld = N;
   auto at = [&](int i, int j) { return A + i + (int64_t)j * ld; };
-  const int round = h->precision == LU_BF16 ? 1 : (h->precision == LU_FP16 ? 2 : 0);
+  const int round = h->precision == LU_BF16 ? 1
+                    : h->precision == LU_FP16 ? 2
+                    : h->precision == LU_FP8  ? 3
+                                              : 0;
   const size_t nev = h->ev_ops.size();
   if (h->profile) {
     // int8-sliced: S(S+1)/2 int8 products of 2 m n k operations each
@@ -493,9 +500,15 @@ static int trailing_block(LUHandle* h, hipStream_t s, int k, int kb, int r_off, 
     MPDOK_REQUIRE(r_off == 0 && nrows == m3, "trailing update: row ranges need adaptive slicing");
     MPDOK_TRY(ozaki_apply(s, h->slices, m3, n3, kb, c_off, ncols,
                           reinterpret_cast<double*>(at(k + kb, k + kb)), ld, oz_ws(h, k)));
+  } else if (round > 0) {
+    // BF16 / FP16 / scaled FP8 operands on the matrix cores, FP32 accumulation
+    if constexpr (std::is_same<T, float>::value)
+      MPDOK_TRY(gemm_lowp(s, nrows, ncols, kb, -1.f, at(k + kb + r_off, k), ld,
+                          at(k, k + kb + c_off), ld, 1.f, at(k + kb + r_off, k + kb + c_off), ld,
+                          round, h->d_sc));
   } else {
     MPDOK_TRY(gemm<T>(s, nrows, ncols, kb, T(-1), at(k + kb + r_off, k), ld, at(k, k + kb + c_off),
-                      ld, T(1), at(k + kb + r_off, k + kb + c_off), ld, round));
+                      ld, T(1), at(k + kb + r_off, k + kb + c_off), ld, 0));
   }
   if (h->profile) MPDOK_HIP(hipEventRecord(h->ev[2 * nev + 1], s));
   return OK;
@@ -955,8 +968,15 @@ MPDOK_API int mpdok_gemm(hipStream_t s, int dtype, int m, int n, int k, double a
   if (dtype == mpdok::F64)
     return mpdok::gemm<double>(s, m, n, k, alpha, (const double*)A, lda, (const double*)B, ldb,
                                beta, (double*)C, ldc, 0);
+  if (round >= 1 && round <= 3) {
+    static float* sc = nullptr;  // the FP8 scales of this entry point's launches
+    if (round == 3 && sc == nullptr)
+      MPDOK_HIP(hipMalloc(&sc, mpdok::gemm_lowp_scale_floats() * sizeof(float)));
+    return mpdok::gemm_lowp(s, m, n, k, (float)alpha, (const float*)A, lda, (const float*)B, ldb,
+                            (float)beta, (float*)C, ldc, round, sc);
+  }
   return mpdok::gemm<float>(s, m, n, k, (float)alpha, (const float*)A, lda, (const float*)B, ldb,
-                            (float)beta, (float*)C, ldc, round);
+                            (float)beta, (float*)C, ldc, 0);
 }
 
 // Adaptive slice counts of the int8-sliced variant (default on for slices

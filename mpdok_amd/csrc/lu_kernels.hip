@@ -18,6 +18,8 @@
 // Replaces cusolverDnSgetrf / cusolverDnSgetrs inside lu_ir.so
 // (benchmark_mpdok.ipynb:989-992, mpdok_ops.py:72) and cp.linalg.solve's FP64
 // getrf/getrs for the plain comparator (dense_krylov.py:526,608,681).
+#include <type_traits>
+
 #include "common.h"
 #include "lu.h"
 
@@ -205,6 +207,229 @@ static void gemm_launch(hipStream_t s, dim3 grid, int round, int m, int n, int k
     gemm_kernel<T, 1, TB><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
   else
     gemm_kernel<T, 2, TB><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sA, sB, sC);
+}
+
+// ---------------------------------------------------------------------------
+// Low-precision operands on the matrix cores: C = alpha A B + beta C for FP32
+// column-major operands whose values are rounded on their way into LDS to
+// ROUND 1 = BF16, 2 = FP16 (v_mfma_f32_32x32x16_bf16 / _f16, FP32
+// accumulation), or 3 = FP8 E4M3 after a power-of-two scale per operand
+// (scales[0] for A, scales[1] for B; one scale per panel), multiplied on the
+// BF16 matrix cores: an E4M3 value is exact in BF16, so the products are those
+// of FP8 matrix cores, which would run them at twice the rate. The result is
+// scaled back by 1 / (scales[0] scales[1]) (exact, powers of two).
+//
+// 128 x 128 block tile, 2 x 2 wavefronts of 64 x 64 (four 32 x 32 MFMA tiles
+// each), K step 32, LDS double buffer. The MFMA takes the B tile first, so
+// C's rows run along the lanes: lane l, register r of MFMA tile (ta, tb)
+// holds row bm + wm + 32 ta + l % 32, column bn + wn + 32 tb + 8 (r / 4) +
+// 4 (l / 32) + r % 4, and the read-modify-write of C is coalesced. Both
+// operands give lane l the same eight k of its half (k group l / 32), which is
+// all the product needs.
+// ---------------------------------------------------------------------------
+typedef __bf16 v8bf16 __attribute__((ext_vector_type(8)));
+typedef _Float16 v8f16 __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// Nearest E4M3 value (3 fraction bits, smallest normal 2^-6, subnormal step
+// 2^-9, largest 448; larger magnitudes saturate), ties to even.
+__device__ __forceinline__ float round_e4m3(float v) {
+  const float a = fabsf(v);
+  if (!(a > 0.f)) return v;
+  int e;
+  frexpf(a, &e);                       // a = f 2^e, f in [0.5, 1)
+  const int q = e - 4 > -9 ? e - 4 : -9;  // quantum 2^(e-1-3), or the subnormal step
+  float r = ldexpf(rintf(ldexpf(a, -q)), q);
+  if (r > 448.f) r = 448.f;
+  return copysignf(r, v);
+}
+
+template <int ROUND>
+__device__ __forceinline__ auto to_low(float v, float sc) {
+  if constexpr (ROUND == 2) return (_Float16)v;
+  else if constexpr (ROUND == 3) return (__bf16)round_e4m3(v * sc);
+  else return (__bf16)v;
+}
+
+template <int ROUND>
+__global__ __launch_bounds__(256, 2) void gemm_lowp_kernel(int m, int n, int k, float alpha,
+                                                           const float* __restrict__ A,
+                                                           int64_t lda,
+                                                           const float* __restrict__ B,
+                                                           int64_t ldb, float beta,
+                                                           float* __restrict__ C, int64_t ldc,
+                                                           const float* __restrict__ scales) {
+  constexpr int TB = 128, BK = 32, AS = BK + 8;  // AS: row stride in elements (80 bytes)
+  typedef typename std::conditional<ROUND == 2, _Float16, __bf16>::type E;
+  typedef typename std::conditional<ROUND == 2, v8f16, v8bf16>::type V8;
+  __shared__ __attribute__((aligned(16))) E sA[2][TB][AS];
+  __shared__ __attribute__((aligned(16))) E sB[2][TB][AS];
+  int ti, tj;
+  if (!tile_of(blockIdx.x, (m + TB - 1) / TB, (n + TB - 1) / TB, ti, tj)) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int bm = ti * TB, bn = tj * TB;
+  const float sa = ROUND == 3 ? scales[0] : 1.f, sb = ROUND == 3 ? scales[1] : 1.f;
+  v16f acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v16f(0);
+  // A tile: thread t loads row t % 128 at k = 16 (t / 128) .. + 15 (each load
+  // instruction: 64 consecutive rows of one column); B tile: column t / 2 at
+  // k = 16 (t % 2) .. + 15 (contiguous). Both go to LDS as 16 consecutive k
+  // of one row / column: two 16-byte stores.
+  const int ar = tid & 127, akh = 16 * (tid >> 7);
+  const int bc = tid >> 1, bkh = 16 * (tid & 1);
+  float ra[16], rb[16];
+  const bool inside = bm + TB <= m && bn + TB <= n;
+  auto gload = [&](int k0) {
+    const int i = bm + ar, j = bn + bc;
+    if (inside && k0 + BK <= k) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) ra[e] = A[i + (int64_t)(k0 + akh + e) * lda];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) rb[e] = B[(k0 + bkh + e) + (int64_t)j * ldb];
+      return;
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int kk = k0 + akh + e;
+      ra[e] = (i < m && kk < k) ? A[i + (int64_t)kk * lda] : 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int kk = k0 + bkh + e;
+      rb[e] = (j < n && kk < k) ? B[kk + (int64_t)j * ldb] : 0.f;
+    }
+  };
+  auto sstore = [&](int buf) {
+    V8 a0, a1, b0, b1;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      a0[e] = to_low<ROUND>(ra[e], sa);
+      a1[e] = to_low<ROUND>(ra[8 + e], sa);
+      b0[e] = to_low<ROUND>(rb[e], sb);
+      b1[e] = to_low<ROUND>(rb[8 + e], sb);
+    }
+    *reinterpret_cast<V8*>(&sA[buf][ar][akh]) = a0;
+    *reinterpret_cast<V8*>(&sA[buf][ar][akh + 8]) = a1;
+    *reinterpret_cast<V8*>(&sB[buf][bc][bkh]) = b0;
+    *reinterpret_cast<V8*>(&sB[buf][bc][bkh + 8]) = b1;
+  };
+  const int nk = (k + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int r32 = lane & 31, u = lane >> 5;
+  for (int t = 0; t < nk; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < nk) gload((t + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 16) {
+      V8 fa[2], fb[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        fa[q] = *reinterpret_cast<const V8*>(&sA[buf][wm + 32 * q + r32][ks + 8 * u]);
+        fb[q] = *reinterpret_cast<const V8*>(&sB[buf][wn + 32 * q + r32][ks + 8 * u]);
+      }
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) {
+          if constexpr (ROUND == 2)
+            acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fb[tb], fa[ta], acc[ta][tb], 0, 0, 0);
+          else
+            acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[tb], fa[ta], acc[ta][tb], 0, 0, 0);
+        }
+    }
+    if (t + 1 < nk) sstore(buf ^ 1);
+    __syncthreads();
+  }
+  const float back = alpha / (sa * sb);
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta) {
+    const int i = bm + wm + 32 * ta + r32;
+    if (i >= m) continue;
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int j = bn + wn + 32 * tb + 8 * (r >> 2) + 4 * u + (r & 3);
+        if (j >= n) continue;
+        float* c = C + i + (int64_t)j * ldc;
+        float v = back * acc[ta][tb][r];
+        if (beta != 0.f) v = __builtin_fma(beta, *c, v);
+        *c = v;
+      }
+  }
+}
+
+// max |.| of an m x n column-major FP32 block into a power-of-two scale 2^e
+// with max 2^e <= 448 (the E4M3 range; 1 for a zero block): partial maxima of
+// E4M3_PARTS workgroups, then one workgroup takes the maximum and the scale.
+constexpr int E4M3_PARTS = 256;
+
+__global__ __launch_bounds__(256) void e4m3_max_part_kernel(const float* __restrict__ X,
+                                                            int64_t ldx, int m, int n,
+                                                            float* __restrict__ part) {
+  float mx = 0.f;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < (int64_t)m * n;
+       e += (int64_t)gridDim.x * 256)
+    mx = fmaxf(mx, fabsf(X[e % m + (e / m) * ldx]));
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void e4m3_scale_kernel(const float* __restrict__ part, int np,
+                                                         float* __restrict__ out) {
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < np; i += 256) mx = fmaxf(mx, part[i]);
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    float sc = 1.f;
+    if (mx > 0.f) {
+      int e;
+      frexpf(448.f / mx, &e);  // 448 / mx = f 2^e, f in [0.5, 1): 2^(e-1) <= 448 / mx
+      sc = ldexpf(1.f, e - 1);
+    }
+    out[0] = sc;
+  }
+}
+
+size_t gemm_lowp_scale_floats() { return 2 + 2 * E4M3_PARTS; }
+
+int gemm_lowp(hipStream_t s, int m, int n, int k, float alpha, const float* A, int64_t lda,
+              const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int round,
+              float* scales) {
+  if (m <= 0 || n <= 0) return OK;
+  MPDOK_REQUIRE(round >= 1 && round <= 3, "gemm_lowp: round %d", round);
+  MPDOK_REQUIRE(round != 3 || scales != nullptr, "gemm_lowp: FP8 needs a scale buffer");
+  if (round == 3) {
+    // scales: [0] A's, [1] B's, then the partial maxima of each
+    float* pa = scales + 2;
+    float* pb = pa + E4M3_PARTS;
+    e4m3_max_part_kernel<<<E4M3_PARTS, 256, 0, s>>>(A, lda, m, k, pa);
+    e4m3_max_part_kernel<<<E4M3_PARTS, 256, 0, s>>>(B, ldb, k, n, pb);
+    e4m3_scale_kernel<<<1, 256, 0, s>>>(pa, E4M3_PARTS, scales);
+    e4m3_scale_kernel<<<1, 256, 0, s>>>(pb, E4M3_PARTS, scales + 1);
+  }
+  const dim3 grid(tiles_launch((m + 127) / 128, (n + 127) / 128));
+  if (round == 1)
+    gemm_lowp_kernel<1><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, scales);
+  else if (round == 2)
+    gemm_lowp_kernel<2><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, scales);
+  else
+    gemm_lowp_kernel<3><<<grid, 256, 0, s>>>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, scales);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
 }
 
 // Tile choice: 64 x 64 tiles for the skinny products of the panel and of the

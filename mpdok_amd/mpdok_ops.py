@@ -31,7 +31,7 @@ import torch
 from . import _dev, _lib
 from ._lib import check
 
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3, "ozaki": 4}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3, "ozaki": 4, "fp8": 5}
 PIVOTS = {"none": 0, "partial": 1}
 RESIDUALS = {"fp64": 0, "dot2": 1}
 

@@ -11,6 +11,8 @@ infrastructure only).
                  power-of-two row/column scales (Ozaki, Ogita, Oishi, Rump,
                  Numer. Algorithms 59 (2012)); products of slice pairs are exact
                  integers; pairs with i + j <= s + 1 are kept.
+  lowp_matmul    products of BF16 / FP16 / scaled E4M3 operands (OCP FP8
+                 specification's E4M3FN format) for the precision ladder.
   residual_*     exact and Dot2 residuals (Ogita, Rump, Oishi, SIAM J. Sci.
                  Comput. 26 (2005)).
 """
@@ -151,6 +153,57 @@ class OzakiAdaptiveTrailing:
         S, self.e0 = ozaki_decide(L21, U12, self.smax, self.smin, self.delta, self.e0)
         self.counts.append(S)
         return ozaki_matmul_lead(L21, U12, self.smax, S)
+
+
+# ---------------------------------------------------------------------------
+# Low-precision operands (the precision ladder)
+# ---------------------------------------------------------------------------
+def round_bf16(x):
+    """Nearest BF16 value of FP32 x (8 significant bits), ties to even: the
+    FP32 bit pattern rounded at bit 16."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def round_e4m3(x):
+    """Nearest OCP FP8 E4M3 value (E4M3FN: 3 fraction bits, exponent bias 7,
+    smallest normal 2^-6, subnormal step 2^-9, largest 448; larger magnitudes
+    saturate to 448), ties to even."""
+    x = np.asarray(x, np.float64)
+    a = np.abs(x)
+    _, e = np.frexp(a)                       # a = f 2^e, f in [0.5, 1)
+    q = np.maximum(e - 4, -9)
+    r = np.ldexp(np.rint(np.ldexp(a, -q)), q)
+    r = np.minimum(r, 448.0)
+    return np.where(a > 0, np.copysign(r, x), x)
+
+
+def e4m3_scale(M):
+    """Power-of-two scale 2^(e-1) with 448 / max|M| = f 2^e (f in [0.5, 1)),
+    so that max|M| times the scale stays within 448; 1 for an all-zero M."""
+    mx = float(np.max(np.abs(M), initial=0.0))
+    if mx == 0.0:
+        return 1.0
+    _, e = np.frexp(448.0 / mx)
+    return float(np.ldexp(1.0, int(e) - 1))
+
+
+def lowp_matmul(A, B, kind):
+    """A B with the operands rounded as the matrix cores see them ('bf16',
+    'fp16', or 'fp8': E4M3 after one power-of-two scale per operand, the
+    result scaled back), products and sums exact (FP64) for comparison with an
+    FP32-accumulated device result."""
+    A32 = np.asarray(A, np.float32)
+    B32 = np.asarray(B, np.float32)
+    if kind == "bf16":
+        return round_bf16(A32).astype(np.float64) @ round_bf16(B32).astype(np.float64)
+    if kind == "fp16":
+        return A32.astype(np.float16).astype(np.float64) @ B32.astype(np.float16).astype(np.float64)
+    sa, sb = e4m3_scale(A32), e4m3_scale(B32)
+    Aq = round_e4m3(A32.astype(np.float64) * sa)
+    Bq = round_e4m3(B32.astype(np.float64) * sb)
+    return (Aq @ Bq) / (sa * sb)
 
 
 # ---------------------------------------------------------------------------

@@ -60,6 +60,26 @@ def test_gemm_fp64_random_close(L):
     assert np.allclose(out, A @ B, rtol=1e-13, atol=1e-12)
 
 
+@pytest.mark.parametrize("round_,kind", [(1, "bf16"), (2, "fp16"), (3, "fp8")])
+@pytest.mark.parametrize("m,n,k", [(257, 300, 129), (256, 384, 512), (64, 64, 16)])
+def test_gemm_low_precision_matrix_cores_match_oracle(L, round_, kind, m, n, k):
+    """BF16 / FP16 operands on their matrix cores, and scaled E4M3 values on
+    the BF16 ones: the products of the rounded operands (oracle lowp_matmul,
+    exact) to within FP32 accumulation, C = -A B + C as in the LU."""
+    rng = np.random.default_rng(m + n + round_)
+    A = (rng.standard_normal((m, k)) * 3e-3).astype(np.float32)
+    B = (rng.standard_normal((k, n)) * 0.7).astype(np.float32)
+    C = rng.standard_normal((m, n)).astype(np.float32)
+    out = gemm(L, dev(A, torch.float32), dev(B, torch.float32), dev(C, torch.float32), -1.0, 1.0,
+               round_).cpu().numpy()
+    ref = C.astype(np.float64) - olu.lowp_matmul(A, B, kind)
+    bound = 2.0 ** -23 * (k + 2) * (np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+                                     + np.abs(C))
+    assert np.all(np.abs(out - ref) <= 4 * bound + 1e-30)
+    # and it is the low precision, not FP32 products
+    assert not np.allclose(out, C - A.astype(np.float64) @ B, rtol=1e-6, atol=1e-7)
+
+
 def test_gemm_bf16_rounding_matches_emulation(L):
     rng = np.random.default_rng(4)
     A = rng.standard_normal((64, 128)).astype(np.float32)
@@ -682,3 +702,39 @@ def test_solve_factored_block_equals_column_by_column(L, precision, pivot, resid
         assert torch.equal(Xb[:, t], xt), t
         assert np.linalg.norm(B[:, t] - A @ xt.cpu().numpy()) / np.linalg.norm(B[:, t]) < 1e-11
     s.free_factored()
+
+
+def test_precision_ladder_on_the_3d_generator(L):
+    """Which LU preconditioners carry GMRES-IR (FP64 body, Dot2 residual) to
+    1e-12 on the headline generator at N = 8192 (3-D points, gamma 25,
+    nugget 1e-6, kappa ~ 1e9): the int8-sliced LU does; FP32, BF16, FP16 and
+    scaled-FP8 trailing updates do not (u kappa >> 1: the refinement stalls or
+    the factors break down), and neither does an FP32 Krylov body with the
+    int8-sliced LU (kappa u32 ~ 60). The FP64 LU is the comparator."""
+    import warnings
+    from mpdok_amd.mpdok_ops import MPDOKSolver
+    N = 8192
+    X = np.random.default_rng(42).uniform(0, 1, (N, 3))
+    A = orb.build_kernel(X, "gaussian", 25.0, 1e-6)
+    b = (np.sin(2 * np.pi * X[:, 0]) * np.cos(2 * np.pi * X[:, 1]) * np.cos(2 * np.pi * X[:, 2])
+         + 1e-4 * np.random.default_rng(43).standard_normal(N))
+    Ad = dev(A)
+    out = {}
+    for prec, body in [("ozaki", "fp64"), ("fp64", "fp64"), ("ozaki", "fp32"), ("fp32", "fp64"),
+                       ("bf16", "fp64"), ("fp16", "fp64"), ("fp8", "fp64")]:
+        s = MPDOKSolver()
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            x = s.solve(Ad, b, tol=1e-12, maxiter_outer=4, restart=50, precond="lu",
+                        factor_precision=prec, slices=5, pivot="none", body=body,
+                        residual="dot2").cpu().numpy()
+        out[(prec, body)] = (s.last_info["converged"], s.last_info["history"][-1], x)
+        s.free_fp32_buf()
+    for key in [("ozaki", "fp64"), ("fp64", "fp64")]:
+        conv, rel, x = out[key]
+        assert conv and rel <= 1e-12, key
+        assert np.linalg.norm(olu.residual_longdouble(A, x, b)) / np.linalg.norm(b) <= 1e-12
+    for key in [("ozaki", "fp32"), ("fp32", "fp64"), ("bf16", "fp64"), ("fp16", "fp64"),
+                ("fp8", "fp64")]:
+        conv, rel, _ = out[key]
+        assert not conv and not (rel <= 1e-8), (key, rel)

@@ -496,6 +496,44 @@ def test_ozaki_lu_converges_in_lu_ir():
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-12
 
 
+# ------------------------------------------------------------ low precision
+def test_bf16_and_e4m3_rounding_match_torch_and_fix_every_code():
+    import torch
+    rng = np.random.default_rng(9)
+    x = (rng.standard_normal(20000) * np.exp(rng.uniform(-20, 20, 20000))).astype(np.float32)
+    ref = torch.tensor(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(olu.round_bf16(x), ref)
+    # E4M3 within range (torch's float8_e4m3fn rounds to nearest even and has
+    # no saturation, so compare inside +-448)
+    y = rng.uniform(-448, 448, 20000) * np.exp(rng.uniform(-14, 0, 20000))
+    y = y.astype(np.float32).astype(np.float64)
+    ref8 = torch.tensor(y, dtype=torch.float32).to(torch.float8_e4m3fn).to(torch.float64).numpy()
+    assert np.array_equal(olu.round_e4m3(y), ref8)
+    # every finite E4M3 code is a fixed point; 448 is the largest, larger
+    # values saturate
+    codes = torch.arange(256, dtype=torch.uint8).view(torch.float8_e4m3fn).to(torch.float64).numpy()
+    codes = codes[np.isfinite(codes)]
+    assert np.array_equal(olu.round_e4m3(codes), codes)
+    assert np.max(codes) == 448.0 and olu.round_e4m3(np.array([1e6]))[0] == 448.0
+    assert olu.round_e4m3(np.array([2.0 ** -10]))[0] == 0.0          # a tie: to even, 0
+    assert olu.round_e4m3(np.array([3 * 2.0 ** -10]))[0] == 2.0 ** -8
+    # scale: max |M| times it lands in (224, 448]
+    M = rng.standard_normal((30, 20)) * 1e-5
+    sc = olu.e4m3_scale(M)
+    assert 224 < np.max(np.abs(M)) * sc <= 448 and np.log2(sc) == int(np.log2(sc))
+
+
+def test_lowp_matmul_error_shrinks_with_the_format():
+    rng = np.random.default_rng(10)
+    A = rng.standard_normal((64, 200))
+    B = rng.standard_normal((200, 48))
+    exact = A.astype(np.float32).astype(np.float64) @ B.astype(np.float32).astype(np.float64)
+    errs = {k: np.abs(olu.lowp_matmul(A, B, k) - exact).max() / np.abs(exact).max()
+            for k in ("fp8", "bf16", "fp16")}
+    assert errs["fp16"] < errs["bf16"] < errs["fp8"] < 0.2
+    assert 2.0 ** -14 < errs["fp16"] < 2.0 ** -7 and 2.0 ** -11 < errs["bf16"] < 2.0 ** -4
+
+
 # ---------------------------------------------------------------- residuals
 def test_residual_exact_catches_cancellation_and_longdouble_agrees():
     # b - A x with massive cancellation: exact answer known by construction
