@@ -49,7 +49,7 @@ struct DevBuf {
 };
 
 struct IRWork {
-  DevBuf V, w, z, r, e, H, R, cs, sn, g, ctl, y, dots, orth, gemv, resid;
+  DevBuf V, Z, w, z, r, e, H, R, cs, sn, g, ctl, y, dots, orth, gemv, resid;
   int* ctl_host = nullptr;
   ~IRWork() {
     if (ctl_host) (void)hipHostFree(ctl_host);
@@ -85,6 +85,10 @@ static int host_norm(hipStream_t s, const double* v, int n, DevBuf& dots, double
 }
 
 // One cycle of (right-preconditioned) GMRES for A e = r. Body type TB.
+// With a preconditioner the cycle keeps z_k = inv(M) v_k (FP64) beside the
+// basis, so the correction is e = Z y (flexible GMRES's update form): the
+// same vector as inv(M) (V y) in exact arithmetic, with one triangular-solve
+// pass less per cycle.
 template <typename TB>
 static int gmres_cycle(hipStream_t s, const void* Abody, int64_t ldab, int orderb,
                        LUHandle* M, int N, const double* r, double rnorm, double* e,
@@ -93,7 +97,6 @@ static int gmres_cycle(hipStream_t s, const void* Abody, int64_t ldab, int order
   const int ldh = m + 1;
   TB* V = static_cast<TB*>(W.V.p);
   TB* w = static_cast<TB*>(W.w.p);
-  double* z64 = static_cast<double*>(W.z.p);  // FP64 scratch (preconditioner in/out)
   double* H = static_cast<double*>(W.H.p);
   double* R = static_cast<double*>(W.R.p);
   double* cs = static_cast<double*>(W.cs.p);
@@ -112,13 +115,17 @@ static int gmres_cycle(hipStream_t s, const void* Abody, int64_t ldab, int order
     TB* vk = V + (size_t)k * N;
     const TB* xin = vk;
     if (M != nullptr) {
-      // z = inv(M) v_k  (in FP64), then into the body type
-      MPDOK_TRY(axpby(s, N, 1.0, vk, dt, 0.0, nullptr, F64, z64, F64));
-      MPDOK_TRY(lu_solve(M, s, z64, z64));
-      MPDOK_TRY(axpby(s, N, 1.0, z64, F64, 0.0, nullptr, F64, w, dt));
-      // w currently holds inv(M) v_k; move it to the first free slot of V's tail
-      // is not possible (V_{k+1} is the output), so use e as FP64 staging:
-      xin = w;
+      // z_k = inv(M) v_k in FP64, kept for the update; the matvec takes it in
+      // the body type
+      double* zk = static_cast<double*>(W.Z.p) + (size_t)k * N;
+      MPDOK_TRY(axpby(s, N, 1.0, vk, dt, 0.0, nullptr, F64, zk, F64));
+      MPDOK_TRY(lu_solve(M, s, zk, zk));
+      if (sizeof(TB) == 8) {
+        xin = reinterpret_cast<const TB*>(zk);
+      } else {
+        MPDOK_TRY(axpby(s, N, 1.0, zk, F64, 0.0, nullptr, F64, w, dt));
+        xin = w;
+      }
     }
     TB* y = (M != nullptr) ? reinterpret_cast<TB*>(W.e.p) : w;  // matvec output
     if (sizeof(TB) == 4)
@@ -149,9 +156,12 @@ static int gmres_cycle(hipStream_t s, const void* Abody, int64_t ldab, int order
   }
   double* ydev = static_cast<double*>(W.y.p);
   MPDOK_HIP(hipMemcpyAsync(ydev, yh.data(), sizeof(double) * done, hipMemcpyHostToDevice, s));
-  // e = V y (FP64), then e = inv(M) e
-  MPDOK_TRY((basis_update<TB, double>(s, V, N, done, ydev, e, N, 1)));
-  if (M != nullptr) MPDOK_TRY(lu_solve(M, s, e, e));
+  // e = V y, or e = Z y with a preconditioner (FP64)
+  if (M != nullptr)
+    MPDOK_TRY((basis_update<double, double>(s, static_cast<const double*>(W.Z.p), N, done, ydev,
+                                            e, N, 1)));
+  else
+    MPDOK_TRY((basis_update<TB, double>(s, V, N, done, ydev, e, N, 1)));
   *steps = done;
   return OK;
 }
@@ -170,6 +180,7 @@ static int gmres_ir_impl(hipStream_t s, const double* A64, int64_t lda64, int or
   const size_t tb = o.body == 1 ? 8 : 4;
   const int m = o.restart;
   MPDOK_TRY(W.V.get(tb * (size_t)(m + 1) * N));
+  if (M != nullptr) MPDOK_TRY(W.Z.get(8 * (size_t)m * N));
   MPDOK_TRY(W.w.get(8 * (size_t)N));
   MPDOK_TRY(W.z.get(8 * (size_t)N));
   MPDOK_TRY(W.r.get(8 * (size_t)N));
