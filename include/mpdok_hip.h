@@ -392,6 +392,35 @@ int mpdok_batched_rbf_solve(int nsys, const int* N, const double* const* coords,
                             double* const* X, double tol, int maxiter_outer, int precision,
                             int slices, int nb, int nstreams, int* converged, double* relres);
 
+/* Batched plan: the same systems solved with grouped launches (one launch per
+ * kind of work and block step covers every system with work at that step)
+ * captured into one hipGraph; no reference counterpart (north star's batched
+ * configuration). Systems are padded to multiples of 64 with an identity
+ * block; FP64 no-pivot LU with 64-wide steps; LU-IR with the Dot2 residual and
+ * device-side convergence flags (converged columns and finished systems make
+ * the remaining launches return at once). coords[i], B[i], X[i]: device
+ * pointers kept by the plan (each run reads coords and B and writes X).
+ * update_rows: trailing updates of at least this many rows use 128 x 128
+ * tiles, smaller ones 64 x 64 (chosen by the Python tuner). use_graph: 1 to
+ * capture at creation and replay on each run, 0 to launch eagerly. nrhs <= 8,
+ * N[i] <= 65536. */
+int mpdok_bplan_create(int nsys, const int* N, const double* const* coords, int D, int kind,
+                       const double* param, double reg, int nrhs, const double* const* B,
+                       double* const* X, double tol, int maxiter_outer, int update_rows,
+                       int use_graph, void** plan_out);
+/* Enqueue one solve on s (a graph launch, or the eager sequence); no wait. */
+int mpdok_bplan_run(void* plan, hipStream_t s);
+/* Outcome of the last run in the caller's system order (waits for the
+ * device): converged, relres, steps (updates applied): nsys x nrhs; info:
+ * nsys (zero-pivot column + 1, else 0); hist: relres per refinement step,
+ * (maxiter_outer + 1) x nsys x nrhs, NaN after a column stopped. Any pointer
+ * may be NULL. */
+int mpdok_bplan_results(void* plan, int* converged, double* relres, int* steps, int* info,
+                        double* hist);
+/* Device bytes held and kernel launches per run (graph nodes). */
+int mpdok_bplan_info(void* plan, size_t* bytes, int* launches);
+void mpdok_bplan_destroy(void* plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,12 @@ SIGNATURES = {
                            PI, PI]),
     "mpdok_lu_ir": (I, [P, P, I64, I, P, I, P, P, D, I, I, PI, P, I, PI]),
     "mpdok_batched_rbf_solve": (I, [I, P, P, I, I, P, D, I, P, P, D, I, I, I, I, I, P, P]),
+    # batched plan: grouped launches captured in one hipGraph
+    "mpdok_bplan_create": (I, [I, P, P, I, I, P, D, I, P, P, D, I, I, I, PP]),
+    "mpdok_bplan_run": (I, [P, P]),
+    "mpdok_bplan_results": (I, [P, P, P, P, P, P]),
+    "mpdok_bplan_info": (I, [P, P, PI]),
+    "mpdok_bplan_destroy": (None, [P]),
 }
 
 F64, F32, BF16, F16 = 0, 1, 2, 3

@@ -94,6 +94,124 @@ def solve_batched(systems, tol=1e-12, reg=1e-6, precision="fp64", slices=6, nb=2
                 "relres": rel.reshape(nsys, nrhs), "seconds": secs}
 
 
+class BatchedPlan:
+    """All systems of a batch solved by grouped launches captured in one
+    hipGraph (mpdok_bplan_create): a run is one graph launch, with the
+    assembly, the FP64 no-pivot LU (64-wide steps), LU-IR with the Dot2
+    residual and the copy-out inside it; convergence is tracked on the device.
+
+    systems: list of (X_i (N_i, D) points, eps_i, B_i (N_i, nrhs)) as from
+    c4_workload. The plan keeps device copies of X_i and B_i and one output
+    block per system (self.xs); set_rhs() writes new right-hand sides into
+    those buffers, so the next run() solves them without re-capturing.
+    update_rows: trailing updates with at least this many rows use 128 x 128
+    tiles, smaller ones 64 x 64. use_graph=False launches the same kernels
+    eagerly (for comparison)."""
+
+    def __init__(self, systems, tol=1e-12, reg=1e-6, maxiter_outer=4, update_rows=512,
+                 use_graph=True, kernel="gaussian"):
+        if len(systems) == 0:
+            raise ValueError("BatchedPlan: no systems")
+        kind = {"gaussian": 0, "matern32": 1}[kernel]
+        self._keep, self.xs, self.bs = [], [], []
+        Ns, cps, bps, xps, params = [], [], [], [], []
+        nrhs = D = None
+        for X, eps, B in systems:
+            Xd = _dev.as_device(X, torch.float64).contiguous()
+            Bd = _dev.as_device(np.asarray(B, dtype=np.float64).reshape(Xd.shape[0], -1),
+                                torch.float64, "F")
+            if nrhs is None:
+                nrhs, D = Bd.shape[1], Xd.shape[1]
+            if Bd.shape[1] != nrhs or Xd.shape[1] != D:
+                raise ValueError("BatchedPlan: every system needs the same D and nrhs")
+            Xo = _dev.empty(tuple(Bd.shape), torch.float64, "F")
+            self._keep += [Xd]
+            self.bs.append(Bd)
+            self.xs.append(Xo)
+            Ns.append(Xd.shape[0])
+            cps.append(Xd.data_ptr())
+            bps.append(Bd.data_ptr())
+            xps.append(Xo.data_ptr())
+            params.append(float(eps) ** 2 if kind == 0 else float(eps))
+        self.nsys, self.nrhs, self.maxiter_outer = len(systems), nrhs, maxiter_outer
+        self.N = np.array(Ns)
+        P = ctypes.c_void_p
+        torch.cuda.synchronize()
+        h = ctypes.c_void_p()
+        check(lib().mpdok_bplan_create(
+            self.nsys, (ctypes.c_int * self.nsys)(*Ns), (P * self.nsys)(*cps), D, kind,
+            (ctypes.c_double * self.nsys)(*params), float(reg), nrhs, (P * self.nsys)(*bps),
+            (P * self.nsys)(*xps), float(tol), int(maxiter_outer), int(update_rows),
+            1 if use_graph else 0, ctypes.byref(h)), "bplan_create")
+        self._h = h
+
+    def set_rhs(self, i, B):
+        """Overwrite system i's right-hand sides (same shape) for the next run."""
+        Bd = _dev.as_device(np.asarray(B, dtype=np.float64).reshape(self.bs[i].shape),
+                            torch.float64, "F") if not isinstance(B, torch.Tensor) else B
+        self.bs[i].copy_(Bd)
+
+    def run(self, sync=True):
+        """One solve of every system (one graph launch on torch's stream)."""
+        if self._h is None:
+            raise RuntimeError("BatchedPlan: closed")
+        stream = torch.cuda.current_stream().cuda_stream
+        check(lib().mpdok_bplan_run(self._h, ctypes.c_void_p(stream)), "bplan_run")
+        if sync:
+            torch.cuda.synchronize()
+        return self.xs
+
+    def results(self):
+        """Outcome of the last run: converged, relres, steps (nsys x nrhs),
+        info (nsys), hist ((maxiter_outer + 1) x nsys x nrhs)."""
+        n, r = self.nsys, self.nrhs
+        conv = np.zeros(n * r, np.int32)
+        rel = np.zeros(n * r)
+        steps = np.zeros(n * r, np.int32)
+        info = np.zeros(n, np.int32)
+        hist = np.zeros((self.maxiter_outer + 1) * n * r)
+        P = ctypes.c_void_p
+        check(lib().mpdok_bplan_results(self._h, conv.ctypes.data_as(P), rel.ctypes.data_as(P),
+                                        steps.ctypes.data_as(P), info.ctypes.data_as(P),
+                                        hist.ctypes.data_as(P)), "bplan_results")
+        return {"converged": conv.reshape(n, r).astype(bool), "relres": rel.reshape(n, r),
+                "steps": steps.reshape(n, r), "info": info,
+                "hist": hist.reshape(self.maxiter_outer + 1, n, r)}
+
+    def info(self):
+        b = ctypes.c_size_t(0)
+        k = ctypes.c_int(0)
+        check(lib().mpdok_bplan_info(self._h, ctypes.byref(b), ctypes.byref(k)), "bplan_info")
+        return {"device_bytes": int(b.value), "launches": int(k.value)}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().mpdok_bplan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_batched_graph(systems, tol=1e-12, reg=1e-6, maxiter_outer=4, update_rows=512,
+                        use_graph=True):
+    """Build a BatchedPlan, run it once; returns (xs, info) like solve_batched
+    (info also has 'steps', 'hist' and 'seconds' of the run)."""
+    plan = BatchedPlan(systems, tol=tol, reg=reg, maxiter_outer=maxiter_outer,
+                       update_rows=update_rows, use_graph=use_graph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs = plan.run()
+    secs = time.perf_counter() - t0
+    out = plan.results()
+    out["seconds"] = secs
+    plan.close()
+    return xs, out
+
+
 class BatchTuner:
     """Picks (nstreams, nb, precision) for a batch by timing candidates on a
     sample of it; remembers the choice per size profile (sizes rounded to

@@ -70,6 +70,28 @@ int widen_block_inverse(hipStream_t s, const T* in, T* out, int nbig, int lower)
 template <typename T>
 int seed_block_inverse(hipStream_t s, const T* inv64, T* binv, int nbig);
 
+// Grouped steps of a batched no-pivot LU (lu_kernels.hip). System table
+// entry: factors F (P x P, column-major, ld P, P a multiple of 64), the
+// inverses of the unit-lower and upper triangles of its 64 x 64 diagonal
+// blocks (P x 64 each, ld P; block k at row 64 k) and its zero-pivot slot
+// (the first zero pivot's column + 1; 0 when none).
+struct BLUSys {
+  double* F;
+  double* Linv;
+  double* Uinv;
+  int* info;
+  int P;
+  int pad_;
+};
+// Step k for systems sys[0 .. ns), each with P > 64 k. pref: ns + 1 tile
+// offsets (the tiles of the step's panel products, 2 ceil(m / 64) per system,
+// or of its trailing update, ceil(m / tb)^2, m = P - 64 (k + 1)), ntiles =
+// pref[ns] (the host's copy).
+int blu_diag(hipStream_t s, const BLUSys* sys, int ns, int k);
+int blu_panel(hipStream_t s, const BLUSys* sys, const int* pref, int ns, int ntiles, int k);
+int blu_update(hipStream_t s, const BLUSys* sys, const int* pref, int ns, int ntiles, int k,
+               int tb);
+
 // ozaki.hip
 size_t ozaki_workspace_bytes(int m, int n, int K, int S);
 int ozaki_gemm(hipStream_t s, int S, int m, int n, int K, const double* A, int64_t lda,

@@ -77,10 +77,11 @@ constexpr int GBK = 16, GPAD = 16;
 //
 // gemm_tile: the tile of C that workgroup b of a launch computes (the body of
 // both kernels below).
+// gemm_tile_at: the same for tile (ti, tj) given directly (grouped launches).
 template <typename T, int ROUND, int TB>
-__device__ __forceinline__ void gemm_tile(int b, int m, int n, int k, T alpha, const T* A,
-                                          int64_t lda, const T* B, int64_t ldb, T beta, T* C,
-                                          int64_t ldc) {
+__device__ __forceinline__ void gemm_tile_at(int ti, int tj, int m, int n, int k, T alpha,
+                                             const T* A, int64_t lda, const T* B, int64_t ldb,
+                                             T beta, T* C, int64_t ldc) {
   constexpr int KPH = 256 / TB;      // k phases of the A loads
   constexpr int NA = GBK / KPH;      // A elements per thread
   constexpr int NBL = TB / 16;       // B elements per thread
@@ -91,8 +92,6 @@ __device__ __forceinline__ void gemm_tile(int b, int m, int n, int k, T alpha, c
   // 16 consecutive columns, both fall into distinct banks.
   __shared__ T sB[2][TB][GBK + 1];
   typedef typename Mfma<T>::acc_t acc_t;
-  int ti, tj;
-  if (!tile_of(b, (m + TB - 1) / TB, (n + TB - 1) / TB, ti, tj)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * (TB / 2), wn = (wave >> 1) * (TB / 2);
   const int bm = ti * TB, bn = tj * TB;
@@ -174,6 +173,15 @@ __device__ __forceinline__ void gemm_tile(int b, int m, int n, int k, T alpha, c
         }
       }
     }
+}
+
+template <typename T, int ROUND, int TB>
+__device__ __forceinline__ void gemm_tile(int b, int m, int n, int k, T alpha, const T* A,
+                                          int64_t lda, const T* B, int64_t ldb, T beta, T* C,
+                                          int64_t ldc) {
+  int ti, tj;
+  if (!tile_of(b, (m + TB - 1) / TB, (n + TB - 1) / TB, ti, tj)) return;
+  gemm_tile_at<T, ROUND, TB>(ti, tj, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 // Batched use: blockIdx.y selects the problem (operands at strides sA_, sB_,
@@ -623,11 +631,10 @@ __device__ __forceinline__ float lane_bcast(float x, int src) {
 //     their way out of LDS), staged in registers and written back in place.
 // The rounding differs from the unblocked kernel's (other summation orders).
 template <typename T>
-__global__ __launch_bounds__(256) void diag_factor_blk_kernel(T* __restrict__ A, int64_t lda,
-                                                              int nb, T* __restrict__ Linv,
-                                                              T* __restrict__ Uinv, int ldi,
-                                                              int* __restrict__ info, int col0,
-                                                              int factor) {
+__device__ __forceinline__ void diag_factor_blk_body(T* __restrict__ A, int64_t lda, int nb,
+                                                     T* __restrict__ Linv, T* __restrict__ Uinv,
+                                                     int64_t ldi, int* __restrict__ info,
+                                                     int col0, int factor) {
   __shared__ T s[64][65];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   auto wave_sync = [] {
@@ -818,6 +825,15 @@ __global__ __launch_bounds__(256) void diag_factor_blk_kernel(T* __restrict__ A,
       Linv[r + c * ldi] = r > c ? s[r][c] : (r == c ? T(1) : T(0));
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void diag_factor_blk_kernel(T* __restrict__ A, int64_t lda,
+                                                              int nb, T* __restrict__ Linv,
+                                                              T* __restrict__ Uinv, int ldi,
+                                                              int* __restrict__ info, int col0,
+                                                              int factor) {
+  diag_factor_blk_body<T>(A, lda, nb, Linv, Uinv, ldi, info, col0, factor);
 }
 
 template <typename T>
@@ -1465,6 +1481,98 @@ __global__ void ipiv_to_perm_kernel(const int* __restrict__ ipiv, int* __restric
 }
 int ipiv_to_perm(hipStream_t s, const int* ipiv, int* perm, int n) {
   ipiv_to_perm_kernel<<<1, 64, 0, s>>>(ipiv, perm, n);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches of a batched no-pivot LU (many systems, one launch per
+// kind of work and step). Step k of every system in sys[0 .. ns) (P > 64 k):
+// blu_diag factors the 64 x 64 diagonal block at (64 k, 64 k) and stores the
+// inverses of its unit-lower and upper triangles; blu_panel forms
+// L21 = A21 inv(U11) and U12 = inv(L11) A12 in place (64 x 64 tiles, the
+// aliasing rules of gemm_tile); blu_update does A22 -= L21 U12 (TB x TB
+// tiles). The tiles of all systems form one grid: pref[s] is the first tile
+// of system s (pref[ns] the total), and workgroups are dealt to the XCDs in
+// contiguous ranges as in tile_of, so a system's tiles share an XCD's L2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool grouped_tile(int b, const int* __restrict__ pref, int ns, int& si,
+                                             int& t) {
+  const int T = pref[ns];
+  const int per = (T + NXCD - 1) / NXCD;
+  const int tt = (b % NXCD) * per + b / NXCD;
+  if (b / NXCD >= per || tt >= T) return false;
+  // the largest s with pref[s] <= tt (systems without tiles are skipped)
+  int lo = 0, hi = ns;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pref[mid] <= tt) lo = mid;
+    else hi = mid;
+  }
+  si = lo;
+  t = tt - pref[lo];
+  return true;
+}
+
+__global__ __launch_bounds__(256) void blu_diag_kernel(const BLUSys* __restrict__ sys, int k) {
+  const BLUSys q = sys[blockIdx.x];
+  const int64_t P = q.P, c0 = 64 * (int64_t)k;
+  diag_factor_blk_body<double>(q.F + c0 + c0 * P, P, 64, q.Linv + c0, q.Uinv + c0, P, q.info,
+                               (int)c0, 1);
+}
+
+__global__ __launch_bounds__(256, 2) void blu_panel_kernel(const BLUSys* __restrict__ sys,
+                                                           const int* __restrict__ pref, int ns,
+                                                           int k) {
+  int si, t;
+  if (!grouped_tile(blockIdx.x, pref, ns, si, t)) return;
+  const BLUSys q = sys[si];
+  const int64_t P = q.P, c0 = 64 * (int64_t)k, c1 = c0 + 64;
+  const int m = q.P - (int)c1;
+  const int tm = (m + 63) / 64;
+  double* A21 = q.F + c1 + c0 * P;
+  double* A12 = q.F + c0 + c1 * P;
+  const bool l21 = t < tm;  // L21 row tile t, else U12 column tile t - tm
+  gemm_tile_at<double, 0, 64>(l21 ? t : 0, l21 ? 0 : t - tm, l21 ? m : 64, l21 ? 64 : m, 64, 1.0,
+                              l21 ? A21 : q.Linv + c0, P, l21 ? q.Uinv + c0 : A12, P, 0.0,
+                              l21 ? A21 : A12, P);
+}
+
+template <int TB>
+__global__ __launch_bounds__(256, 2) void blu_update_kernel(const BLUSys* __restrict__ sys,
+                                                            const int* __restrict__ pref, int ns,
+                                                            int k) {
+  int si, t;
+  if (!grouped_tile(blockIdx.x, pref, ns, si, t)) return;
+  const BLUSys q = sys[si];
+  const int64_t P = q.P, c0 = 64 * (int64_t)k, c1 = c0 + 64;
+  const int m = q.P - (int)c1;
+  const int tm = (m + TB - 1) / TB;
+  gemm_tile_at<double, 0, TB>(t % tm, t / tm, m, m, 64, -1.0, q.F + c1 + c0 * P, P,
+                              q.F + c0 + c1 * P, P, 1.0, q.F + c1 + c1 * P, P);
+}
+
+int blu_diag(hipStream_t s, const BLUSys* sys, int ns, int k) {
+  if (ns <= 0) return OK;
+  blu_diag_kernel<<<ns, 256, 0, s>>>(sys, k);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+int blu_panel(hipStream_t s, const BLUSys* sys, const int* pref, int ns, int ntiles, int k) {
+  if (ns <= 0 || ntiles <= 0) return OK;
+  blu_panel_kernel<<<(ntiles + NXCD - 1) / NXCD * NXCD, 256, 0, s>>>(sys, pref, ns, k);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
+int blu_update(hipStream_t s, const BLUSys* sys, const int* pref, int ns, int ntiles, int k,
+               int tb) {
+  MPDOK_REQUIRE(tb == 64 || tb == 128, "blu_update: tile %d", tb);
+  if (ns <= 0 || ntiles <= 0) return OK;
+  const int grid = (ntiles + NXCD - 1) / NXCD * NXCD;
+  if (tb == 128) blu_update_kernel<128><<<grid, 256, 0, s>>>(sys, pref, ns, k);
+  else blu_update_kernel<64><<<grid, 256, 0, s>>>(sys, pref, ns, k);
   MPDOK_HIP(hipGetLastError());
   return OK;
 }

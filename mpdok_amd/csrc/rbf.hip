@@ -17,34 +17,6 @@
 
 namespace mpdok {
 
-constexpr int MAXD = 8;
-
-__device__ __forceinline__ double kernel_value(int kind, double p, double d2) {
-  if (kind == 0) return exp(-p * d2);
-  const double a = sqrt(3.0 * d2) / p;
-  return (1.0 + a) * exp(-a);
-}
-
-// Squared distance by the Gram identity, evaluated in the reference's order:
-// (sq_i + sq_j) - 2 * (c_i . c_j), clamped at zero.
-__device__ __forceinline__ double gram_d2(const double* ci, double sqi, const double* cj, double sqj,
-                                          int D) {
-  double dot = ci[0] * cj[0];
-#pragma unroll
-  for (int d = 1; d < MAXD; ++d)
-    if (d < D) dot = __builtin_fma(ci[d], cj[d], dot);
-  double d2 = (sqi + sqj) - 2.0 * dot;
-  return d2 > 0.0 ? d2 : 0.0;
-}
-
-__device__ __forceinline__ double sqnorm(const double* c, int D) {
-  double s = c[0] * c[0];
-#pragma unroll
-  for (int d = 1; d < MAXD; ++d)
-    if (d < D) s += c[d] * c[d];
-  return s;
-}
-
 template <typename TO> __device__ __forceinline__ TO to_out(double v);
 template <> __device__ __forceinline__ double to_out<double>(double v) { return v; }
 template <> __device__ __forceinline__ float to_out<float>(double v) { return (float)v; }
