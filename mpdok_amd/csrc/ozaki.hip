@@ -665,18 +665,42 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
   double cpre[PRE ? TA : 1][PRE ? TB : 1][16];
   if constexpr (PRE) {
     const int rl = lane & 31, u = lane >> 5;
+    // Every lane of every wavefront issues every load: the first wait below
+    // counts them, and a load skipped by a whole wavefront (an edge tile
+    // whose rows or columns it does not reach) would let that wait pass
+    // before stage 0's copies land. Interior tiles load unconditionally;
+    // edge tiles load from the block's first element where a value lies
+    // outside the block and take 0 for it (never stored).
+    const bool interior = bm + Cfg::BM <= m && bn + Cfg::BN <= col0 + ncols;  // workgroup-uniform
+    if (interior) {
 #pragma unroll
-    for (int ta = 0; ta < TA; ++ta) {
-      const int i = bm + wm + 32 * ta + rl;
+      for (int ta = 0; ta < TA; ++ta) {
+        const int i = bm + wm + 32 * ta + rl;
 #pragma unroll
-      for (int tb = 0; tb < TB; ++tb) {
-        const int jb = bn + wn + 32 * tb + 4 * u;
+        for (int tb = 0; tb < TB; ++tb) {
+          const int jb = bn + wn + 32 * tb + 4 * u;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int jo = 8 * (r >> 2) + (r & 3);
-          cpre[ta][tb][r] = (i < m && jb + jo < col0 + ncols)
-                                ? __builtin_nontemporal_load(C + i + (int64_t)(jb + jo) * ldc)
-                                : 0.0;
+          for (int r = 0; r < 16; ++r) {
+            const int jo = 8 * (r >> 2) + (r & 3);
+            cpre[ta][tb][r] = __builtin_nontemporal_load(C + i + (int64_t)(jb + jo) * ldc);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta) {
+        const int i = bm + wm + 32 * ta + rl;
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+          const int jb = bn + wn + 32 * tb + 4 * u;
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int jo = 8 * (r >> 2) + (r & 3);
+            const bool ok = i < m && jb + jo < col0 + ncols;
+            const double v =
+                __builtin_nontemporal_load(ok ? C + i + (int64_t)(jb + jo) * ldc : C);
+            cpre[ta][tb][r] = ok ? v : 0.0;
+          }
         }
       }
     }

@@ -263,7 +263,8 @@ def _ozaki_adaptive(L, A, B, C, smax, s_fixed=0, smin=2, delta=0, e0=None):
 
 @pytest.mark.parametrize("smax", [3, 4, 5])
 @pytest.mark.parametrize("m,n,K", [(100, 70, 64), (257, 129, 300), (640, 512, 512),
-                                   (2047, 1100, 64)])
+                                   (2047, 1100, 64), (300, 260, 1000), (50, 70, 1024),
+                                   (1100, 700, 1024)])
 def test_ozaki_leading_slices_bit_exact_vs_oracle(L, smax, m, n, K):
     """The runtime-count kernel multiplies the leading S of smax digits: bit for
     bit the oracle's ozaki_matmul_lead for every S in [2, smax]."""
