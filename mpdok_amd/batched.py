@@ -3,14 +3,21 @@
 Each system i: points X_i (N_i x D), Gaussian kernel exp(-eps_i^2 r^2) plus a
 nugget, nrhs right-hand sides; solved to relres <= tol per right-hand side
 (kernel assembly, LU without pivoting, LU-IR with the compensated residual).
-The whole batch runs inside the library (mpdok_batched_rbf_solve): systems
-are spread over several streams, each driven by its own host thread, largest
-first by the longest-processing-time rule, so small systems fill the gaps left
-by large ones.
+Two paths run the whole batch inside the library:
 
-`BatchTuner` picks the number of streams, the LU block size and the factor
-precision by timing candidates on a sample of the batch (every k-th system by
-size, about 24), and caches the choice per size profile. This path has no
+* `BatchedPlan` (mpdok_bplan_*): grouped launches, one per kind of work and
+  step across every system that still has work, captured once into a
+  hipGraph; a solve is one graph launch, with convergence tracked on the
+  device.
+* `solve_batched` (mpdok_batched_rbf_solve): systems spread over several
+  streams, each driven by its own host thread, largest first by the
+  longest-processing-time rule, so small systems fill the gaps left by large
+  ones.
+
+`BatchTuner` times candidates on a sample of the batch (every k-th system by
+size, about 24) and caches the choice per size profile: the tile-shape
+threshold of the graph path, and the number of streams, LU block size and
+factor precision of the stream path. This path has no
 counterpart in the reference; it follows the north star's batched
 configuration (256 systems, N log-uniform in [256, 8192], 4 right-hand sides).
 """
@@ -213,28 +220,84 @@ def solve_batched_graph(systems, tol=1e-12, reg=1e-6, maxiter_outer=4, update_ro
 
 
 class BatchTuner:
-    """Picks (nstreams, nb, precision) for a batch by timing candidates on a
-    sample of it; remembers the choice per size profile (sizes rounded to
-    powers of two)."""
+    """Picks launch parameters for a batch by timing candidates on a sample of
+    it (every k-th system by size, about 24) and remembers the choice per size
+    profile (sizes rounded to powers of two).
+
+    choose(): (nstreams, nb, precision) of the stream path (solve_batched).
+    choose_plan(): the tile-shape threshold of the graph path (BatchedPlan's
+    update_rows: the trailing updates of systems with at least that many rows
+    left run 128 x 128 tiles of 4 wavefronts, smaller ones 64 x 64 tiles)."""
 
     CANDIDATES = {"nstreams": (4, 8, 16), "nb": (256, 512), "precision": ("fp64", "ozaki")}
+    PLAN_ROWS = (256, 512, 1024, 2048)
 
     def __init__(self):
         self.cache = {}
+        self.plan_cache = {}
         self.trials = []
 
     @staticmethod
     def profile(systems):
         return tuple(sorted({int(round(math.log2(len(s[0])))) for s in systems}))
 
+    @staticmethod
+    def _sample(systems):
+        # every k-th system by size: the sample keeps the batch's size mix
+        by_n = sorted(systems, key=lambda s: len(s[0]))
+        return by_n[::max(1, len(by_n) // 24)]
+
+    def choose_plan(self, systems, tol=1e-12, reg=1e-6, maxiter_outer=4):
+        """update_rows for a BatchedPlan of this batch: a plan per candidate
+        on the sample, one warm-up replay, then the time of one replay; the
+        fastest candidate whose systems all converged wins."""
+        key = self.profile(systems)
+        if key in self.plan_cache:
+            return self.plan_cache[key]
+        sample = self._sample(systems)
+        best = None
+        for rows in self.PLAN_ROWS:
+            plan = BatchedPlan(sample, tol=tol, reg=reg, maxiter_outer=maxiter_outer,
+                               update_rows=rows)
+            plan.run()
+            t0 = time.perf_counter()
+            plan.run()
+            secs = time.perf_counter() - t0
+            ok = bool(plan.results()["converged"].all())
+            plan.close()
+            self.trials.append({"update_rows": rows, "seconds": secs, "converged": ok,
+                                "path": "graph"})
+            if ok and (best is None or secs < best[0]):
+                best = (secs, {"update_rows": rows})
+        choice = best[1] if best else {"update_rows": 512}
+        self.plan_cache[key] = choice
+        return choice
+
+    def solve_plan(self, systems, tol=1e-12, reg=1e-6, maxiter_outer=4, **kw):
+        """Build the tuned BatchedPlan for the whole batch and run it once:
+        returns (xs, info) as solve_batched, with info['seconds'] the run (one
+        graph launch), info['build_seconds'] the plan's set-up and capture,
+        info['config'] the choice and info['plan'] its device bytes and
+        captured launches."""
+        cfg = dict(self.choose_plan(systems, tol, reg, maxiter_outer))
+        cfg.update(kw)
+        t0 = time.perf_counter()
+        plan = BatchedPlan(systems, tol=tol, reg=reg, maxiter_outer=maxiter_outer, **cfg)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        xs = plan.run()
+        secs = time.perf_counter() - t0
+        out = plan.results()
+        out.update(seconds=secs, build_seconds=tb, config=cfg, plan=plan.info())
+        plan.close()
+        return xs, out
+
     def choose(self, systems, tol=1e-12, reg=1e-6):
         key = self.profile(systems)
         if key in self.cache:
             return self.cache[key]
-        # every k-th system by size: the sample keeps the batch's size mix
-        by_n = sorted(systems, key=lambda s: len(s[0]))
-        step = max(1, len(by_n) // 24)
-        sample = by_n[::step]
+        sample = self._sample(systems)
         best = None
         for ns, nb, prec in itertools.product(*self.CANDIDATES.values()):
             _, info = solve_batched(sample, tol=tol, reg=reg, precision=prec, nb=nb,

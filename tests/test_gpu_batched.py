@@ -165,6 +165,26 @@ def test_graph_plan_tile_choice_does_not_change_convergence(update_rows):
     b.close()
 
 
+def test_tuner_picks_a_plan_threshold_and_solves_with_it():
+    """The graph path's tuner times each candidate threshold on the sample,
+    caches the choice per size profile, and the tuned plan reaches 1e-12 with
+    the same solutions as a plan built with that threshold by hand."""
+    from mpdok_amd.batched import BatchedPlan, BatchTuner
+    systems = _systems(seed=17, nrhs=2)
+    tuner = BatchTuner()
+    choice = tuner.choose_plan(systems)
+    assert choice["update_rows"] in BatchTuner.PLAN_ROWS
+    assert tuner.choose_plan(systems) is choice
+    assert sum(t.get("path") == "graph" for t in tuner.trials) == len(BatchTuner.PLAN_ROWS)
+    xs, info = tuner.solve_plan(systems)
+    assert info["converged"].all() and (info["relres"] < 1e-12).all()
+    assert info["config"] == choice and info["plan"]["launches"] > 0
+    ref = BatchedPlan(systems, update_rows=choice["update_rows"])
+    for a, b in zip(xs, ref.run()):
+        assert torch.equal(a, b)
+    ref.close()
+
+
 def test_graph_plan_matches_the_stream_path_and_reports_zero_pivots():
     """Against the host-threaded path (FP64 LU, 512-wide panels): both reach
     1e-12 and agree to the accuracy the conditioning allows; a matrix with a
