@@ -554,12 +554,80 @@ __device__ __forceinline__ void ozp_issue(signed char* __restrict__ st, const si
   ozp_issue_op<S, Cfg::NT, Cfg::BN>(st + Cfg::ABYTES, b, slabB, tid, wave);
 }
 
+// LDS address (32-bit) of a pointer into a __shared__ array.
+__device__ __forceinline__ uint32_t oz_lds_addr(const signed char* p) {
+  return static_cast<uint32_t>(
+      reinterpret_cast<uintptr_t>((const __attribute__((address_space(3))) signed char*)p));
+}
+// Fragment reads of one K step (6 or 9 ds_read_b128) issued together and
+// waited for in the same asm statement. The compiler's own wait insertion
+// would place a vmcnt(0) before any LDS read while ordinary global loads and
+// LDS-DMA copies are both in flight (it cannot order the two kinds), which
+// would make the tile-start C loads block the first K steps; these reads
+// follow the explicit stage waits instead.
+__device__ __forceinline__ void oz_ds_read6(v4i& r0, v4i& r1, v4i& r2, v4i& r3, v4i& r4, v4i& r5,
+                                            uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3,
+                                            uint32_t a4, uint32_t a5) {
+  asm volatile(
+      "ds_read_b128 %0, %6\n\t"
+      "ds_read_b128 %1, %7\n\t"
+      "ds_read_b128 %2, %8\n\t"
+      "ds_read_b128 %3, %9\n\t"
+      "ds_read_b128 %4, %10\n\t"
+      "ds_read_b128 %5, %11\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5)
+      : "v"(a0), "v"(a1), "v"(a2), "v"(a3), "v"(a4), "v"(a5)
+      : "memory");
+}
+__device__ __forceinline__ void oz_ds_read9(v4i& r0, v4i& r1, v4i& r2, v4i& r3, v4i& r4, v4i& r5,
+                                            v4i& r6, v4i& r7, v4i& r8, uint32_t a0, uint32_t a1,
+                                            uint32_t a2, uint32_t a3, uint32_t a4, uint32_t a5,
+                                            uint32_t a6, uint32_t a7, uint32_t a8) {
+  asm volatile(
+      "ds_read_b128 %0, %9\n\t"
+      "ds_read_b128 %1, %10\n\t"
+      "ds_read_b128 %2, %11\n\t"
+      "ds_read_b128 %3, %12\n\t"
+      "ds_read_b128 %4, %13\n\t"
+      "ds_read_b128 %5, %14\n\t"
+      "ds_read_b128 %6, %15\n\t"
+      "ds_read_b128 %7, %16\n\t"
+      "ds_read_b128 %8, %17\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5), "=&v"(r6),
+        "=&v"(r7), "=&v"(r8)
+      : "v"(a0), "v"(a1), "v"(a2), "v"(a3), "v"(a4), "v"(a5), "v"(a6), "v"(a7), "v"(a8)
+      : "memory");
+}
+// All fragments of a K step from stage st (TA = 2, TB = 1, S = 2 or 3).
+template <class Cfg, int S, int TA, int TB>
+__device__ __forceinline__ void oz_read_step_asm(const signed char* st, v4i (&fa)[S][TA],
+                                                 v4i (&fb)[S][TB], int offA, int offB) {
+  static_assert(TA == 2 && TB == 1 && (S == 2 || S == 3), "asm fragment reads");
+  const uint32_t base = oz_lds_addr(st);
+  auto A = [&](int i, int ta) { return base + i * Cfg::BM * OZ_BK + offA + ta * 32 * 16; };
+  auto B = [&](int j) { return base + Cfg::ABYTES + j * Cfg::BN * OZ_BK + offB; };
+  if constexpr (S == 2)
+    oz_ds_read6(fb[0][0], fb[1][0], fa[0][0], fa[0][1], fa[1][0], fa[1][1], B(0), B(1), A(0, 0),
+                A(0, 1), A(1, 0), A(1, 1));
+  else
+    oz_ds_read9(fb[0][0], fb[1][0], fb[2][0], fa[0][0], fa[0][1], fa[1][0], fa[1][1], fa[2][0],
+                fa[2][1], B(0), B(1), B(2), A(0, 0), A(0, 1), A(1, 0), A(1, 1), A(2, 0), A(2, 1));
+}
+
 // One K step: st_cur's stage is refilled with step k + NST (tile pointers a, b
 // at that step); the fragments of step k are in (fa, fb); those of step k + 1
 // are read from st_nxt. sched_barrier keeps the compiler from moving the
 // next step's reads back next to their uses or the barrier ahead of the
 // first pair.
-template <class Cfg, int S, int TA, int TB, bool SWAP = false>
+// VMN: the wait count before the barrier (-1: the copies of the NST - 2 later
+// steps stay in flight; more when other loads were issued after step k + 1's
+// copies).
+// ASMRD: the next step's fragments are read after all of this step's
+// products, by oz_read_step_asm (used while the tile-start C loads may be in
+// flight).
+template <class Cfg, int S, int TA, int TB, bool SWAP = false, int VMN = -1, bool ASMRD = false>
 __device__ __forceinline__ void ozp_step(signed char* __restrict__ st_cur,
                                          const signed char* __restrict__ st_nxt,
                                          v16i (&acc)[S][TA][TB], v4i (&fa)[S][TA],
@@ -584,17 +652,18 @@ __device__ __forceinline__ void ozp_step(signed char* __restrict__ st_cur,
         // copies of step k + 1 landed (those of the later steps stay in
         // flight), and this wavefront's reads of step k's fragments are done;
         // after the barrier every wavefront's are, so st_cur is free
-        __builtin_amdgcn_s_waitcnt(oz_vmcnt((Cfg::NST - 2) * Cfg::COPIES) & OZ_LGKM0);
+        __builtin_amdgcn_s_waitcnt(oz_vmcnt(VMN >= 0 ? VMN : (Cfg::NST - 2) * Cfg::COPIES) &
+                                   OZ_LGKM0);
         __builtin_amdgcn_s_barrier();
         ozp_issue<Cfg, S>(st_cur, a, b, slabA, slabB, tid);
       }
-      if (j == S - 1 - i) {  // last use of B fragment j
+      if (!ASMRD && j == S - 1 - i) {  // last use of B fragment j
 #pragma unroll
         for (int tb = 0; tb < TB; ++tb)
           fb[j][tb] = *reinterpret_cast<const v4i*>(st_nxt + Cfg::ABYTES + j * Cfg::BN * OZ_BK +
                                                     offB + tb * 32 * 16);
       }
-      if (j == 0) {  // last use of A fragment i
+      if (!ASMRD && j == 0) {  // last use of A fragment i
 #pragma unroll
         for (int ta = 0; ta < TA; ++ta)
           fa[i][ta] = *reinterpret_cast<const v4i*>(st_nxt + i * Cfg::BM * OZ_BK + offA +
@@ -603,6 +672,7 @@ __device__ __forceinline__ void ozp_step(signed char* __restrict__ st_cur,
       __builtin_amdgcn_sched_barrier(0);
     }
   }
+  if constexpr (ASMRD) oz_read_step_asm<Cfg, S, TA, TB>(st_nxt, fa, fb, offA, offB);
 }
 
 // Kp: a multiple of 4 K steps (128 bytes). The stage arrays (SB bytes each,
@@ -711,8 +781,10 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
   __builtin_amdgcn_s_waitcnt(oz_vmcnt((NST - 1) * Cfg::COPIES + (PRE ? TA * TB * 16 : 0)));
   __builtin_amdgcn_s_barrier();
   v4i fa[S][TA], fb[S][TB];
+  constexpr bool PEEL = PRE && NST == 4 && TA == 2 && TB == 1 && S <= 3;
+  if constexpr (PEEL) oz_read_step_asm<Cfg, S, TA, TB>(st0, fa, fb, offA, offB);
 #pragma unroll
-  for (int q = 0; q < S; ++q) {
+  for (int q = 0; q < (PEEL ? 0 : S); ++q) {
 #pragma unroll
     for (int ta = 0; ta < TA; ++ta)
       fa[q][ta] = *reinterpret_cast<const v4i*>(st0 + q * Cfg::BM * OZ_BK + offA + ta * 32 * 16);
@@ -723,8 +795,29 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
   }
   // past the last K step the refills repeat the last step (never read) and
   // the last step's next-fragment reads fetch a stage that is never used, so
-  // the loop body is the same every time
-  for (int t = 0; t < nk; t += NST) {
+  // the loop body is the same every time. With C read at the start of the
+  // tile (EPI 1), the first NST - 1 steps leave those loads in flight too
+  // (their copies were issued before them), so C streams in under the first
+  // stages' products; the wait of step NST - 1 (for copies issued after the
+  // C loads) takes what is left of it. nk >= 4 (Kp is a multiple of 128).
+  int t0 = 0;
+  if constexpr (PEEL) {
+    constexpr int VP = (NST - 2) * Cfg::COPIES + TA * TB * 16;
+    static_assert(VP <= 63, "vmcnt");
+    ozp_step<Cfg, S, TA, TB, true, VP, true>(st0, st1, acc, fa, fb, Ab + min(4, nk - 1) * kstepA,
+                                       Bb + min(4, nk - 1) * kstepB, slabA, slabB, offA, offB,
+                                       tid);
+    ozp_step<Cfg, S, TA, TB, true, VP, true>(st1, st2, acc, fa, fb, Ab + min(5, nk - 1) * kstepA,
+                                       Bb + min(5, nk - 1) * kstepB, slabA, slabB, offA, offB,
+                                       tid);
+    ozp_step<Cfg, S, TA, TB, true, VP, true>(st2, st3, acc, fa, fb, Ab + min(6, nk - 1) * kstepA,
+                                       Bb + min(6, nk - 1) * kstepB, slabA, slabB, offA, offB,
+                                       tid);
+    ozp_step<Cfg, S, TA, TB, true>(st3, st0, acc, fa, fb, Ab + min(7, nk - 1) * kstepA,
+                                   Bb + min(7, nk - 1) * kstepB, slabA, slabB, offA, offB, tid);
+    t0 = NST;
+  }
+  for (int t = t0; t < nk; t += NST) {
     int k = min(t + NST, nk - 1);
     ozp_step<Cfg, S, TA, TB, (EPI > 0)>(st0, st1, acc, fa, fb, Ab + k * kstepA, Bb + k * kstepB, slabA, slabB,
                              offA, offB, tid);
