@@ -175,6 +175,56 @@ def gemv_profile(N, reps=10):
             "peak_GBps": PEAK_HBM_GBS, "kernels": res}
 
 
+def update_profile(shapes=((30720, 1024), (15360, 1024)), reps=5):
+    """HIP-event times of the int8-sliced trailing-update kernel alone
+    (mpdok_ozaki_apply_sliced after one slicing call) per slice count S = 2 .. 5,
+    at the LU's panel-pair shape (m = n, K = 1024) on random FP64 operands."""
+    import ctypes
+    import torch
+    from mpdok_amd import _dev, _lib
+    L = _lib.load()
+    out = {"metric": "int8-sliced trailing update alone", "unit": "ms", "peak_int8_TOPS":
+           PEAK_INT8_TOPS, "peak_hbm_GBps": PEAK_HBM_GBS, "cases": []}
+    dev = _dev.device()
+    for m, K in shapes:
+        g = torch.Generator(device=dev).manual_seed(0)
+        A = torch.randn(K, m, dtype=torch.float64, device=dev, generator=g).t()   # m x K, col-major
+        B = torch.randn(m, K, dtype=torch.float64, device=dev, generator=g).t()   # K x m, col-major
+        C = torch.randn(m, m, dtype=torch.float64, device=dev, generator=g).t()
+        ws = torch.empty(L.mpdok_ozaki_workspace_bytes(m, m, K, 5), dtype=torch.uint8, device=dev)
+        sc = torch.zeros(2, dtype=torch.int32, device=dev)
+        su, er = ctypes.c_int(0), ctypes.c_int(0)
+        st = L.mpdok_ozaki_gemm_sub_adaptive(_dev.stream(), 5, m, m, K, A.data_ptr(), m,
+                                             B.data_ptr(), K, C.data_ptr(), m, ws.data_ptr(), 2, 2,
+                                             0, -2 ** 31, sc.data_ptr(), ctypes.byref(su),
+                                             ctypes.byref(er))
+        assert st == 0, L.mpdok_last_error()
+        for S in (2, 3, 4, 5):
+            def run():
+                r = L.mpdok_ozaki_apply_sliced(_dev.stream(), 5, m, m, K, C.data_ptr(), m,
+                                               ws.data_ptr(), S, sc.data_ptr())
+                assert r == 0, L.mpdok_last_error()
+            run()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+            torch.cuda.synchronize()
+            for r in range(reps):
+                ev[2 * r].record()
+                run()
+                ev[2 * r + 1].record()
+            torch.cuda.synchronize()
+            ms = min(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps))
+            ops = 2.0 * m * m * K * S * (S + 1) / 2
+            roof = max(ops / (PEAK_INT8_TOPS * 1e12), 16.0 * m * m / (PEAK_HBM_GBS * 1e9))
+            out["cases"].append({"m": m, "n": m, "K": K, "S": S, "ms": ms,
+                                 "ps_per_element": ms * 1e9 / (m * m),
+                                 "int8_TOPS": ops / (ms * 1e-3) / 1e12,
+                                 "algorithmic_TFLOPs": 2.0 * m * m * K / (ms * 1e-3) / 1e12,
+                                 "frac_of_mixed_roofline": roof / (ms * 1e-3)})
+        del A, B, C, ws
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,7 +234,8 @@ def main():
     ap.add_argument("--slices", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--nb", type=int, default=512)
-    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=24576,
+                    help="points of the bounded CPU sample (about 30 s of LAPACK on 16 cores)")
     ap.add_argument("--cpu-gmres-iters", type=int, default=20,
                     help="SciPy GMRES iterations on the full matrix in the CPU baseline (0: skip)")
     ap.add_argument("--cpu-full", action="store_true",
@@ -192,6 +243,9 @@ def main():
     ap.add_argument("--gemv-profile", action="store_true",
                     help="only measure the GEMV / residual kernels and the read-only ceiling "
                          "at N (HIP events) and print them as one JSON line")
+    ap.add_argument("--update-profile", action="store_true",
+                    help="only time the int8-sliced trailing-update kernel per slice count at "
+                         "the LU's panel-pair shape and print one JSON line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip torch solve and plain GMRES")
     ap.add_argument("--configs", action="store_true",
@@ -211,6 +265,9 @@ def main():
     dev = _dev.device()
     if args.gemv_profile:
         print(json.dumps(gemv_profile(N)), flush=True)
+        return 0
+    if args.update_profile:
+        print(json.dumps(update_profile()), flush=True)
         return 0
     log(f"[bench] device {torch.cuda.get_device_name(0)}  N={N}  slices={args.slices}")
     X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
@@ -476,9 +533,10 @@ def other_configs(log):
     torch.cuda.empty_cache()
 
     # factor once, then 72 right-hand sides (the reference's weather-front time
-    # series, run_demo.py:75-109; rbf_demo.ipynb:247): the reference-default
-    # LUIRSolver (FP32 LU with partial pivoting, FP64 refinement) at N = 8192
-    # and 32768, 72 solve_factored calls one at a time and as one block
+    # series, run_demo.py:75-109 with its TOL = 1e-5 and maxiter_outer = 3;
+    # rbf_demo.ipynb:247): the reference-default LUIRSolver (FP32 LU with
+    # partial pivoting, FP64 refinement) at N = 8192 and 32768, 72
+    # solve_factored calls one at a time and as one block
     from mpdok_amd import rbf_kernel
     for N in (8192, 32768):
         coords = rbf_kernel.synthetic_coords(N, seed=0)
@@ -487,10 +545,12 @@ def other_configs(log):
         lu = LUIRSolver()
         lu.factor(A)
         _, tfac = timed(lambda: lu.factor(A))
-        xs = lu.solve_factored(Bs[:, 0].contiguous())
-        _, tone = timed(lambda: [lu.solve_factored(Bs[:, t].contiguous()) for t in range(72)])
-        Xb = lu.solve_factored(Bs)
-        _, tblk = timed(lambda: lu.solve_factored(Bs))
+        kw72 = dict(tol=1e-5, maxiter_outer=3)
+        xs = lu.solve_factored(Bs[:, 0].contiguous(), **kw72)
+        _, tone = timed(lambda: [lu.solve_factored(Bs[:, t].contiguous(), **kw72)
+                                 for t in range(72)])
+        Xb = lu.solve_factored(Bs, **kw72)
+        _, tblk = timed(lambda: lu.solve_factored(Bs, **kw72))
         rels = [float(_ops.norm(_ops.residual(A, Xb[:, t].contiguous(), Bs[:, t].contiguous(),
                                               mode="fp64")) / _ops.norm(Bs[:, t].contiguous()))
                 for t in range(72)]
@@ -499,7 +559,8 @@ def other_configs(log):
             "solve_block_s_per_rhs": tblk / 72, "max_relres": max(rels),
             "all_converged": bool(all(lu.last_info["converged"])), "gamma": gam,
             "method": "LUIRSolver() defaults: FP32 getrf-style LU with partial pivoting, FP64 "
-                      "residual and update, tol 1e-11, maxiter_outer 3"}
+                      "residual and update; solve_factored(tol=1e-5, maxiter_outer=3) as "
+                      "run_demo.py:35,90"}
         log(f"[configs] factor once N={N} {res[f'factor_once_72_rhs_n{N}']}")
         lu.free_factored()
         del A, Bs, Xb, lu
@@ -535,15 +596,8 @@ def other_configs(log):
     del o
     torch.cuda.empty_cache()
 
-    # streamed FP32 operator from page-locked host memory, at the largest N
-    # this process's host memory holds comfortably
+    # streamed FP32 operator from page-locked host memory (17 GB at N = 65536)
     N = 65536
-    try:
-        avail = int([l for l in open("/proc/meminfo") if l.startswith("MemAvailable")][0].split()[1]) * 1024
-        if avail > 3 * 131072 ** 2 * 4:
-            N = 131072
-    except (OSError, IndexError, ValueError):
-        pass
     X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
     o = MPDOKOOCSolver(tile_rows=4096)
     _, tb = timed(lambda: o.build(X, reg=1e-4, store="ram", verbose=False, kernel="matern32",

@@ -347,6 +347,12 @@ int mpdok_ozaki_gemm_sub_adaptive(hipStream_t s, int smax, int m, int n, int K, 
                                   int64_t lda, const double* B, int64_t ldb, double* C,
                                   int64_t ldc, void* work, int s_fixed, int smin, int delta,
                                   int e0_in, int* dev_scratch, int* s_used, int* e_ref);
+/* C -= the product of the leading s_fixed of the smax digits that the last
+ * mpdok_ozaki_gemm_sub_adaptive call with the same (smax, m, n, K, work)
+ * cut, without slicing again (times the trailing-update kernel alone).
+ * dev_scratch: 2 device ints. No reference counterpart: measurement. */
+int mpdok_ozaki_apply_sliced(hipStream_t s, int smax, int m, int n, int K, double* C, int64_t ldc,
+                             void* work, int s_fixed, int* dev_scratch);
 
 /* ---- refinement drivers -----------------------------------------------------
  * GMRES-IR: FP64 outer refinement (residual mode 0 FP64 / 1 Dot2) around one

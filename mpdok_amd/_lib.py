@@ -102,6 +102,7 @@ SIGNATURES = {
     "mpdok_gemm": (I, [P, I, I, I, I, D, P, I64, P, I64, D, P, I64, I]),
     "mpdok_ozaki_workspace_bytes": (SZ, [I, I, I, I]),
     "mpdok_ozaki_gemm_sub": (I, [P, I, I, I, I, P, I64, P, I64, P, I64, P]),
+    "mpdok_ozaki_apply_sliced": (I, [P, I, I, I, I, P, I64, P, I, P]),
     "mpdok_ozaki_gemm_sub_adaptive": (I, [P, I, I, I, I, P, I64, P, I64, P, I64, P, I, I, I, I, P,
                                           PI, PI]),
     # refinement drivers

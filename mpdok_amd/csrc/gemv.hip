@@ -155,16 +155,24 @@ __global__ __launch_bounds__(256) void gemv_cm_partial_kernel(const T* __restric
   }
 }
 
+// y = alpha * (sum of the column-chunk partial sums), 64 rows per workgroup:
+// wavefront q sums chunks q, q + 4, q + 8, ... of its lane's row, and the four
+// sums are added in the fixed order (s0 + s1) + (s2 + s3), so every row uses
+// the same summation order on every call.
 template <typename T>
 __global__ __launch_bounds__(256) void gemv_cm_reduce_kernel(const T* __restrict__ part, int M,
                                                              int nchunks, T* __restrict__ y,
                                                              T alpha, const int* __restrict__ skip) {
   if (skip != nullptr && *skip) return;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= M) return;
+  __shared__ T sh[4][64];
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + r;
   T s = T(0);
-  for (int c = 0; c < nchunks; ++c) s += part[(int64_t)c * M + i];
-  y[i] = alpha * s;
+  if (i < M)
+    for (int c = q; c < nchunks; c += 4) s += part[(int64_t)c * M + i];
+  sh[q][r] = s;
+  __syncthreads();
+  if (q == 0 && i < M) y[i] = alpha * ((sh[0][r] + sh[1][r]) + (sh[2][r] + sh[3][r]));
 }
 
 // ---------------------------------------------------------------------------
@@ -441,7 +449,7 @@ static int gemv_launch(hipStream_t s, const T* A, int64_t lda, int order, int M,
       gemv_cm_partial_kernel<T, true><<<grid, 256, 0, s>>>(A, lda, M, N, x, part, cpc, skip);
     else
       gemv_cm_partial_kernel<T, false><<<grid, 256, 0, s>>>(A, lda, M, N, x, part, cpc, skip);
-    gemv_cm_reduce_kernel<T><<<(M + 255) / 256, 256, 0, s>>>(part, M, nch, y, alpha, skip);
+    gemv_cm_reduce_kernel<T><<<(M + 63) / 64, 256, 0, s>>>(part, M, nch, y, alpha, skip);
   }
   MPDOK_HIP(hipGetLastError());
   return OK;

@@ -1226,6 +1226,18 @@ MPDOK_API int mpdok_ozaki_gemm_sub(hipStream_t s, int S, int m, int n, int K, co
   return mpdok::ozaki_gemm(s, S, m, n, K, A, lda, B, ldb, C, ldc, work);
 }
 
+// C -= the product of the leading s_fixed of the smax digits the last
+// mpdok_ozaki_gemm_sub_adaptive call with the same (smax, m, n, K, work) cut,
+// without slicing again: times the trailing-update kernel alone.
+// dev_scratch: 2 device ints (the count is written to dev_scratch[1]).
+MPDOK_API int mpdok_ozaki_apply_sliced(hipStream_t s, int smax, int m, int n, int K, double* C,
+                                       int64_t ldc, void* work, int s_fixed, int* dev_scratch) {
+  MPDOK_REQUIRE(s_fixed >= 2 && s_fixed <= smax, "ozaki_apply_sliced: s_fixed=%d outside [2, %d]",
+                s_fixed, smax);
+  MPDOK_HIP(hipMemcpyAsync(dev_scratch + 1, &s_fixed, sizeof(int), hipMemcpyHostToDevice, s));
+  return mpdok::ozaki_apply_dyn(s, smax, m, n, K, 0, m, 0, n, C, ldc, work, dev_scratch + 1);
+}
+
 // The adaptive pieces, for parity tests: slice with smax slices, then
 // C -= the product of the leading S slices, with S given (s_fixed >= 2) or
 // picked by the decision rule against a reference exponent sum e0_in
