@@ -38,6 +38,10 @@ template <typename TI, typename TO>
 int copy2d(hipStream_t s, const TI* src, int64_t lds, TO* dst, int64_t ldd, int m, int n);
 template <typename T>
 int laswp(hipStream_t s, T* A, int64_t lda, int c0, int c1, const int* ipiv, int r0, int r1);
+// getf2 on the sub-panel [j0, j0 + w) x rows [j0, N), w <= 64, in one
+// cooperative launch with implicit pivoting: rows stay in place, ipiv[j0 ..
+// j0 + w) receives LAPACK's interchanges, which the caller applies to every
+// column (laswp over [0, N)).
 template <typename T>
 int panel_gepp(hipStream_t s, T* A, int64_t lda, int N, int j0, int w, int* ipiv, int* info,
                void* work);

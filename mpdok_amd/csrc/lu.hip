@@ -384,9 +384,10 @@ static int panel_factor(LUHandle* h, hipStream_t s, int k, int kb, T* W, bool sl
     T* Ui = Uinv + (size_t)(j / 64) * 4096;
     const int m2 = N - j - jb, n2 = k + kb - j - jb;
     if (h->pivot == PIV_PARTIAL) {
+      // the sub-panel is factored with implicit pivoting (rows in place);
+      // its interchanges then move whole rows, panel columns included
       MPDOK_TRY(panel_gepp<T>(s, A, ld, N, j, jb, h->ipiv, h->d_info, h->piv_work));
-      MPDOK_TRY(laswp<T>(s, A, ld, 0, j, h->ipiv, j, j + jb));
-      MPDOK_TRY(laswp<T>(s, A, ld, j + jb, N, h->ipiv, j, j + jb));
+      MPDOK_TRY(laswp<T>(s, A, ld, 0, N, h->ipiv, j, j + jb));
       MPDOK_TRY(diag_factor<T>(s, at(j, j), ld, jb, Li, Ui, 64, h->d_info, j, 0));
       // U12 = inv(L11) A12, in place (m = jb <= 64: each workgroup its own columns)
       if (n2 > 0)

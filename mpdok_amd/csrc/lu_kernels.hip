@@ -907,111 +907,231 @@ template int laswp<double>(hipStream_t, double*, int64_t, int, int, const int*, 
 template int laswp<float>(hipStream_t, float*, int64_t, int, int, const int*, int, int);
 
 // ---------------------------------------------------------------------------
-// Partial pivoting inside a sub-panel of width w (columns j0..j0+w), rows
-// [j, N). Column c = j - j0 of the sub-panel:
-//   pivot_kernel : max |A[i, j]| over i >= j (ties -> smallest i); the last
-//                  block to finish reduces the block maxima, records ipiv[j]
-//                  and swaps rows j and p inside the sub-panel columns.
-//   elim_kernel  : A[i, j] /= A[j, j]; A[i, j+1:j0+w] -= A[i, j] A[j, j+1:j0+w].
+// Partial pivoting (LAPACK getf2 on one sub-panel) in one cooperative launch.
+// Sub-panel: columns [j0, j0 + w), w <= 64, rows [j0, N). Workgroup b holds
+// rows j0 + 256 b + t (thread t) with their w values in registers for the
+// whole sub-panel. Pivoting is implicit: rows stay where they are, a flag
+// marks the rows already chosen, and every column takes one grid barrier:
+//   each workgroup finds its largest |a_ic| over the unchosen rows and
+//   publishes that value, its row index and the row's w values (double
+//   buffered by column parity); after the barrier every workgroup picks the
+//   overall maximum (ties: the smallest row index), reads the winning row as
+//   the pivot row u, and its unchosen rows take l = a_ic / u_c (unscaled if
+//   u_c = 0, which sets info) and a_ic' -= l u_c' for c' > c.
+// The multipliers, updates and their order are getf2's, so after the rows
+// are permuted (laswp over every column with the returned ipiv) the panel
+// holds getf2's L and U. Workgroup 0 finally turns the sequence of chosen
+// rows into LAPACK's ipiv (interchanges applied in order) with two index
+// maps over the panel's rows in the workspace.
 // ---------------------------------------------------------------------------
-constexpr int PIV_ROWS = 1024;
+constexpr int GP_W = 64;     // widest sub-panel
+constexpr int GP_HDR = 1024;  // barrier words at the start of the workspace
 
-template <typename T>
-__global__ __launch_bounds__(256) void pivot_kernel(T* __restrict__ A, int64_t lda, int N, int j,
-                                                    int j0, int w, double* __restrict__ pval,
-                                                    int* __restrict__ pidx,
-                                                    unsigned* __restrict__ counter,
-                                                    int* __restrict__ ipiv, int* __restrict__ info) {
-  const T* col = A + (int64_t)j * lda;
-  const int base = j + blockIdx.x * PIV_ROWS;
-  double best = -1.0;
-  int bi = 0x7fffffff;
-  for (int i = base + threadIdx.x; i < min(N, base + PIV_ROWS); i += 256) {
-    const double v = fabs((double)col[i]);
-    if (v > best) { best = v; bi = i; }
-  }
-  __shared__ double sv[256];
-  __shared__ int si[256];
-  sv[threadIdx.x] = best;
-  si[threadIdx.x] = bi;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      const double v2 = sv[threadIdx.x + off];
-      const int i2 = si[threadIdx.x + off];
-      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
-        sv[threadIdx.x] = v2;
-        si[threadIdx.x] = i2;
+struct GeppWork {
+  unsigned* bar;   // barrier words (gp_grid_barrier)
+  double* cval;    // [2][G] largest |a| per workgroup
+  int* cidx;       // [2][G] its row
+  double* crow;    // [2][G][GP_W] its values
+  int* rowat;      // [N] row at a position of the panel (interchange maps)
+  int* posof;      // [N] position of a row
+};
+
+// Grid barrier of the panel kernel, entered by thread 0 of each workgroup
+// (the caller synchronises the workgroup around it). bar[0] arrivals,
+// bar[1] generation, bar[2] spin-limit flag.
+__device__ __forceinline__ bool gp_grid_barrier0(unsigned* bar, unsigned nblocks) {
+  const unsigned g = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // the arrival releases this wavefront's candidate stores and, for the last
+  // arrival, acquires every other workgroup's
+  if (__hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nblocks - 1) {
+    __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    // every workgroup of a cooperative launch is resident, so the others
+    // arrive; the bound only turns a broken launch into an error
+    long spins = 0;
+    while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > (1L << 23)) {
+        atomicExch(bar + 2, 1u);
+        return false;
       }
     }
-    __syncthreads();
   }
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    pval[blockIdx.x] = sv[0];
-    pidx[blockIdx.x] = si[0];
-    __threadfence();
-    last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  __shared__ int p;
-  if (threadIdx.x == 0) {
-    double bv = -1.0;
-    int bidx = j;
-    for (int b = 0; b < (int)gridDim.x; ++b) {
-      const double v = __hip_atomic_load(pval + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const int ii = __hip_atomic_load(pidx + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (v > bv) { bv = v; bidx = ii; }
-    }
-    p = bidx;
-    ipiv[j] = bidx;
-    if (bv == 0.0) atomicCAS(info, 0, j + 1);
-    *counter = 0u;
-  }
-  __syncthreads();
-  if (p != j && threadIdx.x < w) {
-    T* c = A + (int64_t)(j0 + threadIdx.x) * lda;
-    T a = c[j];
-    c[j] = c[p];
-    c[p] = a;
+  return true;
+}
+
+// (value, index) maximum with ties to the smaller index
+__device__ __forceinline__ void gp_better(double& v, int& i, double v2, int i2) {
+  if (v2 > v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void elim_kernel(T* __restrict__ A, int64_t lda, int N, int j,
-                                                   int j0, int w) {
-  const int i = j + 1 + blockIdx.x * 256 + threadIdx.x;
-  if (i >= N) return;
-  const T piv = A[j + (int64_t)j * lda];
-  T l = A[i + (int64_t)j * lda];
-  if (piv != T(0)) l = l / piv;
-  A[i + (int64_t)j * lda] = l;
-  for (int c = j + 1; c < j0 + w; ++c)
-    A[i + (int64_t)c * lda] = __builtin_fma(-l, A[j + (int64_t)c * lda], A[i + (int64_t)c * lda]);
+__global__ __launch_bounds__(256) void gepp_panel_kernel(T* __restrict__ A, int64_t lda, int N,
+                                                         int j0, int w, int* __restrict__ ipiv,
+                                                         int* __restrict__ info, GeppWork ws) {
+  const int G = gridDim.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int row = j0 + blockIdx.x * 256 + t;
+  const bool valid = row < N;
+  T a[GP_W];
+#pragma unroll
+  for (int c = 0; c < GP_W; ++c) a[c] = (valid && c < w) ? A[row + (int64_t)(j0 + c) * lda] : T(0);
+  bool chosen = !valid;
+  __shared__ double sv[4];
+  __shared__ int si[4];
+  __shared__ int s_best;  // this workgroup's candidate row / the pivot row
+  __shared__ double s_max;
+  __shared__ double cand[GP_W];
+  __shared__ int s_ok;
+  __shared__ T u[GP_W];
+  __shared__ int seq[GP_W];
+  if (t == 0) s_ok = 1;
+  for (int c = 0; c < w; ++c) {
+    T ac = T(0);
+#pragma unroll
+    for (int q = 0; q < GP_W; ++q)
+      if (q == c) ac = a[q];
+    // this workgroup's candidate (a NaN counts as the largest, so a row is
+    // always chosen while unchosen rows remain)
+    const double fa = fabs((double)ac);
+    double v = chosen ? -1.0 : (fa == fa ? fa : HUGE_VAL);
+    int vi = chosen ? 0x7fffffff : row;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      gp_better(v, vi, __shfl_xor(v, off, 64), __shfl_xor(vi, off, 64));
+    if (lane == 0) {
+      sv[wv] = v;
+      si[wv] = vi;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double bv = sv[0];
+      int bi = si[0];
+      for (int q = 1; q < 4; ++q) gp_better(bv, bi, sv[q], si[q]);
+      s_best = bi;
+      s_max = bv;
+    }
+    __syncthreads();
+    if (row == s_best) {
+#pragma unroll
+      for (int q = 0; q < GP_W; ++q) cand[q] = (double)a[q];
+    }
+    __syncthreads();
+    // wavefront 0 publishes the candidate, takes the grid barrier and reads
+    // the winner back: every global access of the exchange stays in one
+    // wavefront, so its own release and acquire order them
+    const int slot = c & 1;
+    if (wv == 0) {
+      ws.crow[((size_t)slot * G + blockIdx.x) * GP_W + lane] = cand[lane];
+      if (lane == 0) {
+        ws.cval[slot * G + blockIdx.x] = s_max;
+        ws.cidx[slot * G + blockIdx.x] = s_best;
+        if (!gp_grid_barrier0(ws.bar, (unsigned)G)) s_ok = 0;
+      }
+      if (__shfl(s_ok, 0, 64)) {
+        double bv = -1.0;
+        int bi = 0x7fffffff, bb = 0;
+        for (int q = lane; q < G; q += 64) {
+          const double v2 = ws.cval[slot * G + q];
+          const int i2 = ws.cidx[slot * G + q];
+          if (v2 > bv || (v2 == bv && i2 < bi)) {
+            bv = v2;
+            bi = i2;
+            bb = q;
+          }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const double v2 = __shfl_xor(bv, off, 64);
+          const int i2 = __shfl_xor(bi, off, 64), b2 = __shfl_xor(bb, off, 64);
+          if (v2 > bv || (v2 == bv && i2 < bi)) {
+            bv = v2;
+            bi = i2;
+            bb = b2;
+          }
+        }
+        u[lane] = (T)ws.crow[((size_t)slot * G + bb) * GP_W + lane];
+        if (lane == 0) {
+          s_best = bi;
+          seq[c] = bi;
+          if (bv == 0.0 && blockIdx.x == 0) atomicCAS(info, 0, j0 + c + 1);
+        }
+      }
+    }
+    __syncthreads();
+    if (!s_ok) {
+      if (t == 0) atomicCAS(info, 0, -1);  // reported as a failed factorisation
+      return;
+    }
+    if (row == s_best) chosen = true;
+    if (!chosen) {
+      const T uc = u[c];
+      const T l = uc != T(0) ? ac / uc : ac;
+#pragma unroll
+      for (int q = 0; q < GP_W; ++q) {
+        if (q == c) a[q] = l;
+        if (q > c) a[q] = __builtin_fma(-l, u[q], a[q]);
+      }
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < GP_W; ++c)
+      if (c < w) A[row + (int64_t)(j0 + c) * lda] = a[c];
+  }
+  if (blockIdx.x != 0) return;
+  // ipiv: bring row seq[k] to position j0 + k by one interchange per k, in
+  // order (maps over the panel's rows, relative to j0)
+  const int h = N - j0;
+  for (int q = t; q < h; q += 256) {
+    ws.rowat[q] = q;
+    ws.posof[q] = q;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int k = 0; k < w; ++k) {
+      const int r = seq[k] - j0;
+      if (r < 0 || r >= h) {  // cannot happen while a row is chosen at every column
+        atomicCAS(info, 0, -1);
+        return;
+      }
+      const int qpos = ws.posof[r];
+      const int rk = ws.rowat[k];
+      ws.rowat[qpos] = rk;
+      ws.posof[rk] = qpos;
+      ws.rowat[k] = r;
+      ws.posof[r] = k;
+      ipiv[j0 + k] = j0 + qpos;
+    }
+  }
 }
 
 template <typename T>
 int panel_gepp(hipStream_t s, T* A, int64_t lda, int N, int j0, int w, int* ipiv, int* info,
                void* work) {
-  // work: pval[nblk] (double), pidx[nblk] (int), counter (unsigned)
-  const int maxblk = (N + PIV_ROWS - 1) / PIV_ROWS;
-  double* pval = static_cast<double*>(work);
-  int* pidx = reinterpret_cast<int*>(pval + maxblk);
-  unsigned* counter = reinterpret_cast<unsigned*>(pidx + maxblk + 1);
-  for (int j = j0; j < j0 + w && j < N; ++j) {
-    const int nblk = (N - j + PIV_ROWS - 1) / PIV_ROWS;
-    pivot_kernel<T><<<nblk, 256, 0, s>>>(A, lda, N, j, j0, w, pval, pidx, counter, ipiv, info);
-    if (j + 1 < N)
-      elim_kernel<T><<<(N - j - 1 + 255) / 256, 256, 0, s>>>(A, lda, N, j, j0, w);
-  }
-  MPDOK_HIP(hipGetLastError());
+  MPDOK_REQUIRE(w >= 1 && w <= GP_W && j0 + w <= N, "panel_gepp: bad sub-panel");
+  const int G = (N - j0 + 255) / 256;
+  char* p = static_cast<char*>(work);
+  GeppWork ws;
+  ws.bar = reinterpret_cast<unsigned*>(p);
+  const int gmax = (N + 255) / 256;
+  ws.cval = reinterpret_cast<double*>(p + GP_HDR);
+  ws.cidx = reinterpret_cast<int*>(ws.cval + 2 * gmax);
+  ws.crow = reinterpret_cast<double*>(p + GP_HDR + 2 * gmax * 16);
+  ws.rowat = reinterpret_cast<int*>(ws.crow + (size_t)2 * gmax * GP_W);
+  ws.posof = ws.rowat + N;
+  MPDOK_HIP(hipMemsetAsync(ws.bar, 0, GP_HDR, s));
+  void* args[] = {&A, &lda, &N, &j0, &w, &ipiv, &info, &ws};
+  MPDOK_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&gepp_panel_kernel<T>),
+                                       dim3(G), dim3(256), args, 0, s));
   return OK;
 }
 size_t panel_gepp_workspace_bytes(int N) {
-  const int maxblk = (N + PIV_ROWS - 1) / PIV_ROWS;
-  return (size_t)maxblk * (sizeof(double) + sizeof(int)) + 64;
+  const size_t gmax = (N + 255) / 256;
+  return GP_HDR + gmax * 2 * 16 + gmax * 2 * GP_W * sizeof(double) + (size_t)2 * N * sizeof(int) + 64;
 }
 template int panel_gepp<double>(hipStream_t, double*, int64_t, int, int, int, int*, int*, void*);
 template int panel_gepp<float>(hipStream_t, float*, int64_t, int, int, int, int*, int*, void*);

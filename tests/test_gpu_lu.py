@@ -379,7 +379,7 @@ def test_lu_nopivot_fp64_matches_oracle(L, N, nb):
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
 
 
-@pytest.mark.parametrize("N", [130, 700])
+@pytest.mark.parametrize("N", [130, 700, 2500])
 def test_lu_partial_pivot_matches_lapack(L, N):
     rng = np.random.default_rng(N + 1)
     A = rng.standard_normal((N, N))
