@@ -104,9 +104,11 @@ int ozaki_slice(hipStream_t s, int S, int m, int n, int K, const double* A, int6
                 const double* B, int64_t ldb, void* work, int parts = 3);
 // C[row0 : row0 + nrows, col0 : col0 + ncols] -= the product of the leading
 // *Sdev slices of the last ozaki_slice call for (smax, m, n, K); row0 a
-// multiple of 128, col0 of 64.
+// multiple of 128, col0 of 64. Ci (if not null): the block's values before
+// the update, read from there (leading dimension ldci) instead of from C.
 int ozaki_apply_dyn(hipStream_t s, int smax, int m, int n, int K, int row0, int nrows, int col0,
-                    int ncols, double* C, int64_t ldc, void* work, const int* Sdev);
+                    int ncols, double* C, int64_t ldc, void* work, const int* Sdev,
+                    const double* Ci = nullptr, int64_t ldci = 0);
 int ozaki_decide(hipStream_t s, int smax, int m, int n, int K, void* work, int smin, int delta,
                  int first, int* e0, int* s_out);
 int ozaki_apply(hipStream_t s, int S, int m, int n, int K, int col0, int ncols, double* C,

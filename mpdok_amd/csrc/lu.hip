@@ -95,6 +95,11 @@ struct LUHandle {
   // decision run on the lookahead stream right after its panels (the main
   // stream updates the group's rows before the rest of its trailing update)
   int rows_side_min = 0;
+  // the input matrix (column-major FP64) while the first trailing update of
+  // the int8-sliced variant reads its values from there instead of from a
+  // copy in the handle (null otherwise)
+  const double* c_in = nullptr;
+  int64_t ld_in = 0;
   int* d_S = nullptr;
   float* d_sc = nullptr;  // FP8 operand scales of the running trailing update
   std::vector<int> steps_S;  // host copy after a factorisation
@@ -494,9 +499,10 @@ static int trailing_block(LUHandle* h, hipStream_t s, int k, int kb, int r_off, 
     MPDOK_HIP(hipEventRecord(h->ev[2 * nev], s));
   }
   if (adaptive_on(h)) {
+    const double* ci = k == 0 && h->c_in ? h->c_in + kb + (int64_t)kb * h->ld_in : nullptr;
     MPDOK_TRY(ozaki_apply_dyn(s, h->slices, m3, n3, kb, r_off, nrows, c_off, ncols,
                               reinterpret_cast<double*>(at(k + kb, k + kb)), ld, oz_ws(h, k),
-                              h->d_S + 1 + step_of(h, k)));
+                              h->d_S + 1 + step_of(h, k), ci, h->ld_in));
   } else if (h->precision == LU_OZAKI) {
     MPDOK_REQUIRE(r_off == 0 && nrows == m3, "trailing update: row ranges need adaptive slicing");
     MPDOK_TRY(ozaki_apply(s, h->slices, m3, n3, kb, c_off, ncols,
@@ -662,7 +668,20 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
     // the step, once it waits for each group's panels.
     auto u_on_main = [&](size_t g) { return g >= 1 && !rows_on_side(g); };
     MPDOK_TRY(group_u_inverses<T>(h, h->side, 0, gs[1], Wp));
-    if (rest_A) MPDOK_TRY(copy_columns(h, s, rest_A, rest_lda, rest_order, gs[1], N));
+    // The first trailing update writes every entry of rows and columns
+    // [gs[1], N): with adaptive int8 slices and a column-major input it reads
+    // them from the input, and only the first group's rows of the later
+    // columns (its block-row solve works on them in place) are copied.
+    h->c_in = nullptr;
+    if (rest_A && rest_order == COL_MAJOR && adaptive_on(h) && gs.size() > 2) {
+      MPDOK_TRY((copy2d<double, double>(s, rest_A + (int64_t)gs[1] * rest_lda, rest_lda,
+                                        reinterpret_cast<double*>(at(0, gs[1])), ld, gs[1],
+                                        N - gs[1])));
+      h->c_in = rest_A;
+      h->ld_in = rest_lda;
+    } else if (rest_A) {
+      MPDOK_TRY(copy_columns(h, s, rest_A, rest_lda, rest_order, gs[1], N));
+    }
     for (size_t g = 0; g + 1 < gs.size(); ++g) {
       const int k = gs[g], kb = gs[g + 1] - k;
       const int n3 = N - k - kb;
@@ -704,6 +723,7 @@ static int factor_impl(LUHandle* h, hipStream_t s, const double* rest_A, int64_t
       if (!u_on_main(g + 1)) MPDOK_TRY(group_u_inverses<T>(h, h->side, k + kb, nk, Wp));
       if (u_on_main(g)) MPDOK_TRY(group_u_inverses<T>(h, s, k, kb, W));
     }
+    h->c_in = nullptr;
     // everything the lookahead stream holds, U inverses included
     MPDOK_HIP(hipEventRecord(h->ev_panel, h->side));
     MPDOK_HIP(hipStreamWaitEvent(s, h->ev_panel, 0));
@@ -743,6 +763,7 @@ int lu_factor(LUHandle* h, hipStream_t s, const double* A, int64_t lda, int orde
   // on the lookahead path only the first step's columns before the
   // factorisation starts (it copies the rest beside the first panel)
   build_schedule(h);
+  h->c_in = nullptr;
   const bool split = lookahead_on(h);
   MPDOK_TRY(copy_columns(h, s, A, lda, order, 0, split ? h->group_start[1] : N));
   const int st = fp64_storage(h->precision)

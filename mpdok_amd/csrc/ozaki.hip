@@ -688,7 +688,8 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
                                          const signed char* __restrict__ As,
                                          const signed char* __restrict__ Bs,
                                          const int* __restrict__ ea, const int* __restrict__ eb,
-                                         double* __restrict__ C, int64_t ldc, int group,
+                                         double* __restrict__ C, int64_t ldc,
+                                         const double* __restrict__ Ci, int64_t ldci, int group,
                                          signed char* __restrict__ st0,
                                          signed char* __restrict__ st1,
                                          signed char* __restrict__ st2,
@@ -752,7 +753,7 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
             const int jo = 8 * (r >> 2) + (r & 3);
-            cpre[ta][tb][r] = __builtin_nontemporal_load(C + i + (int64_t)(jb + jo) * ldc);
+            cpre[ta][tb][r] = __builtin_nontemporal_load(Ci + i + (int64_t)(jb + jo) * ldci);
           }
         }
       }
@@ -768,7 +769,7 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
             const int jo = 8 * (r >> 2) + (r & 3);
             const bool ok = i < m && jb + jo < col0 + ncols;
             const double v =
-                __builtin_nontemporal_load(ok ? C + i + (int64_t)(jb + jo) * ldc : C);
+                __builtin_nontemporal_load(ok ? Ci + i + (int64_t)(jb + jo) * ldci : Ci);
             cpre[ta][tb][r] = ok ? v : 0.0;
           }
         }
@@ -877,7 +878,7 @@ __device__ __forceinline__ void ozp_body(int m, int n, int mp, int np, int Kp, i
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
       const int j = bn + wn + 32 * tb + cph + 2 * c;
-      dst[c] = (i < m && j < col0 + ncols) ? C[i + (int64_t)j * ldc] : 0.0;
+      dst[c] = (i < m && j < col0 + ncols) ? Ci[i + (int64_t)j * ldci] : 0.0;
     }
   };
   auto lds_sync = [] {
@@ -923,7 +924,7 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmp_kernel(
   __shared__ __attribute__((aligned(16))) signed char st2[NST > 2 ? SB : 16];
   __shared__ __attribute__((aligned(16))) signed char st3[NST > 2 ? SB : 16];
   ozp_body<S, WM, WN, TA, TB, NST, OCC, SB>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C, ldc,
-                                            group, st0, st1, st2, st3);
+                                            C, ldc, group, st0, st1, st2, st3);
 }
 
 // The same with the slice count read on the device (*Sdev, 2 .. SMAX): the
@@ -933,7 +934,8 @@ template <int SMAX, int WM, int WN, int TA, int TB, int NST, int OCC, int EPI = 
 __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmd_kernel(
     int m, int n, int mp, int np, int Kp, int col0, int ncols, const signed char* __restrict__ As,
     const signed char* __restrict__ Bs, const int* __restrict__ ea, const int* __restrict__ eb,
-    double* __restrict__ C, int64_t ldc, int group, const int* __restrict__ Sdev) {
+    double* __restrict__ C, int64_t ldc, const double* __restrict__ Ci, int64_t ldci, int group,
+    const int* __restrict__ Sdev) {
   typedef OzPCfg<SMAX, WM, WN, TA, TB, NST, OCC> Cfg;
   // a stage array also stages the epilogue's FP64 tiles (EPI 0)
   constexpr int SBE = (WM * WN + NST - 1) / NST * Cfg::EPI_WAVE;
@@ -947,16 +949,16 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void ozaki_gemmd_kernel(
   // the C prefetch (EPI 1) holds 32 TA TB more registers: S <= 3 only
   if (S <= 2)
     ozp_body<2, WM, WN, TA, TB, NST, OCC, SB, EPI>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
-                                              ldc, group, st0, st1, st2, st3);
+                                              ldc, Ci, ldci, group, st0, st1, st2, st3);
   else if (SMAX > 3 && S == 3)
     ozp_body<3, WM, WN, TA, TB, NST, OCC, SB, EPI>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
-                                              ldc, group, st0, st1, st2, st3);
+                                              ldc, Ci, ldci, group, st0, st1, st2, st3);
   else if (SMAX > 4 && S == 4)
     ozp_body<(SMAX > 4 ? 4 : SMAX), WM, WN, TA, TB, NST, OCC, SB, 0>(
-        m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C, ldc, group, st0, st1, st2, st3);
+        m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C, ldc, Ci, ldci, group, st0, st1, st2, st3);
   else
     ozp_body<SMAX, WM, WN, TA, TB, NST, OCC, SB, (SMAX > 3 ? 0 : EPI)>(m, n, mp, np, Kp, col0, ncols, As, Bs, ea, eb, C,
-                                                 ldc, group, st0, st1, st2, st3);
+                                                 ldc, Ci, ldci, group, st0, st1, st2, st3);
 }
 
 // Slice count of one trailing update from the exponent maxima of its panels:
@@ -1101,11 +1103,12 @@ static int ozaki_apply_s(hipStream_t s, int m, int n, int K, int col0, int ncols
 
 template <int SMAX, int WM, int WN, int TA, int TB, int NST, int OCC, int EPI>
 static int oz_dyn_launch(hipStream_t s, const OzLayout& L, int m, int n, int col0, int ncols,
-                         double* C, int64_t ldc, int group, const int* Sdev) {
+                         double* C, int64_t ldc, const double* Ci, int64_t ldci, int group,
+                         const int* Sdev) {
   typedef OzPCfg<SMAX, WM, WN, TA, TB, NST, OCC> Cfg;
   const int grid = tiles_launch((m + Cfg::BM - 1) / Cfg::BM, (ncols + Cfg::BN - 1) / Cfg::BN);
   ozaki_gemmd_kernel<SMAX, WM, WN, TA, TB, NST, OCC, EPI><<<grid, Cfg::NT, 0, s>>>(
-      m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, group, Sdev);
+      m, n, L.mp, L.np, L.Kp, col0, ncols, L.As, L.Bs, L.ea, L.eb, C, ldc, Ci, ldci, group, Sdev);
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
@@ -1113,9 +1116,11 @@ static int oz_dyn_launch(hipStream_t s, const OzLayout& L, int m, int n, int col
 // sel = (smax - 3) * 2 + epi
 template <int WM, int WN, int TA, int TB, int NST, int OCC>
 static int oz_dyn_pick(hipStream_t s, int sel, const OzLayout& L, int m, int n, int col0, int ncols,
-                       double* C, int64_t ldc, int group, const int* Sdev) {
-#define OZ_DYN(SM, EP) \
-  oz_dyn_launch<SM, WM, WN, TA, TB, NST, OCC, EP>(s, L, m, n, col0, ncols, C, ldc, group, Sdev)
+                       double* C, int64_t ldc, const double* Ci, int64_t ldci, int group,
+                       const int* Sdev) {
+#define OZ_DYN(SM, EP)                                                                         \
+  oz_dyn_launch<SM, WM, WN, TA, TB, NST, OCC, EP>(s, L, m, n, col0, ncols, C, ldc, Ci, ldci, group, \
+                                                  Sdev)
   switch (sel) {
     case 0: return OZ_DYN(3, 0);
     case 1: return OZ_DYN(3, 1);
@@ -1129,9 +1134,18 @@ static int oz_dyn_pick(hipStream_t s, int sel, const OzLayout& L, int m, int n, 
 
 // C[:, col0 : col0 + ncols] -= the product of the leading *Sdev slices of the
 // last ozaki_slice call with smax slices (smax <= 5; *Sdev read on the device).
+// With Ci not null the values subtracted from are read from Ci (leading
+// dimension ldci, the same block) and the results written to C: the first
+// update of a factorisation reads the input matrix in place of a copy of it.
 int ozaki_apply_dyn(hipStream_t s, int smax, int m, int n, int K, int row0, int nrows, int col0,
-                    int ncols, double* C, int64_t ldc, void* work, const int* Sdev) {
+                    int ncols, double* C, int64_t ldc, void* work, const int* Sdev,
+                    const double* Ci, int64_t ldci) {
   if (nrows <= 0 || ncols <= 0 || K <= 0) return OK;
+  if (Ci == nullptr) {
+    Ci = C;
+    ldci = ldc;
+  }
+  MPDOK_REQUIRE(ldci >= m, "ozaki_apply_dyn: ld of the input C %lld < m = %d", (long long)ldci, m);
   MPDOK_REQUIRE(col0 % 64 == 0 && col0 + ncols <= n, "ozaki_apply_dyn: bad column range");
   MPDOK_REQUIRE(row0 % 128 == 0 && row0 >= 0 && row0 + nrows <= m, "ozaki_apply_dyn: bad row range");
   MPDOK_REQUIRE(smax >= 3 && smax <= 5, "ozaki_apply_dyn: smax must be in [3, 5], got %d", smax);
@@ -1141,6 +1155,7 @@ int ozaki_apply_dyn(hipStream_t s, int smax, int m, int n, int K, int row0, int 
   L.As += (size_t)row0 * OZ_BK;
   L.ea += row0;
   C += row0;
+  Ci += row0;
   m = nrows;
   const int g = tuning_get(TUNE_OZAKI_GROUP);
   const int group = g > 0 ? g : GROUP_M;
@@ -1154,10 +1169,10 @@ int ozaki_apply_dyn(hipStream_t s, int smax, int m, int n, int K, int row0, int 
   const int epi = tuning_get(TUNE_OZAKI_EPI) == 0 ? 1 : 0;
   const int sel = (smax - 3) * 2 + epi;  // 0 .. 5
   switch (shape) {
-    case 1: return oz_dyn_pick<2, 2, 2, 1, 2, 2>(s, sel, L, m, n, col0, ncols, C, ldc, group, Sdev);
-    case 2: return oz_dyn_pick<2, 4, 2, 1, 2, 1>(s, sel, L, m, n, col0, ncols, C, ldc, group, Sdev);
-    case 3: return oz_dyn_pick<4, 2, 2, 1, 2, 1>(s, sel, L, m, n, col0, ncols, C, ldc, group, Sdev);
-    default: return oz_dyn_pick<2, 4, 2, 1, 4, 1>(s, sel, L, m, n, col0, ncols, C, ldc, group, Sdev);
+    case 1: return oz_dyn_pick<2, 2, 2, 1, 2, 2>(s, sel, L, m, n, col0, ncols, C, ldc, Ci, ldci, group, Sdev);
+    case 2: return oz_dyn_pick<2, 4, 2, 1, 2, 1>(s, sel, L, m, n, col0, ncols, C, ldc, Ci, ldci, group, Sdev);
+    case 3: return oz_dyn_pick<4, 2, 2, 1, 2, 1>(s, sel, L, m, n, col0, ncols, C, ldc, Ci, ldci, group, Sdev);
+    default: return oz_dyn_pick<2, 4, 2, 1, 4, 1>(s, sel, L, m, n, col0, ncols, C, ldc, Ci, ldci, group, Sdev);
   }
 }
 

@@ -213,6 +213,39 @@ def test_row_lookahead_gives_the_same_factors(L, precision, N, nb):
     assert np.array_equal(out[0][2], out[1][2])
 
 
+@pytest.mark.parametrize("N,nb,rows", [(4096, 512, 1024), (3000, 512, 1024), (2048, 128, 0)])
+def test_first_update_reading_the_input_gives_the_same_factors(L, N, nb, rows):
+    """The int8-sliced variant's first trailing update reads a column-major
+    FP64 input where it lies (its leading dimension padded or not) instead of
+    from a copy in the handle; a row-major input is copied first. Both give
+    the same factors and slice counts bit for bit, with panel pairs, row
+    lookahead, and a ragged N whose edge tiles are partly outside the matrix;
+    the input is left unchanged."""
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((N, N)) + N * np.eye(N)   # nonsymmetric, no pivoting needed
+    from mpdok_amd.mpdok_ops import _LUHandle
+    inputs = {"F": dev(A), "C": dev(A, order="C")}
+    # column-major view with leading dimension N + 37
+    inputs["F, ld = N + 37"] = torch.empty((N, N + 37), dtype=torch.float64,
+                                           device="cuda").t()[:N]
+    inputs["F, ld = N + 37"].copy_(torch.as_tensor(A))
+    out = {}
+    for name, Ad in inputs.items():
+        before = Ad.clone()
+        h = _LUHandle(L, N, "ozaki", "none", nb, 5, True)
+        h.set_update_panels(2)
+        h.set_row_lookahead(rows)
+        assert h.factor(Ad) == 0
+        LU, _ = _factors(L, h, N)
+        out[name] = (LU.copy(), h.slice_counts())
+        h.close()
+        assert torch.equal(Ad, before), name
+    assert len(out["F"][1]) > 1
+    for name in ("C", "F, ld = N + 37"):
+        assert np.array_equal(out["F"][0], out[name][0]), name
+        assert out["F"][1] == out[name][1], name
+
+
 @pytest.mark.parametrize("rows", [0, 1024])
 def test_wide_block_solve_matches_own_factors(L, rows):
     """Panels of 512 columns in pairs, N a multiple of 1024: the 512-wide U
