@@ -237,6 +237,58 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
   }
 }
 
+// The same for Kp <= OZ_SC_KMAX, one wavefront per column, the column read
+// once: coalesced nontemporal loads (lane-strided) into LDS while the
+// maximum is formed, then each lane takes 16 consecutive k from LDS (rows
+// padded by 16 bytes every 16 values, so the 16-byte reads of a wavefront
+// spread over the banks). Same digits and exponents as slice_cols_kernel.
+constexpr int OZ_SC_KMAX = 2048;
+template <int S>
+__global__ __launch_bounds__(256) void slice_cols_lds_kernel(const double* __restrict__ B,
+                                                             int64_t ldb, int n, int K, int np,
+                                                             int Kp, signed char* __restrict__ Bs,
+                                                             int* __restrict__ eb) {
+  constexpr int LD = OZ_SC_KMAX + OZ_SC_KMAX / 16 * 2;  // padded doubles per column
+  __shared__ __attribute__((aligned(16))) double colbuf[4][LD];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  if (j >= np) return;  // wavefront-uniform; no workgroup barrier below
+  const double* col = B + (int64_t)j * ldb;
+  double* cb = colbuf[w];
+  double mx = 0.0;
+  for (int k = lane; k < Kp; k += 64) {
+    const double v = (j < n && k < K) ? __builtin_nontemporal_load(col + k) : 0.0;
+    cb[k + (k >> 4) * 2] = v;
+    mx = fmax(mx, fabs(v));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  const int e = oz_exponent(mx);
+  if (lane == 0) eb[j] = e;
+  // this wavefront's LDS stores before its reads
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const size_t slab = (size_t)np * Kp;
+  for (int k0 = lane * 16; k0 < Kp; k0 += 64 * 16) {
+    const double* src = cb + k0 + (k0 >> 4) * 2;
+    signed char dig[S][16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      signed char d[S];
+      make_digits<S>(src[c], e, d);
+#pragma unroll
+      for (int q = 0; q < S; ++q) dig[q][c] = d[q];
+    }
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      v4i wv;
+      __builtin_memcpy(&wv, dig[q], 16);
+      *reinterpret_cast<v4i*>(Bs + q * slab + oz_rec(np, j, k0)) = wv;
+    }
+  }
+}
+
 // C (m x n, column-major, ldc) -= sliced product, columns [col0, col0 + ncols).
 // mp, np: padded row / column counts of the slice arrays; Kp a multiple of 128.
 //
@@ -1055,7 +1107,12 @@ static int ozaki_slice_s(hipStream_t s, int m, int n, int K, const double* A, in
       slice_rows_tile_kernel<S><<<g, 256, 0, s>>>(A, lda, m, K, L.mp, L.Kp, L.pm, L.As, L.ea);
     }
   }
-  if (parts & 2) slice_cols_kernel<S><<<L.np / 4, 256, 0, s>>>(B, ldb, n, K, L.np, L.Kp, L.Bs, L.eb);
+  if (parts & 2) {
+    if (L.Kp <= OZ_SC_KMAX)
+      slice_cols_lds_kernel<S><<<L.np / 4, 256, 0, s>>>(B, ldb, n, K, L.np, L.Kp, L.Bs, L.eb);
+    else
+      slice_cols_kernel<S><<<L.np / 4, 256, 0, s>>>(B, ldb, n, K, L.np, L.Kp, L.Bs, L.eb);
+  }
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
