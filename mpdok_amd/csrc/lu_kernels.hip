@@ -630,6 +630,60 @@ __device__ __forceinline__ float lane_bcast(float x, int src) {
 //     matrix cores (16 x 16 output tiles, triangular operands masked on
 //     their way out of LDS), staged in registers and written back in place.
 // The rounding differs from the unblocked kernel's (other summation orders).
+// Sub-panel P of the blocked diagonal kernel after its elimination: U12 of
+// rows c0 .. c0 + 15 over columns c0 + 16 .. 63 (one column per thread,
+// substituted in registers, k ascending per row), then the rank-16 update of
+// rows and columns c0 + 16 .. 63 (thread (tr, tc) takes rows
+// c0 + 16 + tr + 16 a and columns c0 + 16 + tc + 16 b, a, b < NB, so each
+// operand read from LDS serves NB products; per element the even and odd k
+// run in two chains, added at the end).
+template <int P, typename T>
+__device__ __forceinline__ void diag_blk_trailing(T (*s)[65], int t) {
+  constexpr int c0 = 16 * P, ncol = 48 - c0, NB = ncol / 16;
+  if (t < ncol) {
+    const int c = c0 + 16 + t;
+    T u[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) u[i] = s[c0 + i][c];
+#pragma unroll
+    for (int i = 1; i < 16; ++i) {
+      T acc = u[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k) acc = __builtin_fma(-s[c0 + i][c0 + k], u[k], acc);
+      u[i] = acc;
+    }
+#pragma unroll
+    for (int i = 1; i < 16; ++i) s[c0 + i][c] = u[i];
+  }
+  __syncthreads();
+  {
+    const int tr = t & 15, tc = t >> 4;
+    T lr[NB][16], uc[NB][16];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int k = 0; k < 16; ++k) lr[a][k] = s[c0 + 16 + tr + 16 * a][c0 + k];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int k = 0; k < 16; ++k) uc[b][k] = s[c0 + k][c0 + 16 + tc + 16 * b];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int r = c0 + 16 + tr + 16 * a, c = c0 + 16 + tc + 16 * b;
+        T a0 = s[r][c], a1 = T(0);
+#pragma unroll
+        for (int k = 0; k < 16; k += 2) {
+          a0 = __builtin_fma(-lr[a][k], uc[b][k], a0);
+          a1 = __builtin_fma(-lr[a][k + 1], uc[b][k + 1], a1);
+        }
+        s[r][c] = a0 + a1;
+      }
+  }
+  __syncthreads();
+}
+
 template <typename T>
 __device__ __forceinline__ void diag_factor_blk_body(T* __restrict__ A, int64_t lda, int nb,
                                                      T* __restrict__ Linv, T* __restrict__ Uinv,
@@ -685,29 +739,12 @@ __device__ __forceinline__ void diag_factor_blk_body(T* __restrict__ A, int64_t 
       }
       __syncthreads();
       if (p == 3) break;
-      // U12 of the sub-panel: rows c0 .. c0 + 15, columns c0 + 16 .. 63
-      const int ncol = 48 - c0;
-      if (t < ncol) {
-        const int c = c0 + 16 + t;
-        for (int i = c0 + 1; i < c0 + 16; ++i) {
-          T acc = s[i][c];
-          for (int k = c0; k < i; ++k) acc = __builtin_fma(-s[i][k], s[k][c], acc);
-          s[i][c] = acc;
-        }
-      }
-      __syncthreads();
-      // rank-16 update of rows and columns c0 + 16 .. 63
-      for (int e = t; e < ncol * ncol; e += 256) {
-        const int r = c0 + 16 + e % ncol, c = c0 + 16 + e / ncol;
-        T a0 = s[r][c], a1 = T(0);
-#pragma unroll
-        for (int k = 0; k < 16; k += 2) {
-          a0 = __builtin_fma(-s[r][c0 + k], s[c0 + k][c], a0);
-          a1 = __builtin_fma(-s[r][c0 + k + 1], s[c0 + k + 1][c], a1);
-        }
-        s[r][c] = a0 + a1;
-      }
-      __syncthreads();
+      if (p == 0)
+        diag_blk_trailing<0, T>(s, t);
+      else if (p == 1)
+        diag_blk_trailing<1, T>(s, t);
+      else
+        diag_blk_trailing<2, T>(s, t);
     }
     for (int e = t; e < 64 * 64; e += 256) {
       const int r = e & 63, c = e >> 6;
