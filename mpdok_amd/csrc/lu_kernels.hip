@@ -752,18 +752,26 @@ __device__ __forceinline__ void diag_factor_blk_body(T* __restrict__ A, int64_t 
     }
   }
   // ---- inverses of the 16 x 16 diagonal blocks: wavefront 0 U, 1 L ----
+  // One lane per column, substitution in registers, right-looking (each
+  // finished x_i is added into the sums of the rows it feeds at once), so
+  // the dependent chain per row is one product; U's diagonal reciprocals
+  // are formed up front.
   if (w < 2) {
     const int b = lane >> 4, c = lane & 15, o = 16 * b;
-    T x[16];
+    T x[16], acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = T(0);
     if (w == 0) {  // column c of inv(U_bb): x_c = 1/u_cc, x_i = -(sum_{k>i} u_ik x_k)/u_ii
+      T d[16];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] = k == c ? T(1) / s[o + k][o + k] : T(0);
+      for (int k = 0; k < 16; ++k) d[k] = T(1) / s[o + k][o + k];
 #pragma unroll
-      for (int i = 14; i >= 0; --i) {
-        T acc = T(0);
+      for (int k = 0; k < 16; ++k) x[k] = k == c ? d[k] : T(0);
 #pragma unroll
-        for (int k = i + 1; k < 16; ++k) acc = __builtin_fma(s[o + i][o + k], x[k], acc);
-        if (i < c) x[i] = -acc / s[o + i][o + i];
+      for (int i = 15; i >= 0; --i) {
+        if (i < c) x[i] = -acc[i] * d[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) acc[k] = __builtin_fma(s[o + k][o + i], x[i], acc[k]);
       }
       wave_sync();
 #pragma unroll
@@ -773,11 +781,10 @@ __device__ __forceinline__ void diag_factor_blk_body(T* __restrict__ A, int64_t 
 #pragma unroll
       for (int k = 0; k < 16; ++k) x[k] = k == c ? T(1) : T(0);
 #pragma unroll
-      for (int i = 1; i < 16; ++i) {
-        T acc = T(0);
+      for (int i = 0; i < 16; ++i) {
+        if (i > c) x[i] = -acc[i];
 #pragma unroll
-        for (int k = 0; k < i; ++k) acc = __builtin_fma(s[o + i][o + k], x[k], acc);
-        if (i > c) x[i] = -acc;
+        for (int k = i + 1; k < 16; ++k) acc[k] = __builtin_fma(s[o + k][o + i], x[i], acc[k]);
       }
       wave_sync();
 #pragma unroll
@@ -789,7 +796,9 @@ __device__ __forceinline__ void diag_factor_blk_body(T* __restrict__ A, int64_t 
   // ---- recursive doubling on the matrix cores: h = 16, 32 ----
   // Work units: (triangle, pair, 16 x 16 output tile), wavefront w takes
   // units w and w + 4. Lane layout of the 16 x 16 x 4 MFMA: A[l % 16][l / 16],
-  // B[l / 16][l % 16], D[Mfma<T>::row(l, r)][l % 16].
+  // B[l / 16][l % 16], D[Mfma<T>::row(l, r)][l % 16]. Every operand of both
+  // units is read from LDS first (masked values replaced by zeros after the
+  // read), then the products run back to back.
   typedef typename Mfma<T>::acc_t acc_t;
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
@@ -801,6 +810,7 @@ __device__ __forceinline__ void diag_factor_blk_body(T* __restrict__ A, int64_t 
       // phase 1: U: X = -(P inv(D))          L: X = -(P + inv_s(D) P)
       // (inv_s: the strictly lower part of a unit-lower inverse)
       acc_t res[2];
+      T av[2][8], bv[2][8];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int u = w + 4 * q;
@@ -808,34 +818,44 @@ __device__ __forceinline__ void diag_factor_blk_body(T* __restrict__ A, int64_t 
         if (u >= units) continue;
         const int tri = u / per, pr = (u / (nt * nt)) % npair, tile = u % (nt * nt);
         const int o = 2 * h * pr, ti = 16 * (tile % nt), tj = 16 * (tile / nt);
-        acc_t acc = acc_t(0);
         if (tri == 1) {  // L: the added term (C, or P)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) acc[r] = s[o + h + ti + Mfma<T>::row(lane, r)][o + tj + li];
+          for (int r = 0; r < 4; ++r)
+            res[q][r] = s[o + h + ti + Mfma<T>::row(lane, r)][o + tj + li];
         }
         const int ia = ti + li, jb = tj + li;  // A row, B column of this lane
-        for (int k0 = 0; k0 < h; k0 += 4) {
-          const int k = k0 + lk;
-          T a, b;
+#pragma unroll
+        for (int ks = 0; ks < h / 4; ++ks) {
+          const int k = 4 * ks + lk;
           if (tri == 0) {
             if (phase == 0) {
-              a = k >= ia ? s[o + ia][o + k] : T(0);
-              b = s[o + k][o + h + jb];
+              const T a = s[o + ia][o + k];
+              av[q][ks] = k >= ia ? a : T(0);
+              bv[q][ks] = s[o + k][o + h + jb];
             } else {
-              a = s[o + ia][o + h + k];
-              b = k <= jb ? s[o + h + k][o + h + jb] : T(0);
+              const T bb = s[o + h + k][o + h + jb];
+              av[q][ks] = s[o + ia][o + h + k];
+              bv[q][ks] = k <= jb ? bb : T(0);
             }
           } else {
             if (phase == 0) {
-              a = s[o + h + ia][o + k];
-              b = k > jb ? s[o + k][o + jb] : T(0);
+              const T bb = s[o + k][o + jb];
+              av[q][ks] = s[o + h + ia][o + k];
+              bv[q][ks] = k > jb ? bb : T(0);
             } else {
-              a = k < ia ? s[o + h + ia][o + h + k] : T(0);
-              b = s[o + h + k][o + jb];
+              const T a = s[o + h + ia][o + h + k];
+              av[q][ks] = k < ia ? a : T(0);
+              bv[q][ks] = s[o + h + k][o + jb];
             }
           }
-          acc = Mfma<T>::mma(a, b, acc);
         }
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (w + 4 * q >= units) continue;
+        acc_t acc = res[q];
+#pragma unroll
+        for (int ks = 0; ks < h / 4; ++ks) acc = Mfma<T>::mma(av[q][ks], bv[q][ks], acc);
         res[q] = phase == 0 ? acc : -acc;
       }
       __syncthreads();
