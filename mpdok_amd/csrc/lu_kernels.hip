@@ -993,21 +993,31 @@ struct GeppWork {
   int* posof;      // [N] position of a row
 };
 
+// Device-coherent loads and stores (relaxed atomics at device scope): they
+// reach the point of coherence directly, so the exchange of the panel
+// kernel needs no cache-wide write-back or invalidation (a release or an
+// acquire at device scope does one of those on every call).
+template <typename V>
+__device__ __forceinline__ V gp_ld(const V* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename V>
+__device__ __forceinline__ void gp_st(V* p, V v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Grid barrier of the panel kernel, entered by thread 0 of each workgroup
-// (the caller synchronises the workgroup around it). bar[0] arrivals,
-// bar[1] generation, bar[2] spin-limit flag.
-__device__ __forceinline__ bool gp_grid_barrier0(unsigned* bar, unsigned nblocks) {
-  const unsigned g = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  // the arrival releases this wavefront's candidate stores and, for the last
-  // arrival, acquires every other workgroup's
-  if (__hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nblocks - 1) {
-    __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
+// after its wavefront's device-coherent stores have completed (the caller
+// waits for them and synchronises the workgroup around the barrier).
+// bar[0] counts the arrivals of the whole launch (zero at its start); the
+// barrier of column c is passed once it reaches target = (c + 1) G.
+// bar[2] is the spin-limit flag.
+__device__ __forceinline__ bool gp_grid_barrier0(unsigned* bar, unsigned target) {
+  if (__hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 < target) {
     // every workgroup of a cooperative launch is resident, so the others
     // arrive; the bound only turns a broken launch into an error
     long spins = 0;
-    while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+    while (gp_ld(bar) < target) {
       __builtin_amdgcn_s_sleep(1);
       if (++spins > (1L << 23)) {
         atomicExch(bar + 2, 1u);
@@ -1015,6 +1025,7 @@ __device__ __forceinline__ bool gp_grid_barrier0(unsigned* bar, unsigned nblocks
       }
     }
   }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   return true;
 }
 
@@ -1078,22 +1089,26 @@ __global__ __launch_bounds__(256) void gepp_panel_kernel(T* __restrict__ A, int6
     }
     __syncthreads();
     // wavefront 0 publishes the candidate, takes the grid barrier and reads
-    // the winner back: every global access of the exchange stays in one
-    // wavefront, so its own release and acquire order them
+    // the winner back: every global access of the exchange is
+    // device-coherent and stays in one wavefront, whose stores complete
+    // before its arrival is counted and whose reads follow the barrier
     const int slot = c & 1;
     if (wv == 0) {
-      ws.crow[((size_t)slot * G + blockIdx.x) * GP_W + lane] = cand[lane];
+      gp_st(ws.crow + ((size_t)slot * G + blockIdx.x) * GP_W + lane, cand[lane]);
       if (lane == 0) {
-        ws.cval[slot * G + blockIdx.x] = s_max;
-        ws.cidx[slot * G + blockIdx.x] = s_best;
-        if (!gp_grid_barrier0(ws.bar, (unsigned)G)) s_ok = 0;
+        gp_st(ws.cval + slot * G + blockIdx.x, s_max);
+        gp_st(ws.cidx + slot * G + blockIdx.x, s_best);
       }
+      // the wavefront's stores complete before its arrival is counted
+      __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      if (lane == 0 && !gp_grid_barrier0(ws.bar, (unsigned)(c + 1) * (unsigned)G)) s_ok = 0;
       if (__shfl(s_ok, 0, 64)) {
         double bv = -1.0;
         int bi = 0x7fffffff, bb = 0;
         for (int q = lane; q < G; q += 64) {
-          const double v2 = ws.cval[slot * G + q];
-          const int i2 = ws.cidx[slot * G + q];
+          const double v2 = gp_ld(ws.cval + slot * G + q);
+          const int i2 = gp_ld(ws.cidx + slot * G + q);
           if (v2 > bv || (v2 == bv && i2 < bi)) {
             bv = v2;
             bi = i2;
@@ -1110,7 +1125,7 @@ __global__ __launch_bounds__(256) void gepp_panel_kernel(T* __restrict__ A, int6
             bb = b2;
           }
         }
-        u[lane] = (T)ws.crow[((size_t)slot * G + bb) * GP_W + lane];
+        u[lane] = (T)gp_ld(ws.crow + ((size_t)slot * G + bb) * GP_W + lane);
         if (lane == 0) {
           s_best = bi;
           seq[c] = bi;
