@@ -94,8 +94,11 @@ def test_gemm_bf16_rounding_matches_emulation(L):
 
 
 @pytest.mark.parametrize("S", [3, 4, 5, 6, 7])
-@pytest.mark.parametrize("m,n,K", [(1, 1, 1), (100, 70, 64), (257, 129, 300), (640, 512, 512)])
+@pytest.mark.parametrize("m,n,K", [(1, 1, 1), (100, 70, 64), (257, 129, 300), (640, 512, 512),
+                                   (64, 130, 2048), (96, 80, 2100)])
 def test_ozaki_gemm_bit_exact_vs_oracle(L, S, m, n, K):
+    """K = 2048 is the longest K whose column slicing stages a column in LDS;
+    K = 2100 takes the slicing kernel that reads the columns directly."""
     rng = np.random.default_rng(S * 1000 + m)
     A = rng.standard_normal((m, K)) * np.exp(rng.uniform(-5, 5, (m, 1)))
     B = rng.standard_normal((K, n)) * np.exp(rng.uniform(-5, 5, (1, n)))
