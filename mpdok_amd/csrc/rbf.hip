@@ -23,45 +23,66 @@ template <> __device__ __forceinline__ float to_out<float>(double v) { return (f
 
 // Output element (i, j), i in [row0, row0+rows), j in [col0, col0+cols), stored at
 // out[(i-row0)*ld + (j-col0)] (ROW_MAJOR) or out[(i-row0) + (j-col0)*ld] (COL_MAJOR).
-// The fast thread index follows the contiguous output dimension; the other
-// point is uniform across a wavefront and comes in through scalar loads.
-template <typename TO, int ORDER>
+// Block: 64 fast x 64 slow indices; the fast index follows the contiguous
+// output dimension, and each of the 4 wavefronts takes 16 slow indices. The
+// block's 64 slow-index points (coordinates and squared norms) are staged in
+// LDS once and read by broadcast, so the loop over a wavefront's slow indices
+// issues no memory reads. DT: the dimension when fixed at compile time, 0: D
+// at run time. Stores are nontemporal: the output is written once and far
+// larger than the caches.
+template <typename TO, int ORDER, int DT>
 __global__ __launch_bounds__(256) void rbf_tile_kernel(const double* __restrict__ coords, int D,
                                                        int kind, double p, double reg, int reg_mode,
                                                        int row0, int rows, int col0, int cols,
                                                        TO* __restrict__ out, int64_t ld) {
-  // fast index f (64 per block), slow index s (64 per block: 4 waves x 16)
-  const int f = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int sbase = blockIdx.y * 64 + (threadIdx.x >> 6) * 16;
+  constexpr int NC = DT ? DT : MAXD;  // coordinates kept per point
+  __shared__ double sp[64][NC + 1];   // coordinates, then the squared norm
+  const int t = threadIdx.x;
   const int nf = ORDER == ROW_MAJOR ? cols : rows;
   const int ns = ORDER == ROW_MAJOR ? rows : cols;
-  if (f >= nf) return;
   const int goff_f = ORDER == ROW_MAJOR ? col0 : row0;
   const int goff_s = ORDER == ROW_MAJOR ? row0 : col0;
+  const int n = DT ? DT : D;
+  const int s0 = blockIdx.y * 64;
+  if (t < 64) {
+    const bool in = s0 + t < ns;
+    double c[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d)
+      c[d] = in && d < n ? coords[(int64_t)(goff_s + s0 + t) * D + d] : 0.0;
+#pragma unroll
+    for (int d = 0; d < NC; ++d) sp[t][d] = c[d];
+    sp[t][NC] = sqnorm<DT>(c, D);
+  }
+  __syncthreads();
+  const int f = blockIdx.x * 64 + (t & 63);
+  if (f >= nf) return;
   double cf[MAXD];
 #pragma unroll
-  for (int d = 0; d < MAXD; ++d) cf[d] = d < D ? coords[(int64_t)(goff_f + f) * D + d] : 0.0;
-  const double sqf = sqnorm(cf, D);
-  for (int k = 0; k < 16; ++k) {
-    const int s = sbase + k;
-    if (s >= ns) break;
+  for (int d = 0; d < MAXD; ++d) cf[d] = d < n ? coords[(int64_t)(goff_f + f) * D + d] : 0.0;
+  const double sqf = sqnorm<DT>(cf, D);
+  const int sw = (t >> 6) * 16;  // this wavefront's first slow index in the block
+  const int kn = ns - s0 - sw < 16 ? ns - s0 - sw : 16;
+  TO* o = out + (int64_t)(s0 + sw) * ld + f;
+  for (int k = 0; k < kn; ++k) {
+    const int sl = sw + k;
     double cs[MAXD];
 #pragma unroll
-    for (int d = 0; d < MAXD; ++d) cs[d] = d < D ? coords[(int64_t)(goff_s + s) * D + d] : 0.0;
-    const double sqs = sqnorm(cs, D);
+    for (int d = 0; d < MAXD; ++d) cs[d] = d < NC && d < n ? sp[sl][d] : 0.0;
+    const double sqs = sp[sl][NC];
     // The reference forms D2[i, j] with i the row point: keep that operand order.
-    double d2 = ORDER == ROW_MAJOR ? gram_d2(cs, sqs, cf, sqf, D) : gram_d2(cf, sqf, cs, sqs, D);
-    double v = kernel_value(kind, p, d2);
-    const bool diag = (goff_f + f) == (goff_s + s);
-    TO o;
+    const double d2 = ORDER == ROW_MAJOR ? gram_d2<DT>(cs, sqs, cf, sqf, D)
+                                         : gram_d2<DT>(cf, sqf, cs, sqs, D);
+    const double v = kernel_value(kind, p, d2);
+    const bool diag = goff_f + f == goff_s + s0 + sl;
+    TO e;
     if (diag && reg_mode == 0) {
-      o = to_out<TO>(v + reg);
+      e = to_out<TO>(v + reg);
     } else {
-      o = to_out<TO>(v);
-      if (diag) o = o + (TO)reg;
+      e = to_out<TO>(v);
+      if (diag) e = e + (TO)reg;
     }
-    const int64_t idx = ORDER == ROW_MAJOR ? (int64_t)s * ld + f : (int64_t)s * ld + f;
-    out[idx] = o;
+    __builtin_nontemporal_store(e, o + (int64_t)k * ld);
   }
 }
 
@@ -81,17 +102,22 @@ int rbf_assemble(hipStream_t st, const double* coords, int N, int D, int kind, d
   const int ns = order == ROW_MAJOR ? rows : cols;
   dim3 grid((nf + 63) / 64, (ns + 63) / 64);
   MPDOK_REQUIRE(grid.y <= 65535, "rbf_assemble: tile too tall");
-#define LAUNCH(TO, ORD)                                                                     \
-  rbf_tile_kernel<TO, ORD><<<grid, 256, 0, st>>>(coords, D, kind, p, reg, reg_mode, row0, rows, \
-                                                 col0, cols, (TO*)out, ld)
-  if (out_dtype == F64) {
-    if (order == ROW_MAJOR) LAUNCH(double, ROW_MAJOR); else LAUNCH(double, COL_MAJOR);
-  } else if (out_dtype == F32) {
-    if (order == ROW_MAJOR) LAUNCH(float, ROW_MAJOR); else LAUNCH(float, COL_MAJOR);
-  } else {
-    MPDOK_REQUIRE(false, "rbf_assemble: unsupported output dtype %d", out_dtype);
+  MPDOK_REQUIRE(out_dtype == F64 || out_dtype == F32, "rbf_assemble: unsupported output dtype %d",
+                out_dtype);
+#define LAUNCH_T(TO, ORD, DT)                                                           \
+  rbf_tile_kernel<TO, ORD, DT><<<grid, 256, 0, st>>>(coords, D, kind, p, reg, reg_mode, row0, \
+                                                     rows, col0, cols, (TO*)out, ld)
+#define LAUNCH(DT)                                                 \
+  if (out_dtype == F64) {                                          \
+    if (order == ROW_MAJOR) LAUNCH_T(double, ROW_MAJOR, DT);       \
+    else LAUNCH_T(double, COL_MAJOR, DT);                          \
+  } else {                                                         \
+    if (order == ROW_MAJOR) LAUNCH_T(float, ROW_MAJOR, DT);        \
+    else LAUNCH_T(float, COL_MAJOR, DT);                           \
   }
+  MPDOK_RBF_DIM_DISPATCH(D, LAUNCH);
 #undef LAUNCH
+#undef LAUNCH_T
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
@@ -202,30 +228,31 @@ __device__ __forceinline__ double kernel_dparam(int kind, double p, double d2) {
   return a * a * exp(-a) / p;
 }
 
-template <int ELEM, int DERIV, typename TX>
+template <int ELEM, int DERIV, int DT, typename TX>
 __device__ __forceinline__ double rbf_row_sum(const double* __restrict__ ci, double sqi, int i,
                                               const double* __restrict__ C, int NC, int D,
                                               int kind, double p, double reg, int same,
                                               const TX* __restrict__ x, double (*cj)[256],
                                               double* sqj, double* xj, int t, int ph) {
+  const int n = DT ? DT : D;
   double acc = 0.0;
   for (int j0 = 0; j0 < NC; j0 += 256) {
     const int j = j0 + t;
     double c[MAXD];
 #pragma unroll
-    for (int d = 0; d < MAXD; ++d) c[d] = (j < NC && d < D) ? C[(int64_t)j * D + d] : 0.0;
+    for (int d = 0; d < MAXD; ++d) c[d] = (j < NC && d < n) ? C[(int64_t)j * D + d] : 0.0;
 #pragma unroll
     for (int d = 0; d < MAXD; ++d)
-      if (d < D) cj[d][t] = c[d];
-    sqj[t] = sqnorm(c, D);
+      if (d < n) cj[d][t] = c[d];
+    sqj[t] = sqnorm<DT>(c, D);
     xj[t] = j < NC ? (double)x[j] : 0.0;
     __syncthreads();
     const int jn = NC - j0 < 256 ? NC - j0 : 256;
     for (int k = ph; k < jn; k += 8) {
       double cc[MAXD];
 #pragma unroll
-      for (int d = 0; d < MAXD; ++d) cc[d] = d < D ? cj[d][k] : 0.0;
-      const double d2 = gram_d2(ci, sqi, cc, sqj[k], D);
+      for (int d = 0; d < MAXD; ++d) cc[d] = d < n ? cj[d][k] : 0.0;
+      const double d2 = gram_d2<DT>(ci, sqi, cc, sqj[k], D);
       double e;
       if (DERIV) {
         e = kernel_dparam(kind, p, d2);
@@ -247,15 +274,16 @@ __device__ __forceinline__ double rbf_row_sum(const double* __restrict__ ci, dou
   return acc;
 }
 
-// y[i - row0] = sum_j E_ij x_j  (y in TY).
-template <int ELEM, typename TX, typename TY>
+// y[i - row0] = sum_j E_ij x_j  (y in TY). DT: the dimension when fixed at
+// compile time, 0: D at run time.
+template <int ELEM, int DT, typename TX, typename TY>
 __global__ __launch_bounds__(256) void rbf_matvec_kernel(const double* __restrict__ R,
                                                          const double* __restrict__ C, int NC,
                                                          int D, int kind, double p, double reg,
                                                          int same, int row0, int rows,
                                                          const TX* __restrict__ x,
                                                          TY* __restrict__ y) {
-  __shared__ double cj[MAXD][256];
+  __shared__ double cj[DT ? DT : MAXD][256];
   __shared__ double sqj[256];
   __shared__ double xj[256];
   __shared__ double part[8][33];
@@ -263,11 +291,12 @@ __global__ __launch_bounds__(256) void rbf_matvec_kernel(const double* __restric
   const int il = blockIdx.x * 32 + r;
   const int i = row0 + il;
   const bool live = il < rows;
+  const int n = DT ? DT : D;
   double ci[MAXD];
 #pragma unroll
-  for (int d = 0; d < MAXD; ++d) ci[d] = (live && d < D) ? R[(int64_t)i * D + d] : 0.0;
-  const double acc = rbf_row_sum<ELEM, 0, TX>(ci, sqnorm(ci, D), i, C, NC, D, kind, p, reg, same,
-                                             x, cj, sqj, xj, t, ph);
+  for (int d = 0; d < MAXD; ++d) ci[d] = (live && d < n) ? R[(int64_t)i * D + d] : 0.0;
+  const double acc = rbf_row_sum<ELEM, 0, DT, TX>(ci, sqnorm<DT>(ci, D), i, C, NC, D, kind, p, reg,
+                                                 same, x, cj, sqj, xj, t, ph);
   part[ph][r] = acc;
   __syncthreads();
   if (ph == 0 && live) {
@@ -280,24 +309,26 @@ __global__ __launch_bounds__(256) void rbf_matvec_kernel(const double* __restric
 // Bilinear form u^T (dK/dp) w over rows [0, NR) of R and all points of C:
 // per-block partial sums into part_out[blockIdx.x], reduced in a fixed order
 // by bilinear_reduce_kernel.
+template <int DT>
 __global__ __launch_bounds__(256) void rbf_dbilinear_kernel(const double* __restrict__ R, int NR,
                                                             const double* __restrict__ C, int NC,
                                                             int D, int kind, double p,
                                                             const double* __restrict__ u,
                                                             const double* __restrict__ w,
                                                             double* __restrict__ part_out) {
-  __shared__ double cj[MAXD][256];
+  __shared__ double cj[DT ? DT : MAXD][256];
   __shared__ double sqj[256];
   __shared__ double xj[256];
   __shared__ double part[8][33];
   const int t = threadIdx.x, r = t & 31, ph = t >> 5;
   const int i = blockIdx.x * 32 + r;
   const bool live = i < NR;
+  const int n = DT ? DT : D;
   double ci[MAXD];
 #pragma unroll
-  for (int d = 0; d < MAXD; ++d) ci[d] = (live && d < D) ? R[(int64_t)i * D + d] : 0.0;
-  const double acc = rbf_row_sum<F64, 1, double>(ci, sqnorm(ci, D), i, C, NC, D, kind, p, 0.0, 0,
-                                                 w, cj, sqj, xj, t, ph);
+  for (int d = 0; d < MAXD; ++d) ci[d] = (live && d < n) ? R[(int64_t)i * D + d] : 0.0;
+  const double acc = rbf_row_sum<F64, 1, DT, double>(ci, sqnorm<DT>(ci, D), i, C, NC, D, kind, p,
+                                                     0.0, 0, w, cj, sqj, xj, t, ph);
   part[ph][r] = acc;
   __syncthreads();
   if (ph == 0) {
@@ -344,15 +375,20 @@ int rbf_cross_matvec(hipStream_t st, const double* R, int NR, const double* C, i
   MPDOK_REQUIRE(xy_dtype == F64 || xy_dtype == F32, "rbf_matvec: vector dtype %d", xy_dtype);
   if (rows == 0) return OK;
   const int grid = (rows + 31) / 32;
-#define LAUNCH(E, T)                                                                          \
-  rbf_matvec_kernel<E, T, T><<<grid, 256, 0, st>>>(R, C, NC, D, kind, p, reg, same, row0, rows, \
-                                                   (const T*)x, (T*)y)
-  if (elem_dtype == F64) {
-    if (xy_dtype == F64) LAUNCH(F64, double); else LAUNCH(F64, float);
-  } else {
-    if (xy_dtype == F64) LAUNCH(F32, double); else LAUNCH(F32, float);
+#define LAUNCH_T(E, DT, T)                                                                \
+  rbf_matvec_kernel<E, DT, T, T><<<grid, 256, 0, st>>>(R, C, NC, D, kind, p, reg, same, row0, \
+                                                       rows, (const T*)x, (T*)y)
+#define LAUNCH(DT)                                                       \
+  if (elem_dtype == F64) {                                               \
+    if (xy_dtype == F64) LAUNCH_T(F64, DT, double);                      \
+    else LAUNCH_T(F64, DT, float);                                       \
+  } else {                                                               \
+    if (xy_dtype == F64) LAUNCH_T(F32, DT, double);                      \
+    else LAUNCH_T(F32, DT, float);                                       \
   }
+  MPDOK_RBF_DIM_DISPATCH(D, LAUNCH);
 #undef LAUNCH
+#undef LAUNCH_T
   MPDOK_HIP(hipGetLastError());
   return OK;
 }
@@ -374,7 +410,9 @@ int rbf_dbilinear(hipStream_t st, const double* R, int NR, const double* C, int 
   MPDOK_REQUIRE(work_bytes >= rbf_dbilinear_workspace_bytes(NR), "rbf_dbilinear: workspace");
   const int nb = (NR + 31) / 32;
   double* part = static_cast<double*>(work);
-  rbf_dbilinear_kernel<<<nb, 256, 0, st>>>(R, NR, C, NC, D, kind, p, u, w, part);
+#define LAUNCH(DT) rbf_dbilinear_kernel<DT><<<nb, 256, 0, st>>>(R, NR, C, NC, D, kind, p, u, w, part)
+  MPDOK_RBF_DIM_DISPATCH(D, LAUNCH);
+#undef LAUNCH
   bilinear_reduce_kernel<<<1, 256, 0, st>>>(part, nb, out);
   MPDOK_HIP(hipGetLastError());
   return OK;
