@@ -86,6 +86,82 @@ __global__ __launch_bounds__(256) void rbf_tile_kernel(const double* __restrict_
   }
 }
 
+// A principal block (row0 == col0, rows == cols) is symmetric bit for bit:
+// d2 = (sq_i + sq_j) - 2 (c_i . c_j) makes the same roundings with i and j
+// exchanged (sums and products commute; a fused multiply-add rounds once,
+// after its exact product). Each workgroup takes one 64 x 64 tile of the
+// upper block triangle (fast-index tile bf <= slow-index tile bs), evaluates
+// it once and writes it twice: along the fast index straight from registers
+// and, off the diagonal, its mirror through a per-wavefront LDS transpose, so
+// both writes are coalesced. Half the kernel evaluations of rbf_tile_kernel,
+// the same values.
+template <typename TO, int DT>
+__global__ __launch_bounds__(256) void rbf_sym_kernel(const double* __restrict__ coords, int D,
+                                                      int kind, double p, double reg, int reg_mode,
+                                                      int n0, int n, TO* __restrict__ out,
+                                                      int64_t ld) {
+  constexpr int NC = DT ? DT : MAXD;
+  __shared__ double sp[64][NC + 1];  // slow-index points: coordinates, squared norm
+  __shared__ TO tv[4][16][65];       // per wavefront: its 16 slow x 64 fast values
+  // tile pair from the linear index L = bs (bs + 1) / 2 + bf, 0 <= bf <= bs
+  const int64_t L = blockIdx.x;
+  int64_t bs = (int64_t)((sqrt(8.0 * (double)L + 1.0) - 1.0) * 0.5);
+  while ((bs + 1) * (bs + 2) / 2 <= L) ++bs;
+  while (bs * (bs + 1) / 2 > L) --bs;
+  const int fb = (int)(L - bs * (bs + 1) / 2) * 64, s0 = (int)bs * 64;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int nd = DT ? DT : D;
+  if (t < 64) {
+    const bool in = s0 + t < n;
+    double c[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d)
+      c[d] = in && d < nd ? coords[(int64_t)(n0 + s0 + t) * D + d] : 0.0;
+#pragma unroll
+    for (int d = 0; d < NC; ++d) sp[t][d] = c[d];
+    sp[t][NC] = sqnorm<DT>(c, D);
+  }
+  __syncthreads();
+  const int f = fb + lane;
+  const int sw = w * 16;
+  const int kn = n - s0 - sw < 16 ? n - s0 - sw : 16;
+  if (f < n) {
+    double cf[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) cf[d] = d < nd ? coords[(int64_t)(n0 + f) * D + d] : 0.0;
+    const double sqf = sqnorm<DT>(cf, D);
+    TO* o = out + (int64_t)(s0 + sw) * ld + f;
+    for (int k = 0; k < kn; ++k) {
+      const int sl = sw + k;
+      double cs[MAXD];
+#pragma unroll
+      for (int d = 0; d < MAXD; ++d) cs[d] = d < NC && d < nd ? sp[sl][d] : 0.0;
+      const double v = kernel_value(kind, p, gram_d2<DT>(cf, sqf, cs, sp[sl][NC], D));
+      const bool diag = f == s0 + sl;
+      TO e;
+      if (diag && reg_mode == 0) {
+        e = to_out<TO>(v + reg);
+      } else {
+        e = to_out<TO>(v);
+        if (diag) e = e + (TO)reg;
+      }
+      __builtin_nontemporal_store(e, o + (int64_t)k * ld);
+      tv[w][k][lane] = e;
+    }
+  }
+  if (fb == s0) return;  // a diagonal tile is its own mirror
+  // The mirror: fast index s, slow index f. Each store covers 4 slow x 16
+  // fast indices; the values come from lanes of this wavefront only (LDS
+  // accesses of one wavefront complete in order).
+  const int kk = lane & 15, q = lane >> 4;
+  if (kk >= kn) return;
+  TO* m = out + (int64_t)fb * ld + s0 + sw + kk;
+  for (int i = 0; i < 16; ++i) {
+    const int fl = 4 * i + q;
+    if (fb + fl < n) __builtin_nontemporal_store(tv[w][kk][fl], m + (int64_t)fl * ld);
+  }
+}
+
 int rbf_assemble(hipStream_t st, const double* coords, int N, int D, int kind, double p,
                  double reg, int reg_mode, int row0, int rows, int col0, int cols, void* out,
                  int out_dtype, int64_t ld, int order) {
@@ -104,6 +180,23 @@ int rbf_assemble(hipStream_t st, const double* coords, int N, int D, int kind, d
   MPDOK_REQUIRE(grid.y <= 65535, "rbf_assemble: tile too tall");
   MPDOK_REQUIRE(out_dtype == F64 || out_dtype == F32, "rbf_assemble: unsupported output dtype %d",
                 out_dtype);
+  if (row0 == col0 && rows == cols) {
+    const int64_t T = (rows + 63) / 64, pairs = T * (T + 1) / 2;
+    MPDOK_REQUIRE(pairs <= 0x7fffffff, "rbf_assemble: tile too large");
+#define LAUNCH_S(DT)                                                                          \
+  if (out_dtype == F64)                                                                        \
+    rbf_sym_kernel<double, DT><<<(unsigned)pairs, 256, 0, st>>>(coords, D, kind, p, reg,       \
+                                                                reg_mode, row0, rows,          \
+                                                                (double*)out, ld);             \
+  else                                                                                         \
+    rbf_sym_kernel<float, DT><<<(unsigned)pairs, 256, 0, st>>>(coords, D, kind, p, reg,        \
+                                                               reg_mode, row0, rows, (float*)out, \
+                                                               ld)
+    MPDOK_RBF_DIM_DISPATCH(D, LAUNCH_S);
+#undef LAUNCH_S
+    MPDOK_HIP(hipGetLastError());
+    return OK;
+  }
 #define LAUNCH_T(TO, ORD, DT)                                                           \
   rbf_tile_kernel<TO, ORD, DT><<<grid, 256, 0, st>>>(coords, D, kind, p, reg, reg_mode, row0, \
                                                      rows, col0, cols, (TO*)out, ld)

@@ -127,6 +127,33 @@ def test_rbf_assembly_gaussian_fp64_and_fp32(ops, D, order):
     assert np.all(np.abs(A32.astype(np.float64) - ref.astype(np.float32)) <= ulp)
 
 
+@pytest.mark.parametrize("D", [1, 2, 3, 5])
+@pytest.mark.parametrize("order", ["C", "F"])
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_rbf_symmetric_block_equals_general_tiles(ops, D, order, dt):
+    """A principal block (row0 == col0, rows == cols) is evaluated one tile
+    pair at a time and written twice; any other tile is evaluated element by
+    element. Both give the same values bit for bit, with a partial last tile,
+    with an offset principal block and for both nugget modes."""
+    N = 300
+    X = np.random.default_rng(10 + D).uniform(0, 1, (N, D))
+    Xd = dev(X)
+
+    def asm(kind, p, rm, rows, cols, row0=0, col0=0):
+        out = dev(np.zeros((rows, cols)), dt, order)
+        return ops.rbf_assemble(Xd, kind, p, 1e-3, out, row0=row0, rows=rows, col0=col0,
+                                cols=cols, reg_mode=rm).cpu().numpy()
+
+    for kind, p in ((0, 25.0), (1, 0.1)):
+        for rm in (0, 1):
+            full = asm(kind, p, rm, N, N)
+            assert np.array_equal(full, full.T)
+            assert np.array_equal(full[:, :N - 1], asm(kind, p, rm, N, N - 1))
+            assert np.array_equal(full[:, N - 1:], asm(kind, p, rm, N, 1, col0=N - 1))
+            assert np.array_equal(full[37:237, 37:237], asm(kind, p, rm, 200, 200, 37, 37))
+            assert np.array_equal(full[37:237, 40:240], asm(kind, p, rm, 200, 200, 37, 40))
+
+
 def test_rbf_tile_fp32_nugget_after_cast_and_offsets(ops):
     N = 500
     X = np.random.default_rng(3).uniform(0, 1, (N, 3))
