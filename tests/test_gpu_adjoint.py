@@ -51,18 +51,25 @@ def test_cross_matvec_and_dbilinear_match_closed_form():
     assert b == pytest.approx(ref, rel=1e-11, abs=1e-12 * np.abs(u) @ (_d2(R, C) * E) @ np.abs(x))
 
 
-@pytest.mark.parametrize("eps,log_sigma", [(6.0, -6.0), (4.0, -9.0)])
-def test_adjoint_gradient_matches_dense_fp64(eps, log_sigma):
+# (eps, log_sigma, train points, validation points, relative tolerance of the
+# gradients). log sigma = -12 is the adjoint workload's own noise level (a
+# nugget of e^-12 = 6.1e-6): the system's condition number grows about e^3
+# over log sigma = -9, and both the GPU solve and the LAPACK reference lose
+# digits with it.
+@pytest.mark.parametrize("eps,log_sigma,n,nv,rtol", [(6.0, -6.0, 1024, 256, 1e-7),
+                                                     (4.0, -9.0, 1024, 256, 1e-7),
+                                                     (6.0, -12.0, 2048, 512, 1e-5)])
+def test_adjoint_gradient_matches_dense_fp64(eps, log_sigma, n, nv, rtol):
     from mpdok_amd.adjoint import rbf_validation_loss
-    X, y, Xv, yv = _problem(1024, 256, 7)
+    X, y, Xv, yv = _problem(n, nv, 7)
     e = torch.tensor(eps, dtype=torch.float64, requires_grad=True)
     s = torch.tensor(log_sigma, dtype=torch.float64, requires_grad=True)
     L = rbf_validation_loss(e, s, X, y, Xv, yv, tol=1e-13)
     L.backward()
     L_ref, de_ref, ds_ref = dense_loss_and_grad(eps, log_sigma, X, y, Xv, yv)
-    assert float(L) == pytest.approx(L_ref, rel=1e-9)
-    assert float(e.grad) == pytest.approx(de_ref, rel=1e-7)
-    assert float(s.grad) == pytest.approx(ds_ref, rel=1e-7)
+    assert float(L.detach()) == pytest.approx(L_ref, rel=rtol * 1e-2)
+    assert float(e.grad) == pytest.approx(de_ref, rel=rtol)
+    assert float(s.grad) == pytest.approx(ds_ref, rel=rtol)
 
 
 def test_adjoint_gradient_matches_central_differences():

@@ -173,8 +173,14 @@ def test_rbf_assembly_matern32(ops):
     ref = orb.build_kernel(X, "matern32", 0.1, 1e-4)
     out = dev(np.zeros((N, N)), torch.float64, "F")
     A = ops.rbf_assemble(dev(X), 1, 0.1, 1e-4, out).cpu().numpy()
-    # sqrt near d2 = 0 amplifies the Gram-identity rounding: absolute tolerance
-    assert np.allclose(A, ref, rtol=1e-13, atol=1e-7)
+    # Each side's d2 carries up to ~4u (sq_i + sq_j) <= 8 D u of Gram-identity
+    # rounding. With a = sqrt(3 d2) / l, the kernel (1 + a) e^-a moves by
+    # |dk/dd2| = 1.5 e^-a / l^2 <= 1.5 / l^2 per unit of d2 (finite at
+    # d2 = 0, where sqrt alone would not be), so the values differ by at most
+    # 1.5 / l^2 * 2 * 8 D u in absolute terms, plus a few ulps relative.
+    ell, u = 0.1, 2.0 ** -53
+    atol = 1.5 / ell ** 2 * 2 * 8 * 3 * u
+    assert np.allclose(A, ref, rtol=1e-14, atol=atol)
     assert np.array_equal(A, A.T)
 
 
