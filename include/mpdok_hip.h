@@ -167,6 +167,16 @@ int mpdok_rbf_mean_d2(hipStream_t s, const double* coords, int n, int D, double*
 int mpdok_rbf_matvec(hipStream_t s, const double* coords, int N, int D, int kind, double p,
                      double reg, int elem_dtype, const void* x, void* y, int xy_dtype, int row0,
                      int rows);
+/* The same product over all N rows (row0 = 0, rows = N), each kernel value
+ * evaluated once and used for both E_ij and E_ji (the matrix is symmetric bit
+ * for bit): half the evaluations of mpdok_rbf_matvec, sums in another fixed
+ * order. work: device memory of mpdok_rbf_symv_workspace_bytes(N) bytes
+ * (N^2 / 32 rounded up to 256-point tiles), contents not kept between calls.
+ * Serves the same callers (mpdok_ooc.py:205-227 and the FP32 product). */
+size_t mpdok_rbf_symv_workspace_bytes(int N);
+int mpdok_rbf_symv(hipStream_t s, const double* coords, int N, int D, int kind, double p,
+                   double reg, int elem_dtype, const void* x, void* y, int xy_dtype, void* work,
+                   size_t work_bytes);
 /* Cross-kernel product y = K(R, C) x (FP64, no nugget), R: NR points, C: NC
  * points, both (n, D) row-major: the interpolant at query points
  * (rbf_demo.ipynb:527, enkf_solver.py:162-200) and K_v alpha / K_v^T g of the

@@ -59,6 +59,8 @@ SIGNATURES = {
     "mpdok_rbf_mean_d2": (I, [P, P, I, I, P]),
     "mpdok_rbf_diag": (I, [P, P, I, I, I, D, D, I, I, P, I]),
     "mpdok_rbf_matvec": (I, [P, P, I, I, I, D, D, I, P, P, I, I, I]),
+    "mpdok_rbf_symv_workspace_bytes": (SZ, [I]),
+    "mpdok_rbf_symv": (I, [P, P, I, I, I, D, D, I, P, P, I, P, SZ]),
     "mpdok_rbf_cross_matvec": (I, [P, P, I, P, I, I, I, D, P, P]),
     "mpdok_rbf_dbilinear_workspace_bytes": (SZ, [I]),
     "mpdok_rbf_dbilinear": (I, [P, P, I, P, I, I, I, D, P, P, P, P, SZ]),

@@ -448,6 +448,108 @@ __global__ __launch_bounds__(256) void bilinear_reduce_kernel(const double* __re
   if (threadIdx.x == 0) out[0] = s[0];
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-free y = E x over a whole point set (all rows, the same points on
+// both sides). E is symmetric bit for bit (see rbf_sym_kernel), so each
+// workgroup evaluates one pair of 256-point tiles I <= J once and uses each
+// element twice: in the row sums of I (E_IJ x_J) and in the column sums of J
+// (E_IJ^T x_I = E_JI x_I). The 256 partial sums of each go to P[I T + J] and
+// P[J T + I] (T tiles); rbf_symv_reduce_kernel then adds P[I T + 0 .. T) in
+// that order, so results do not depend on timing. Half the kernel evaluations
+// of rbf_matvec_kernel; the sums are ordered differently.
+// Thread t holds row point I*256 + t. Off the diagonal, lane l of a wavefront
+// takes column (l + s) mod 256 at step s: every lane sees every column, the
+// lanes of a step see different columns (conflict-free LDS reads), and the
+// column sums are kept per wavefront in LDS (no two lanes of a step touch the
+// same one; wavefronts have their own), added in a fixed order at the end.
+// ---------------------------------------------------------------------------
+template <int ELEM>
+__device__ __forceinline__ double elem_value(int kind, double p, double reg, double d2, bool diag) {
+  const double v = kernel_value(kind, p, d2);
+  if (ELEM == F64) return diag ? v + reg : v;
+  float f = (float)v;
+  if (diag) f = f + (float)reg;
+  return (double)f;
+}
+
+template <int ELEM, int DT, typename TX>
+__global__ __launch_bounds__(256) void rbf_symv_kernel(const double* __restrict__ C, int N, int D,
+                                                       int kind, double p, double reg,
+                                                       const TX* __restrict__ x,
+                                                       double* __restrict__ P, int T) {
+  constexpr int NC = DT ? DT : MAXD;
+  __shared__ double cj[NC][256];
+  __shared__ double sqj[256];
+  __shared__ double xj[256];
+  __shared__ double cs[4][256];
+  // tile pair from the linear index L = J (J + 1) / 2 + I, 0 <= I <= J
+  const int64_t L = blockIdx.x;
+  int64_t J = (int64_t)((sqrt(8.0 * (double)L + 1.0) - 1.0) * 0.5);
+  while ((J + 1) * (J + 2) / 2 <= L) ++J;
+  while (J * (J + 1) / 2 > L) --J;
+  const int I = (int)(L - J * (J + 1) / 2);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int nd = DT ? DT : D;
+  const int i = I * 256 + t, j = (int)J * 256 + t;
+  {
+    double c[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; ++d) c[d] = (j < N && d < nd) ? C[(int64_t)j * D + d] : 0.0;
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+      if (d < nd) cj[d][t] = c[d];
+    sqj[t] = sqnorm<DT>(c, D);
+    xj[t] = j < N ? (double)x[j] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cs[q][t] = 0.0;
+  }
+  double ci[MAXD];
+#pragma unroll
+  for (int d = 0; d < MAXD; ++d) ci[d] = (i < N && d < nd) ? C[(int64_t)i * D + d] : 0.0;
+  const double sqi = sqnorm<DT>(ci, D);
+  const double xi = i < N ? (double)x[i] : 0.0;
+  __syncthreads();
+  // Columns past N carry x = 0 and rows past N x = 0: their (finite) values
+  // add nothing to the sums that are kept.
+  double rs = 0.0;
+  if (I == J) {
+    for (int k = 0; k < 256; ++k) {
+      double cc[MAXD];
+#pragma unroll
+      for (int d = 0; d < MAXD; ++d) cc[d] = d < nd ? cj[d][k] : 0.0;
+      const double e =
+          elem_value<ELEM>(kind, p, reg, gram_d2<DT>(ci, sqi, cc, sqj[k], D), k == t && i < N);
+      rs = __builtin_fma(e, xj[k], rs);
+    }
+  } else {
+    double* csw = cs[w];
+    for (int st = 0; st < 256; ++st) {
+      const int k = (lane + st) & 255;
+      double cc[MAXD];
+#pragma unroll
+      for (int d = 0; d < MAXD; ++d) cc[d] = d < nd ? cj[d][k] : 0.0;
+      const double e = elem_value<ELEM>(kind, p, reg, gram_d2<DT>(ci, sqi, cc, sqj[k], D), false);
+      rs = __builtin_fma(e, xj[k], rs);
+      csw[k] = __builtin_fma(e, xi, csw[k]);
+    }
+  }
+  if (i < N) P[((int64_t)I * T + J) * 256 + t] = rs;
+  if (I == J) return;
+  __syncthreads();
+  if (j < N) P[((int64_t)J * T + I) * 256 + t] = (cs[0][t] + cs[1][t]) + (cs[2][t] + cs[3][t]);
+}
+
+template <typename TY>
+__global__ __launch_bounds__(256) void rbf_symv_reduce_kernel(const double* __restrict__ P, int N,
+                                                              int T, TY* __restrict__ y) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const double* q = P + (int64_t)blockIdx.x * T * 256 + threadIdx.x;
+  double s = 0.0;
+  for (int J = 0; J < T; ++J) s += q[(int64_t)J * 256];
+  y[i] = (TY)s;
+}
+
 static int rbf_check(const double* R, const double* C, int D, int kind, double p) {
   MPDOK_REQUIRE(R && C, "rbf: null coordinates");
   MPDOK_REQUIRE(D >= 1 && D <= MAXD, "rbf: D must be in [1, %d], got %d", MAXD, D);
@@ -492,6 +594,48 @@ int rbf_matvec(hipStream_t st, const double* coords, int N, int D, int kind, dou
                           xy_dtype, row0, rows);
 }
 
+size_t rbf_symv_workspace_bytes(int N) {
+  const size_t T = N > 0 ? ((size_t)N + 255) / 256 : 0;
+  return T * T * 256 * sizeof(double);
+}
+
+int rbf_symv(hipStream_t st, const double* coords, int N, int D, int kind, double p, double reg,
+             int elem_dtype, const void* x, void* y, int xy_dtype, void* work, size_t work_bytes) {
+  MPDOK_TRY(rbf_check(coords, coords, D, kind, p));
+  MPDOK_REQUIRE(x && y, "rbf_symv: null vector");
+  MPDOK_REQUIRE(N >= 0, "rbf_symv: N < 0");
+  MPDOK_REQUIRE(elem_dtype == F64 || elem_dtype == F32, "rbf_symv: element dtype %d", elem_dtype);
+  MPDOK_REQUIRE(xy_dtype == F64 || xy_dtype == F32, "rbf_symv: vector dtype %d", xy_dtype);
+  if (N == 0) return OK;
+  MPDOK_REQUIRE(work && work_bytes >= rbf_symv_workspace_bytes(N),
+                "rbf_symv: workspace of %zu bytes, %zu needed", work_bytes,
+                rbf_symv_workspace_bytes(N));
+  const int T = (N + 255) / 256;
+  const int64_t pairs = (int64_t)T * (T + 1) / 2;
+  MPDOK_REQUIRE(pairs <= 0x7fffffff, "rbf_symv: N too large");
+  double* P = static_cast<double*>(work);
+#define LAUNCH_T(E, DT, TX)                                                               \
+  rbf_symv_kernel<E, DT, TX><<<(unsigned)pairs, 256, 0, st>>>(coords, N, D, kind, p, reg, \
+                                                              (const TX*)x, P, T)
+#define LAUNCH(DT)                                       \
+  if (elem_dtype == F64) {                               \
+    if (xy_dtype == F64) LAUNCH_T(F64, DT, double);      \
+    else LAUNCH_T(F64, DT, float);                       \
+  } else {                                               \
+    if (xy_dtype == F64) LAUNCH_T(F32, DT, double);      \
+    else LAUNCH_T(F32, DT, float);                       \
+  }
+  MPDOK_RBF_DIM_DISPATCH(D, LAUNCH);
+#undef LAUNCH
+#undef LAUNCH_T
+  if (xy_dtype == F64)
+    rbf_symv_reduce_kernel<double><<<T, 256, 0, st>>>(P, N, T, (double*)y);
+  else
+    rbf_symv_reduce_kernel<float><<<T, 256, 0, st>>>(P, N, T, (float*)y);
+  MPDOK_HIP(hipGetLastError());
+  return OK;
+}
+
 size_t rbf_dbilinear_workspace_bytes(int NR) { return (size_t)((NR + 31) / 32 + 1) * 8; }
 
 int rbf_dbilinear(hipStream_t st, const double* R, int NR, const double* C, int NC, int D,
@@ -517,6 +661,13 @@ MPDOK_API int mpdok_rbf_matvec(hipStream_t st, const double* coords, int N, int 
                                double p, double reg, int elem_dtype, const void* x, void* y,
                                int xy_dtype, int row0, int rows) {
   return mpdok::rbf_matvec(st, coords, N, D, kind, p, reg, elem_dtype, x, y, xy_dtype, row0, rows);
+}
+MPDOK_API size_t mpdok_rbf_symv_workspace_bytes(int N) { return mpdok::rbf_symv_workspace_bytes(N); }
+MPDOK_API int mpdok_rbf_symv(hipStream_t st, const double* coords, int N, int D, int kind,
+                             double p, double reg, int elem_dtype, const void* x, void* y,
+                             int xy_dtype, void* work, size_t work_bytes) {
+  return mpdok::rbf_symv(st, coords, N, D, kind, p, reg, elem_dtype, x, y, xy_dtype, work,
+                         work_bytes);
 }
 MPDOK_API int mpdok_rbf_cross_matvec(hipStream_t st, const double* R, int NR, const double* C,
                                      int NC, int D, int kind, double p, const double* x,

@@ -17,13 +17,14 @@ What runs where:
                     tile t (mpdok_stream_gemv_f32). The on-disk store is read by
                     a background thread into two page-locked buffers.
   _tiled_dgemv_fp64 matrix-free FP64 product, kernel values recomputed in
-                    registers (mpdok_rbf_matvec, FP64 elements).
+                    registers, each once for both symmetric entries
+                    (mpdok_rbf_symv, FP64 elements).
   _inner_gmres      FP32 GMRES with two classical Gram-Schmidt passes (DGKS) and
                     Givens rotations on the device; stop at |g_k+1|/beta < 1e-6.
 
 Extension (not in the reference): ``store='free'`` keeps nothing but the
 coordinates and recomputes the FP32 tile values inside the inner product as
-well (mpdok_rbf_matvec, FP32 elements: bit for bit the stored tile's values),
+well (mpdok_rbf_symv, FP32 elements: bit for bit the stored tile's values),
 ``kernel='matern32'`` with ``length_scale`` selects the Matern-3/2 kernel, and
 ``solve(..., inner='cg', precond='jacobi')`` refines with a Jacobi-preconditioned
 FP32 CG as the inner solver (SPD kernels; the FP64 residual, update and stopping
@@ -67,7 +68,7 @@ class MPDOKOOCSolver:
         # The reference keeps the squared norms of the coordinates here (its
         # "bring your own matrix" recipe sets them). The fused kernels here
         # recompute them per tile, so the attribute is accepted and unused.
-        self._sq = None
+        self._symv_ws = None
         self._kind = GAUSSIAN
         self._pinned = None      # torch tensor owning the page-locked _ram_buf
         self._dev_tiles = None   # device workspace: two tiles
@@ -256,15 +257,35 @@ class MPDOKOOCSolver:
         torch.cuda.current_stream().synchronize()
         return y
 
+    def _free_product(self, v, elem):
+        """y = E v (v's dtype) with E's values recomputed from the coordinates
+        (elem F64: FP64 values, nugget in FP64; F32: the stored FP32 tile's
+        values). Each value is evaluated once for E_ij and E_ji, with a device
+        workspace of N^2 / 32 bytes kept between calls; beyond an eighth of
+        the device's memory, every value is evaluated per row instead."""
+        N = self.N
+        y = torch.empty(N, dtype=v.dtype, device=v.device)
+        c = self._coords_dev()
+        xy = F64 if v.dtype == torch.float64 else F32
+        nbytes = lib().mpdok_rbf_symv_workspace_bytes(N)
+        cap = torch.cuda.get_device_properties(v.device).total_memory // 8
+        if nbytes <= cap:
+            if self._symv_ws is None or self._symv_ws.numel() < nbytes:
+                self._symv_ws = None
+                self._symv_ws = torch.empty(nbytes, dtype=torch.uint8, device=v.device)
+            check(lib().mpdok_rbf_symv(_dev.stream(), c.data_ptr(), N, c.shape[1], self._kind,
+                                       self._param(), float(self.reg), elem, v.data_ptr(),
+                                       y.data_ptr(), xy, self._symv_ws.data_ptr(), nbytes),
+                  "rbf_symv")
+        else:
+            check(lib().mpdok_rbf_matvec(_dev.stream(), c.data_ptr(), N, c.shape[1], self._kind,
+                                         self._param(), float(self.reg), elem, v.data_ptr(),
+                                         y.data_ptr(), xy, 0, N), "rbf_matvec")
+        return y
+
     def _tiled_sgemv_free(self, v_fp32):
         """y_fp32 = A_fp32 v with the FP32 tile values recomputed on the fly."""
-        v = _dev.as_device(v_fp32, torch.float32).contiguous()
-        y = torch.empty(self.N, dtype=torch.float32, device=v.device)
-        c = self._coords_dev()
-        check(lib().mpdok_rbf_matvec(_dev.stream(), c.data_ptr(), self.N, c.shape[1], self._kind,
-                                     self._param(), float(self.reg), F32, v.data_ptr(),
-                                     y.data_ptr(), F32, 0, self.N), "rbf_matvec")
-        return y
+        return self._free_product(_dev.as_device(v_fp32, torch.float32).contiguous(), F32)
 
     def _tiled_sgemv(self, v_fp32):
         """Dispatch on the store."""
@@ -278,13 +299,7 @@ class MPDOKOOCSolver:
     def _tiled_dgemv_fp64(self, v_fp64):
         """y_fp64 = A_fp64 v with every FP64 tile value recomputed from the
         coordinates (nugget added in FP64)."""
-        v = _dev.as_device(v_fp64, torch.float64).contiguous()
-        y = torch.empty(self.N, dtype=torch.float64, device=v.device)
-        c = self._coords_dev()
-        check(lib().mpdok_rbf_matvec(_dev.stream(), c.data_ptr(), self.N, c.shape[1], self._kind,
-                                     self._param(), float(self.reg), F64, v.data_ptr(),
-                                     y.data_ptr(), F64, 0, self.N), "rbf_matvec")
-        return y
+        return self._free_product(_dev.as_device(v_fp64, torch.float64).contiguous(), F64)
 
     # ---- inner FP32 GMRES (mpdok_ooc.py:231-311) ---------------------------
     def _inner_gmres(self, rhs_fp64, restart):
@@ -414,6 +429,6 @@ class MPDOKOOCSolver:
         self._stage = None
         self._dev_tiles = None
         self._coords = None
-        self._sq = None
+        self._symv_ws = None
         if torch.cuda.is_available():
             torch.cuda.empty_cache()

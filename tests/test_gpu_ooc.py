@@ -83,6 +83,46 @@ def test_bring_your_own_matrix_recipe():
     assert np.linalg.norm(b - A64 @ x) / np.linalg.norm(b) < 1e-9
 
 
+@pytest.mark.parametrize("N,D", [(1, 3), (255, 2), (256, 3), (513, 1), (700, 3), (300, 5)])
+@pytest.mark.parametrize("kind,param", [(0, 25.0), (1, 0.1)])
+def test_symmetric_free_product_matches_row_product(N, D, kind, param):
+    """mpdok_rbf_symv (each kernel value evaluated once, used for E_ij and
+    E_ji) against mpdok_rbf_matvec (each row on its own): the same elements
+    summed in another order, within the summation bound; the same result
+    bit for bit on every call; a short workspace refused."""
+    _solver(256)
+    from mpdok_amd import _dev
+    from mpdok_amd._lib import F32, F64, check, lib
+    X = _pts(N, D, 5)
+    c = torch.as_tensor(X).cuda().contiguous()
+    E = orb.build_kernel(X, "gaussian" if kind == 0 else "matern32", param, 1e-4)
+    rng = np.random.default_rng(N)
+    nb = lib().mpdok_rbf_symv_workspace_bytes(N)
+    assert nb == ((N + 255) // 256) ** 2 * 256 * 8
+    ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    for elem in (F64, F32):
+        for dt in (torch.float64, torch.float32):
+            xy = F64 if dt == torch.float64 else F32
+            v = torch.as_tensor(rng.standard_normal(N)).to(dt).cuda()
+            ys = [torch.empty(N, dtype=dt, device="cuda") for _ in range(3)]
+            for y in ys[:2]:
+                check(lib().mpdok_rbf_symv(_dev.stream(), c.data_ptr(), N, D, kind, param, 1e-4,
+                                           elem, v.data_ptr(), y.data_ptr(), xy, ws.data_ptr(),
+                                           nb), "rbf_symv")
+            check(lib().mpdok_rbf_matvec(_dev.stream(), c.data_ptr(), N, D, kind, param, 1e-4,
+                                         elem, v.data_ptr(), ys[2].data_ptr(), xy, 0, N),
+                  "rbf_matvec")
+            a, b, r = (y.double().cpu().numpy() for y in ys)
+            assert np.array_equal(a, b)
+            # two FP64 sums of the same N products: (N + 1) u each; then the
+            # rounding of each result to the output type
+            uo = 2.0 ** -53 if dt == torch.float64 else 2.0 ** -24
+            bound = np.abs(E) @ np.abs(v.double().cpu().numpy())
+            assert np.all(np.abs(a - r) <= (2 * (N + 1) * 2.0 ** -53 + 2 * uo) * bound * 1.01)
+    assert lib().mpdok_rbf_symv(_dev.stream(), c.data_ptr(), N, D, kind, param, 1e-4, F64,
+                                v.data_ptr(), ys[0].data_ptr(), F32, ws.data_ptr(), nb - 8) != 0
+
+
 @pytest.mark.parametrize("kind,param", [("gaussian", 25.0), ("matern32", 0.1)])
 def test_matrix_free_products_match_oracle(kind, param):
     """FP64 product (mpdok_ooc.py:205-227) and the FP32-element product agree
