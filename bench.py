@@ -175,6 +175,62 @@ def gemv_profile(N, reps=10):
             "peak_GBps": PEAK_HBM_GBS, "kernels": res}
 
 
+def rbf_profile(N=32768, NF=65536, reps=5):
+    """HIP-event times of kernel-matrix assembly at N points (FP64 Gaussian and
+    Matern-3/2 principal blocks, FP32, and a block that is not principal, so
+    every entry is evaluated) against a plain fill of the same FP64 matrix,
+    and of the matrix-free FP64 products at NF points (symmetric: each value
+    once for both entries; by rows: every entry)."""
+    import torch
+    from mpdok_amd import _dev, _ops, _lib
+    L = _lib.load()
+    X = _dev.as_device(np.random.default_rng(42).uniform(0.0, 1.0, (N, 3)), torch.float64)
+    A64 = _dev.empty((N, N), torch.float64, "F")
+    A32 = _dev.empty((N, N), torch.float32, "C")
+    G = A64[:, :N - 1]
+    XF = _dev.as_device(np.random.default_rng(1).uniform(0.0, 1.0, (NF, 3)), torch.float64)
+    x = _dev.as_device(np.random.default_rng(2).standard_normal(NF), torch.float64)
+    y = torch.empty_like(x)
+    nb = L.mpdok_rbf_symv_workspace_bytes(NF)
+    ws = torch.empty(nb, dtype=torch.uint8, device=x.device)
+
+    def symv(kind, p):
+        assert L.mpdok_rbf_symv(_dev.stream(), XF.data_ptr(), NF, 3, kind, p, 1e-4, 0,
+                                x.data_ptr(), y.data_ptr(), 0, ws.data_ptr(), nb) == 0
+
+    def rows(kind, p):
+        assert L.mpdok_rbf_matvec(_dev.stream(), XF.data_ptr(), NF, 3, kind, p, 1e-4, 0,
+                                  x.data_ptr(), y.data_ptr(), 0, 0, NF) == 0
+
+    w64, w32, e = 8.0 * N * N, 4.0 * N * N, float(NF) * NF
+    cases = [("fill_f64", lambda: A64.fill_(1.0), w64, "B"),
+             ("assemble_gaussian_f64", lambda: _ops.rbf_assemble(X, 0, 25.0, 1e-6, A64), w64, "B"),
+             ("assemble_matern_f64", lambda: _ops.rbf_assemble(X, 1, 0.1, 1e-4, A64), w64, "B"),
+             ("assemble_gaussian_f32", lambda: _ops.rbf_assemble(X, 0, 25.0, 1e-6, A32), w32, "B"),
+             ("assemble_gaussian_f64_not_principal",
+              lambda: _ops.rbf_assemble(X, 0, 25.0, 1e-6, G, cols=N - 1), 8.0 * N * (N - 1), "B"),
+             ("matrix_free_matern_f64_symmetric", lambda: symv(1, 0.1), e, "entries"),
+             ("matrix_free_matern_f64_rows", lambda: rows(1, 0.1), e, "entries"),
+             ("matrix_free_gaussian_f64_symmetric", lambda: symv(0, 25.0), e, "entries"),
+             ("matrix_free_gaussian_f64_rows", lambda: rows(0, 25.0), e, "entries")]
+    res = {}
+    for name, fn, amount, unit in cases:
+        fn()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+        torch.cuda.synchronize()
+        for r in range(reps):
+            ev[2 * r].record()
+            fn()
+            ev[2 * r + 1].record()
+        torch.cuda.synchronize()
+        ms = min(ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(reps))
+        rate = amount / (ms * 1e-3)
+        res[name] = ({"ms": ms, "TBps_written": rate / 1e12, "frac_of_peak": rate / 1e9 / PEAK_HBM_GBS}
+                     if unit == "B" else {"ms": ms, "matrix_entries_per_s": rate})
+    return {"metric": "RBF assembly at N=%d and matrix-free products at N=%d" % (N, NF),
+            "kernels": res}
+
+
 def update_profile(shapes=((30720, 1024), (15360, 1024)), reps=5):
     """HIP-event times of the int8-sliced trailing-update kernel alone
     (mpdok_ozaki_apply_sliced after one slicing call) per slice count S = 2 .. 5,
@@ -246,6 +302,8 @@ def main():
     ap.add_argument("--update-profile", action="store_true",
                     help="only time the int8-sliced trailing-update kernel per slice count at "
                          "the LU's panel-pair shape and print one JSON line")
+    ap.add_argument("--rbf-profile", action="store_true",
+                    help="time RBF assembly and the matrix-free products, then exit")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip torch solve and plain GMRES")
     ap.add_argument("--configs", action="store_true",
@@ -268,6 +326,9 @@ def main():
         return 0
     if args.update_profile:
         print(json.dumps(update_profile()), flush=True)
+        return 0
+    if args.rbf_profile:
+        print(json.dumps(rbf_profile()), flush=True)
         return 0
     log(f"[bench] device {torch.cuda.get_device_name(0)}  N={N}  slices={args.slices}")
     X = np.random.default_rng(42).uniform(0.0, 1.0, (N, 3))
