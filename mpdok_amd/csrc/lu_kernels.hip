@@ -922,9 +922,40 @@ __global__ __launch_bounds__(256) void copy2d_kernel(const TI* __restrict__ src,
   for (int j = blockIdx.y; j < n; j += gridDim.y) dst[i + (int64_t)j * ldd] = (TO)src[i + (int64_t)j * lds];
 }
 
+// FP64 copy with 16-byte accesses for even m, even leading dimensions and
+// 16-byte aligned bases: each lane moves one row pair of four columns per
+// iteration, the four loads issued before the stores.
+typedef double cp_d2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void copy2d_v2_kernel(const cp_d2* __restrict__ src,
+                                                        int64_t lds2, cp_d2* __restrict__ dst,
+                                                        int64_t ldd2, int m2, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m2) return;
+  for (int j = blockIdx.y * 4; j < n; j += gridDim.y * 4) {
+    cp_d2 v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (j + c < n) v[c] = __builtin_nontemporal_load(src + i + (int64_t)(j + c) * lds2);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (j + c < n) dst[i + (int64_t)(j + c) * ldd2] = v[c];
+  }
+}
+
 template <typename TI, typename TO>
 int copy2d(hipStream_t s, const TI* src, int64_t lds, TO* dst, int64_t ldd, int m, int n) {
   if (m <= 0 || n <= 0) return OK;
+  if constexpr (std::is_same<TI, double>::value && std::is_same<TO, double>::value) {
+    if (m % 2 == 0 && lds % 2 == 0 && ldd % 2 == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0 &&
+        (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+      const int m2 = m / 2, nq = (n + 3) / 4;
+      dim3 grid((m2 + 255) / 256, nq < 4096 ? nq : 4096);
+      copy2d_v2_kernel<<<grid, 256, 0, s>>>(reinterpret_cast<const cp_d2*>(src), lds / 2,
+                                            reinterpret_cast<cp_d2*>(dst), ldd / 2, m2, n);
+      MPDOK_HIP(hipGetLastError());
+      return OK;
+    }
+  }
   dim3 grid((m + 255) / 256, n < 4096 ? n : 4096);
   copy2d_kernel<TI, TO><<<grid, 256, 0, s>>>(src, lds, dst, ldd, m, n);
   MPDOK_HIP(hipGetLastError());
